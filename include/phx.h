@@ -1,0 +1,300 @@
+/*
+ * phx.h -- C ABI of the MI355X (gfx950) ray-query engine.
+ *
+ * This is the drop-in boundary for the CUDA_TRACER backend of PhantasyEngineTestbed. Every struct
+ * below is byte-identical to the reference type it replaces, so the arrays the reference's host
+ * code builds (buildStaticBVH + sanitizeBVH) can be handed over unchanged. Every entry point cites
+ * the reference interface it replaces (paths are relative to the reference repository root).
+ *
+ * Conventions
+ *  - All functions return phx_status; PHX_OK is 0. No exceptions cross the ABI and nothing calls
+ *    exit/abort. The message of the last failure on the calling thread is phx_last_error().
+ *  - Pointers flagged PHX_MEM_DEVICE are device pointers (hipMalloc'd or phx_device_alloc'd);
+ *    otherwise they are host pointers and the call copies through a staging buffer.
+ *  - A context owns one HIP stream. Calls on one context are issued in order on that stream.
+ *    Calls without PHX_ASYNC return after the work has completed.
+ */
+#ifndef PHX_H
+#define PHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/* Status codes                                                                               */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef enum phx_status {
+	PHX_OK = 0,
+	PHX_ERR_INVALID_ARGUMENT = 1, /* null pointer, zero/oversized count, bad flag combination  */
+	PHX_ERR_INVALID_HANDLE = 2,   /* released or foreign context/scene/bvh/server handle        */
+	PHX_ERR_OUT_OF_MEMORY = 3,    /* host or device allocation failed                           */
+	PHX_ERR_MALFORMED_BVH = 4,    /* child index out of range, missing end marker, too deep ... */
+	PHX_ERR_HIP = 5,              /* a HIP runtime call failed (message holds hipGetErrorString)*/
+	PHX_ERR_NO_DEVICE = 6,        /* no HIP device, or device index out of range                */
+	PHX_ERR_BUDGET = 7,           /* served form: request can never fit the memory budget       */
+	PHX_ERR_NOT_READY = 8,        /* served form: request still in flight (phx_poll)            */
+	PHX_ERR_CANCELLED = 9,        /* served form: request was cancelled before it ran           */
+	PHX_ERR_INVALID_SCENE = 10    /* scene input unusable (no triangles, non-finite vertex ...) */
+} phx_status;
+
+/* Returns the message of the last failed call on this thread ("" if none). */
+const char* phx_last_error(void);
+
+/* Library version string, e.g. "phx 0.1.0 gfx950". */
+const char* phx_version(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Data layouts (byte-identical to the reference)                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+/* BVHNode -- source/phantasy_engine/ray_tracer_common/BVHNode.hpp:23-149 (64 B).
+ * fData = [lc.min.xyz, lc.max.x] [lc.max.yz, rc.min.xy] [rc.min.z, rc.max.xyz];
+ * iData = [lcIndex, rcIndex, lcNumTriangles, rcNumTriangles]. A child with numTriangles != 0 is
+ * a leaf whose index is the bitwise-negated (~) first triangle index; otherwise it is the index of
+ * an inner node. Node 0 is the root. */
+typedef struct PhxBVHNode {
+	float fData[12];
+	int32_t iData[4];
+} PhxBVHNode;
+
+/* TriangleVertices -- ray_tracer_common/Triangle.hpp:17-23 (48 B). v0..v2 are xyzw; w is 0 except
+ * in the last triangle of a leaf, where all three w are -1 (end marker, StaticBVHBuilder.cpp:19-45). */
+typedef struct PhxTriangleVertices {
+	float v0[4];
+	float v1[4];
+	float v2[4];
+} PhxTriangleVertices;
+
+/* TriangleData -- ray_tracer_common/Triangle.hpp:25-37 (64 B). */
+typedef struct PhxTriangleData {
+	float n0[3];
+	float n1[3];
+	float n2[3];
+	float uv0[2];
+	float uv1[2];
+	float uv2[2];
+	uint32_t materialIndex;
+} PhxTriangleData;
+
+/* RayIn -- source/cuda_ray_tracer/kernels/RayCastKernel.hpp:22-37 (32 B):
+ * (origin.xyz, minDist) (dir.xyz, maxDist). */
+typedef struct PhxRayIn {
+	float origin[3];
+	float minDist;
+	float dir[3];
+	float maxDist;
+} PhxRayIn;
+
+/* RayHit -- kernels/RayCastKernel.hpp:39-45 (16 B), filled with the CPU traversal's
+ * RayCastResult semantics (ray_tracer_common/BVHTraversal.hpp:18-27): on a miss
+ * triangleIndex = 0xFFFFFFFF and t = u = v = FLT_MAX. */
+typedef struct PhxRayHit {
+	uint32_t triangleIndex;
+	float t;
+	float u;
+	float v;
+} PhxRayHit;
+
+/* Material -- source/phantasy_engine/rendering/Material.hpp:17-55 (48 B):
+ * iData = [albedoTex, roughnessTex, metallicTex, pad] (-1 = none), albedo rgba,
+ * fData2 = [roughness, metallic, pad, pad]. */
+typedef struct PhxMaterial {
+	int32_t iData[4];
+	float albedo[4];
+	float fData2[4];
+} PhxMaterial;
+
+/* SphereLight -- source/phantasy_engine/level/SphereLight.hpp:13-26 (36 B with padding). */
+typedef struct PhxSphereLight {
+	float pos[3];
+	float radius;
+	float range;
+	float strength[3];
+	uint8_t staticShadows;
+	uint8_t dynamicShadows;
+	uint8_t pad[2];
+} PhxSphereLight;
+
+/* CameraDef -- ray_tracer_common/GenerateRays.hpp:16-25: ray = normalize(dir + cx*dX + cy*dY),
+ * (cx, cy) = loc / res * 2 - 1 (pixel corner, no half-pixel offset). */
+typedef struct PhxCameraDef {
+	float origin[3];
+	float dir[3];
+	float dX[3];
+	float dY[3];
+} PhxCameraDef;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Flags                                                                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+#define PHX_MEM_DEVICE 0x1u /* ray/hit/image pointers are device pointers                    */
+#define PHX_ASYNC 0x2u      /* return after enqueueing on the context stream (device ptrs only)*/
+
+/* Traversal algorithm of phx_cast_closest (the any-hit result does not depend on it). */
+#define PHX_TRAVERSAL_REFERENCE_ORDER 0x00u /* the CPU castRay visiting order: bit-identical hits */
+#define PHX_TRAVERSAL_NEAR_FIRST 0x10u      /* nearer child first; ties broken by castRay order  */
+#define PHX_TRAVERSAL_MASK 0xF0u
+
+/* Ray order of phx_gen_primary_rays. */
+#define PHX_RAYS_ROW_MAJOR 0u  /* id = y * w + x (CPUTracerRenderer.cpp:84-97)                   */
+#define PHX_RAYS_TILED_8X8 1u  /* 8x8-tile order of getRayIdx (CudaDeviceHelpers.cuh:49-62)       */
+
+/* ------------------------------------------------------------------------------------------ */
+/* Host-side SBVH build (replaces buildStaticFrom + sanitizeBVH,                              */
+/* ray_tracer_common/StaticBVHBuilder.hpp:10-21, StaticBVHBuilder.cpp:213-260)                */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct PhxBVH PhxBVH;
+
+/* Builds the split BVH of `numTriangles` triangles exactly as buildStaticFrom + sanitizeBVH do
+ * (Stich 2009 SBVH with the reference's parameters: splitAlpha 1e-5, unit SAH costs, min leaf 1,
+ * 32 spatial bins, MaxDepth 64, MaxSpatialDepth 48). `datas` may be NULL. `numThreads` = 0 uses
+ * all host cores; the output does not depend on it. Non-finite vertices -> PHX_ERR_INVALID_SCENE. */
+phx_status phx_bvh_build(const PhxTriangleVertices* triangles, const PhxTriangleData* datas,
+                         uint32_t numTriangles, uint32_t numThreads, PhxBVH** outBvh);
+
+/* Read-only views of the built arrays (valid until phx_bvh_release). */
+phx_status phx_bvh_arrays(const PhxBVH* bvh, const PhxBVHNode** nodes, uint32_t* numNodes,
+                          const PhxTriangleVertices** triangles, const PhxTriangleData** datas,
+                          uint32_t* numTriangles);
+
+void phx_bvh_release(PhxBVH* bvh);
+
+/* Checks a sanitized BVH: child indices in range, leaf ranges in range and terminated by exactly
+ * one end marker, every inner node reachable once, depth <= 64. Writes the depth (root = 1). */
+phx_status phx_bvh_validate(const PhxBVHNode* nodes, uint32_t numNodes,
+                            const PhxTriangleVertices* triangles, uint32_t numTriangles,
+                            uint32_t* outDepth);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Context, device memory                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct PhxContext PhxContext;
+
+/* Replaces the CudaTracerRenderer constructor's device setup (CudaTracerRenderer.cpp:123-167):
+ * binds `device`, reads its properties, creates the context stream. No device reset on destroy. */
+phx_status phx_create(int device, PhxContext** outCtx);
+void phx_destroy(PhxContext* ctx);
+
+/* The context's hipStream_t, as an opaque pointer (for events / interop). */
+void* phx_context_stream(PhxContext* ctx);
+
+/* Number of compute units of the context's device. */
+uint32_t phx_context_num_cus(PhxContext* ctx);
+
+/* Device buffers (CudaBuffer.hpp:93-108 equivalents). Allocations are 256-B aligned. */
+phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr);
+phx_status phx_device_free(PhxContext* ctx, void* ptr);
+phx_status phx_copy_to_device(PhxContext* ctx, void* dst, const void* src, size_t bytes);
+phx_status phx_copy_to_host(PhxContext* ctx, void* dst, const void* src, size_t bytes);
+phx_status phx_synchronize(PhxContext* ctx);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Static scene upload (replaces CudaTracerRenderer::setStaticScene, CudaTracerRenderer.cpp:251-   */
+/* 270, and setMaterialsAndTextures' material upload, :192-239)                                */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct PhxScene PhxScene;
+
+/* Copies the three SBVH arrays (+ optional materials and lights) into device memory owned by the
+ * returned scene. The BVH is validated first (PHX_ERR_MALFORMED_BVH). `datas`, `materials` and
+ * `lights` may be NULL with a zero count. The scene stays resident until phx_scene_release. */
+phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t numNodes,
+                            const PhxTriangleVertices* triangles, const PhxTriangleData* datas,
+                            uint32_t numTriangles, const PhxMaterial* materials,
+                            uint32_t numMaterials, const PhxSphereLight* lights,
+                            uint32_t numLights, PhxScene** outScene);
+void phx_scene_release(PhxScene* scene);
+
+/* Device bytes held by a scene. */
+size_t phx_scene_device_bytes(const PhxScene* scene);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Cast rays                                                                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Closest hit. Replaces launchRayCastKernel / launchRayCastNoPersistenceKernel
+ * (kernels/RayCastKernel.hpp:57-61, RayCastKernel.cu:806-851). Hit/t/u/v follow the CPU castRay
+ * (BVHTraversal.hpp:37-128) with tMin = ray.minDist, tMax = ray.maxDist: a triangle is accepted
+ * iff t < closest.t && tMin <= t <= tMax; a child box is entered iff it is hit, tOut > tMin and
+ * tIn < min(tMax, closest.t). `flags` = PHX_MEM_DEVICE | PHX_ASYNC | PHX_TRAVERSAL_*. */
+phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                            uint32_t numRays, PhxRayHit* hits, uint32_t flags);
+
+/* Any hit (shadow rays). Replaces launchShadowRayCastKernel (RayCastKernel.hpp:63-64,
+ * RayCastKernel.cu:577-801, 853-886): inLight[i] = 1 iff no triangle has tMin <= t < tMax, where
+ * boxes are culled with the castRay box predicate (hit, tOut > tMin, tIn < tMax). One byte/ray. */
+phx_status phx_cast_anyhit(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                           uint32_t numRays, uint8_t* inLight, uint32_t flags);
+
+/* castDebugRay's counters (BVHTraversal.hpp:29-35, 130-235) of phx_cast_closest on the same rays,
+ * summed over the batch, computed on the GPU by a counting variant of the traversal kernel:
+ * counters = {nodesVisited, trianglesIntersected, triangleIntersectionTests,
+ * aabbIntersectionTests, aabbIntersections}. Synchronous; hits are discarded. */
+phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                               uint32_t numRays, uint32_t flags, uint64_t counters[5]);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Ray generation                                                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+/* generateCameraDef (ray_tracer_common/GenerateRays.cpp:12-35) on the host. */
+phx_status phx_camera_def(const float pos[3], const float dir[3], const float up[3],
+                          float vertFovRad, float resX, float resY, PhxCameraDef* outCam);
+
+/* ViewFrustum::set's orthogonalisation (rendering/ViewFrustum.cpp:126-131: dir = normalize(target -
+ * pos), up = normalize(up - dot(up, dir) * dir)) followed by generateCameraDef. */
+phx_status phx_camera_look_at(const float pos[3], const float target[3], const float up[3],
+                              float vertFovRad, float resX, float resY, PhxCameraDef* outCam);
+
+/* Pinhole primary rays (genPrimaryRaysKernel, RayCastKernel.cu:891-919; CPUTracerRenderer.cpp:
+ * 93-97) with minDist = tMin, maxDist = tMax, in PHX_RAYS_ROW_MAJOR or PHX_RAYS_TILED_8X8 order
+ * (the tiled order needs w and h divisible by 8). `rays` holds w*h records. */
+phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32_t width,
+                                uint32_t height, float tMin, float tMax, uint32_t order,
+                                PhxRayIn* rays, uint32_t flags);
+
+/* One cosine-weighted diffuse bounce per ray (createSecondaryRaysKernel's Lambertian branch,
+ * kernels/MaterialKernel.cu:239-312): origin = o + t*d, direction sampleLambertian around the
+ * interpolated TriangleData normal turned to face the incoming ray, minDist 0.001, maxDist
+ * FLT_MAX. Rays that missed (triangleIndex 0xFFFFFFFF) get the reference's dummy ray
+ * (MaterialKernel.cu:41-47). Random numbers: PCG32 stream `i` seeded from (seed, frame). The
+ * scene must have TriangleData. */
+phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                               const PhxRayHit* hits, uint32_t numRays, uint64_t seed,
+                               uint32_t frame, PhxRayIn* outRays, uint32_t flags);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Synthetic scenes of the benchmark workloads (host)                                         */
+/* ------------------------------------------------------------------------------------------ */
+
+/* `n` random triangles: centroid ~ U[0,1]^3, edge lengths ~ U[eMin, eMax] (resampled until they
+ * form a triangle), planar triangle by the law of cosines centred on its centroid, rotated by a
+ * uniform random quaternion. Writes n triangles (w = 0) and, if datas != NULL, their data
+ * (geometric normal in n0..n2, zero uvs, material 0). */
+phx_status phx_scene_random_triangles(uint32_t n, uint64_t seed, float eMin, float eMax,
+                                      PhxTriangleVertices* triangles, PhxTriangleData* datas);
+
+/* Triangle count of phx_scene_heightfield_boxes. */
+uint32_t phx_scene_heightfield_boxes_count(uint32_t grid, uint32_t numBoxes);
+
+/* grid x grid heightfield on [0,1]^2 (x, z), 2 triangles per cell, height = amp * fBm(value
+ * noise, 4 octaves, lacunarity 2, gain 0.5), followed by `numBoxes` axis-aligned boxes (12
+ * triangles each) resting on the surface with lognormal(mu, sigma) edge lengths per axis.
+ * If numMaterials > 0 each triangle gets materialIndex ~ U{0..numMaterials-1}. */
+phx_status phx_scene_heightfield_boxes(uint32_t grid, uint32_t numBoxes, uint64_t seed, float amp,
+                                       float boxMu, float boxSigma, uint32_t numMaterials,
+                                       PhxTriangleVertices* triangles, PhxTriangleData* datas);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* PHX_H */

@@ -1,0 +1,97 @@
+/*
+ * oracle.h -- CPU restatement of the reference's ray-query path. TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library,
+ * and only as the checker (or, in the bench, as the timed CPU baseline). The product (libphx)
+ * never links or calls it.
+ *
+ * Each function names the reference code it restates (paths relative to the reference root).
+ * Parity pinning (what checks each function against something other than itself):
+ *   oracle_cast_closest   - golden fixtures made by the reference's own castRay<144> compiled
+ *                           from its sources (tests/golden), brute force on tiny scenes,
+ *                           hand-built closed-form cases.
+ *   oracle_cast_debug     - the same fixtures (counters of castDebugRay).
+ *   oracle_cast_anyhit    - consistency with closest hit (occluded <=> some hit with
+ *                           tMin <= t < tMax) and brute force on tiny scenes.
+ *   oracle_brute_closest  - closed-form rays against hand-built triangles.
+ *   oracle_primary_rays   - the reference's CPU renderer formula compiled from its sources
+ *                           (golden fixture) and closed form at the image centre/corners.
+ * The structs are the byte layouts of include/phx.h, restated so the oracle stays standalone.
+ */
+#ifndef PHX_ORACLE_H
+#define PHX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OrNode {
+	float f[12];
+	int32_t i[4];
+} OrNode;
+
+typedef struct OrTri {
+	float v[3][4];
+} OrTri;
+
+typedef struct OrRay {
+	float o[3];
+	float tMin;
+	float d[3];
+	float tMax;
+} OrRay;
+
+typedef struct OrHit {
+	uint32_t tri;
+	float t, u, v;
+} OrHit;
+
+/* castDebugRay's DebugRayCastData (BVHTraversal.hpp:29-35). */
+typedef struct OrCounters {
+	uint64_t nodesVisited;
+	uint64_t trianglesIntersected;
+	uint64_t triangleIntersectionTests;
+	uint64_t aabbIntersectionTests;
+	uint64_t aabbIntersections;
+} OrCounters;
+
+/* castRay<144> (BVHTraversal.hpp:37-128) per ray with tMin = ray.tMin, tMax = ray.tMax.
+ * numThreads = 0 uses all cores; rays are handed out in chunks from an atomic counter. */
+void oracle_cast_closest(const OrNode* nodes, const OrTri* tris, const OrRay* rays, uint32_t n,
+                         OrHit* hits, uint32_t numThreads);
+
+/* castDebugRay (BVHTraversal.hpp:130-235): same hits, plus counters summed over all rays and,
+ * if perRay != NULL, per ray (5 x uint32 per ray in OrCounters field order). */
+void oracle_cast_debug(const OrNode* nodes, const OrTri* tris, const OrRay* rays, uint32_t n,
+                       OrHit* hits, OrCounters* total, uint32_t* perRay, uint32_t numThreads);
+
+/* Any hit: castRay's traversal and box predicate with tCurrMax = tMax; occluded iff a triangle
+ * has tMin <= t < tMax (RayCastKernel.cu:759-765). Writes inLight = !occluded. */
+void oracle_cast_anyhit(const OrNode* nodes, const OrTri* tris, const OrRay* rays, uint32_t n,
+                        uint8_t* inLight, uint32_t numThreads);
+
+/* Every triangle tested with Intersection.hpp:24-67 in index order; accepts t < best.t &&
+ * tMin <= t <= tMax (the first index wins ties). No BVH. */
+void oracle_brute_closest(const OrTri* tris, uint32_t numTris, const OrRay* rays, uint32_t n,
+                          OrHit* hits);
+
+/* Pinhole rays, CPUTracerRenderer.cpp:93-97 with CameraDef (origin, dir, dX, dY) = cam[0..11];
+ * order 0 = row major, 1 = 8x8 tiles (CudaDeviceHelpers.cuh:49-62). */
+void oracle_primary_rays(const float cam[12], uint32_t w, uint32_t h, float tMin, float tMax,
+                         uint32_t order, OrRay* rays);
+
+/* Intersection.hpp:24-67 on one triangle; returns hit flag. */
+int oracle_triangle(const float p0[3], const float p1[3], const float p2[3], const float o[3],
+                    const float d[3], float* t, float* u, float* v);
+
+/* Intersection.hpp:77-90 on one box with invDir = 1/d; returns hit flag. */
+int oracle_aabb(const float o[3], const float d[3], const float mn[3], const float mx[3],
+                float* tIn, float* tOut);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

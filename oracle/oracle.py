@@ -1,0 +1,182 @@
+"""ctypes binding of the oracle (liboracle.so) and of the reference's own CPU code compiled from
+its sources (_ref/libphxref.so). TEST INFRASTRUCTURE ONLY: imported by tests/, by
+__graft_entry__.smoke() as the checker and by bench.py's cpu_baseline leg.
+
+Arrays use the numpy dtypes of phantasyenginetestbed_amd.phx (same byte layouts).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libphxref.so"
+REF_FASTMATH_LIB = HERE / "_ref" / "libphxref_fastmath.so"
+REFERENCE_ROOT = Path("/root/reference")
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+
+
+class Counters(C.Structure):
+    _fields_ = [("nodesVisited", C.c_uint64), ("trianglesIntersected", C.c_uint64),
+                ("triangleIntersectionTests", C.c_uint64), ("aabbIntersectionTests", C.c_uint64),
+                ("aabbIntersections", C.c_uint64)]
+
+
+def build(reference: bool | None = None) -> None:
+    """Compiles liboracle.so, and _ref/libphxref.so when the reference tree is present."""
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE), "liboracle.so"], check=True)
+    if reference is None:
+        reference = REFERENCE_ROOT.exists()
+    if reference:
+        subprocess.run(["make", "-s", "-B", "-C", str(HERE), "ref", "asship"], check=True)
+
+
+_olib = None
+_rlib = {}
+
+
+def olib() -> C.CDLL:
+    global _olib
+    if _olib is None:
+        if not ORACLE_LIB.exists():
+            build(reference=False)
+        L = C.CDLL(str(ORACLE_LIB))
+        L.oracle_cast_closest.argtypes = [_P, _P, _P, _U32, _P, _U32]
+        L.oracle_cast_debug.argtypes = [_P, _P, _P, _U32, _P, C.POINTER(Counters), _P, _U32]
+        L.oracle_cast_anyhit.argtypes = [_P, _P, _P, _U32, _P, _U32]
+        L.oracle_brute_closest.argtypes = [_P, _U32, _P, _U32, _P]
+        L.oracle_primary_rays.argtypes = [_P, _U32, _U32, C.c_float, C.c_float, _U32, _P]
+        L.oracle_triangle.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
+        L.oracle_triangle.restype = C.c_int
+        L.oracle_aabb.argtypes = [_P, _P, _P, _P, _P, _P]
+        L.oracle_aabb.restype = C.c_int
+        _olib = L
+    return _olib
+
+
+def have_ref(fastmath: bool = False) -> bool:
+    return (REF_FASTMATH_LIB if fastmath else REF_LIB).exists()
+
+
+def rlib(fastmath: bool = False) -> C.CDLL:
+    if fastmath not in _rlib:
+        path = REF_FASTMATH_LIB if fastmath else REF_LIB
+        if not path.exists():
+            raise RuntimeError(f"{path} not built (needs the reference tree)")
+        L = C.CDLL(str(path))
+        L.phxref_build.argtypes = [_P, _P, _U32, C.POINTER(_U32), C.POINTER(_U32)]
+        L.phxref_build.restype = _P
+        L.phxref_get.argtypes = [_P, _P, _P, _P]
+        L.phxref_free.argtypes = [_P]
+        L.phxref_cast.argtypes = [_P, _P, _P, _U32, _P, _U32]
+        L.phxref_cast_debug.argtypes = [_P, _P, _P, _U32, _P, _P]
+        L.phxref_primary_rays.argtypes = [_P, _P, _P, C.c_float, _U32, _U32, C.c_float,
+                                          C.c_float, _P, _P]
+        _rlib[fastmath] = L
+    return _rlib[fastmath]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---- oracle (C restatement) -----------------------------------------------------------------
+
+def cast_closest(nodes, tris, rays, threads: int = 0):
+    from phantasyenginetestbed_amd.phx import HIT
+    hits = np.empty(len(rays), HIT)
+    olib().oracle_cast_closest(_p(nodes), _p(tris), _p(rays), len(rays), _p(hits), threads)
+    return hits
+
+
+def cast_debug(nodes, tris, rays, threads: int = 0, per_ray: bool = False):
+    from phantasyenginetestbed_amd.phx import HIT
+    hits = np.empty(len(rays), HIT)
+    tot = Counters()
+    pr = np.empty((len(rays), 5), np.uint32) if per_ray else None
+    olib().oracle_cast_debug(_p(nodes), _p(tris), _p(rays), len(rays), _p(hits), C.byref(tot),
+                             _p(pr), threads)
+    totals = {f: getattr(tot, f) for f, _ in Counters._fields_}
+    return hits, totals, pr
+
+
+def cast_anyhit(nodes, tris, rays, threads: int = 0):
+    out = np.empty(len(rays), np.uint8)
+    olib().oracle_cast_anyhit(_p(nodes), _p(tris), _p(rays), len(rays), _p(out), threads)
+    return out
+
+
+def brute_closest(tris, rays):
+    from phantasyenginetestbed_amd.phx import HIT
+    hits = np.empty(len(rays), HIT)
+    olib().oracle_brute_closest(_p(tris), len(tris), _p(rays), len(rays), _p(hits))
+    return hits
+
+
+def primary_rays(cam12, w, h, t_min=1e-4, t_max=None, order=0):
+    from phantasyenginetestbed_amd.phx import FLT_MAX, RAY
+    cam = np.ascontiguousarray(cam12, np.float32)
+    rays = np.empty(w * h, RAY)
+    olib().oracle_primary_rays(_p(cam), w, h, t_min, FLT_MAX if t_max is None else t_max,
+                               order, _p(rays))
+    return rays
+
+
+def triangle(p0, p1, p2, o, d):
+    a = [np.ascontiguousarray(x, np.float32) for x in (p0, p1, p2, o, d)]
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = olib().oracle_triangle(*[_p(x) for x in a], C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+def aabb(o, d, mn, mx):
+    a = [np.ascontiguousarray(x, np.float32) for x in (o, d, mn, mx)]
+    ti, to = C.c_float(), C.c_float()
+    hit = olib().oracle_aabb(*[_p(x) for x in a], C.byref(ti), C.byref(to))
+    return bool(hit), ti.value, to.value
+
+
+# ---- the reference's own code ---------------------------------------------------------------
+
+def ref_build(tris, datas=None, fastmath: bool = False):
+    from phantasyenginetestbed_amd.phx import NODE, TRI, TRIDATA
+    L = rlib(fastmath)
+    nn, nt = C.c_uint32(), C.c_uint32()
+    h = L.phxref_build(_p(tris), _p(datas), len(tris), C.byref(nn), C.byref(nt))
+    nodes = np.empty(nn.value, NODE)
+    out_tris = np.empty(nt.value, TRI)
+    out_datas = np.empty(nt.value, TRIDATA)
+    L.phxref_get(h, _p(nodes), _p(out_tris), _p(out_datas))
+    L.phxref_free(h)
+    return nodes, out_tris, out_datas
+
+
+def ref_cast(nodes, tris, rays, threads: int = 0, fastmath: bool = False):
+    from phantasyenginetestbed_amd.phx import HIT
+    hits = np.empty(len(rays), HIT)
+    rlib(fastmath).phxref_cast(_p(nodes), _p(tris), _p(rays), len(rays), _p(hits), threads)
+    return hits
+
+
+def ref_cast_debug(nodes, tris, rays):
+    from phantasyenginetestbed_amd.phx import HIT
+    hits = np.empty(len(rays), HIT)
+    cnt = np.empty((len(rays), 5), np.uint32)
+    rlib().phxref_cast_debug(_p(nodes), _p(tris), _p(rays), len(rays), _p(hits), _p(cnt))
+    return hits, cnt
+
+
+def ref_primary_rays(pos, dir, up, fov_rad, w, h, t_min, t_max):
+    from phantasyenginetestbed_amd.phx import RAY
+    rays = np.empty(w * h, RAY)
+    cam = np.empty(12, np.float32)
+    f3 = [np.ascontiguousarray(x, np.float32) for x in (pos, dir, up)]
+    rlib().phxref_primary_rays(*[_p(x) for x in f3], fov_rad, w, h, t_min, t_max, _p(rays),
+                               _p(cam))
+    return rays, cam

@@ -1,0 +1,570 @@
+// The C ABI (include/phx.h): contexts, device memory, scene upload, cast-rays and ray generation.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <unordered_set>
+
+#include "kernels.hpp"
+#include "phx_internal.hpp"
+
+namespace phx {
+
+namespace {
+thread_local std::string tLastError;
+}
+
+phx_status fail(phx_status status, const char* fmt, ...)
+{
+	char buf[1024];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	tLastError = buf;
+	return status;
+}
+
+phx_status ok()
+{
+	tLastError.clear();
+	return PHX_OK;
+}
+
+namespace {
+
+// Live handles, so a released or foreign pointer is reported instead of dereferenced.
+struct Registry {
+	std::mutex mu;
+	std::unordered_set<const void*> live;
+	void add(const void* p)
+	{
+		std::lock_guard<std::mutex> g(mu);
+		live.insert(p);
+	}
+	bool remove(const void* p)
+	{
+		std::lock_guard<std::mutex> g(mu);
+		return live.erase(p) != 0;
+	}
+	bool has(const void* p)
+	{
+		std::lock_guard<std::mutex> g(mu);
+		return live.count(p) != 0;
+	}
+};
+
+Registry& registry()
+{
+	static Registry r;
+	return r;
+}
+
+phx_status hipFail(hipError_t e, const char* what)
+{
+	return fail(PHX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define PHX_HIP(call, what)                        \
+	do {                                           \
+		hipError_t e_ = (call);                    \
+		if (e_ != hipSuccess) return hipFail(e_, what); \
+	} while (0)
+
+}  // namespace
+
+}  // namespace phx
+
+using namespace phx;
+
+struct PhxBVH {
+	BuiltBVH bvh;
+};
+
+struct PhxContext {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	uint32_t numCUs = 0;
+	// Staging buffers for calls that pass host pointers, grown on demand.
+	void* stageIn = nullptr;
+	size_t stageInBytes = 0;
+	void* stageOut = nullptr;
+	size_t stageOutBytes = 0;
+};
+
+struct PhxScene {
+	PhxContext* ctx = nullptr;
+	PhxBVHNode* nodes = nullptr;
+	PhxTriangleVertices* tris = nullptr;
+	PhxTriangleData* datas = nullptr;
+	PhxMaterial* materials = nullptr;
+	PhxSphereLight* lights = nullptr;
+	uint32_t numNodes = 0, numTris = 0, numMaterials = 0, numLights = 0;
+	uint32_t depth = 0;
+	size_t bytes = 0;
+};
+
+namespace {
+
+phx_status checkCtx(PhxContext* ctx)
+{
+	if (!ctx || !registry().has(ctx)) return fail(PHX_ERR_INVALID_HANDLE, "invalid context handle");
+	hipError_t e = hipSetDevice(ctx->device);
+	if (e != hipSuccess) return fail(PHX_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+	return PHX_OK;
+}
+
+phx_status checkScene(PhxContext* ctx, const PhxScene* scene)
+{
+	if (!scene || !registry().has(scene)) return fail(PHX_ERR_INVALID_HANDLE, "invalid scene handle");
+	if (scene->ctx != ctx) return fail(PHX_ERR_INVALID_HANDLE, "scene belongs to another context");
+	return PHX_OK;
+}
+
+phx_status growStage(void*& buf, size_t& have, size_t need)
+{
+	if (have >= need) return PHX_OK;
+	if (buf) hipFree(buf);
+	buf = nullptr;
+	have = 0;
+	size_t sz = std::max<size_t>(need, 1u << 20);
+	hipError_t e = hipMalloc(&buf, sz);
+	if (e != hipSuccess) {
+		buf = nullptr;
+		return fail(PHX_ERR_OUT_OF_MEMORY, "staging allocation of %zu bytes failed", sz);
+	}
+	have = sz;
+	return PHX_OK;
+}
+
+template <class T>
+phx_status uploadArray(PhxContext* ctx, T*& dst, const T* src, size_t count, size_t& total)
+{
+	dst = nullptr;
+	if (count == 0) return PHX_OK;
+	size_t bytes = count * sizeof(T);
+	hipError_t e = hipMalloc(reinterpret_cast<void**>(&dst), bytes);
+	if (e != hipSuccess) {
+		dst = nullptr;
+		return fail(PHX_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes failed", bytes);
+	}
+	total += bytes;
+	e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+	if (e != hipSuccess) return hipFail(e, "hipMemcpyAsync(scene upload)");
+	return PHX_OK;
+}
+
+void freeScene(PhxScene* s)
+{
+	hipFree(s->nodes);
+	hipFree(s->tris);
+	hipFree(s->datas);
+	hipFree(s->materials);
+	hipFree(s->lights);
+	delete s;
+}
+
+constexpr uint32_t kMaxSceneElements = 1u << 28;
+
+// Runs `launch(raysDev, outDev)` with host/device handling of the ray and output arrays.
+template <class Launch>
+phx_status castCommon(PhxContext* ctx, const PhxRayIn* rays, uint32_t n, void* out,
+                      size_t outBytesPerRay, uint32_t flags, Launch launch, const char* what)
+{
+	const bool device = (flags & PHX_MEM_DEVICE) != 0;
+	const bool async = (flags & PHX_ASYNC) != 0;
+	if (async && !device)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "%s: PHX_ASYNC needs PHX_MEM_DEVICE", what);
+	if (n == 0) return ok();
+	if (!rays || !out) return fail(PHX_ERR_INVALID_ARGUMENT, "%s: null ray/output pointer", what);
+	if (device) {
+		PHX_HIP(launch(rays, out), what);
+		if (!async) PHX_HIP(hipStreamSynchronize(ctx->stream), what);
+		return ok();
+	}
+	size_t inBytes = size_t(n) * sizeof(PhxRayIn);
+	size_t outBytes = size_t(n) * outBytesPerRay;
+	phx_status st = growStage(ctx->stageIn, ctx->stageInBytes, inBytes);
+	if (st != PHX_OK) return st;
+	st = growStage(ctx->stageOut, ctx->stageOutBytes, outBytes);
+	if (st != PHX_OK) return st;
+	PHX_HIP(hipMemcpyAsync(ctx->stageIn, rays, inBytes, hipMemcpyHostToDevice, ctx->stream), what);
+	PHX_HIP(launch(static_cast<const PhxRayIn*>(ctx->stageIn), ctx->stageOut), what);
+	PHX_HIP(hipMemcpyAsync(out, ctx->stageOut, outBytes, hipMemcpyDeviceToHost, ctx->stream), what);
+	PHX_HIP(hipStreamSynchronize(ctx->stream), what);
+	return ok();
+}
+
+SceneView view(const PhxScene* s) { return SceneView{s->nodes, s->tris, s->numNodes, s->numTris}; }
+
+}  // namespace
+
+extern "C" {
+
+const char* phx_last_error(void) { return tLastError.c_str(); }
+
+const char* phx_version(void) { return "phx 0.1.0 gfx950"; }
+
+phx_status phx_bvh_build(const PhxTriangleVertices* triangles, const PhxTriangleData* datas,
+                         uint32_t numTriangles, uint32_t numThreads, PhxBVH** outBvh)
+{
+	if (!outBvh) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_bvh_build: outBvh is NULL");
+	*outBvh = nullptr;
+	PhxBVH* b = new (std::nothrow) PhxBVH();
+	if (!b) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_bvh_build: out of host memory");
+	phx_status st;
+	try {
+		st = buildSBVH(triangles, datas, numTriangles, numThreads, b->bvh);
+	} catch (const std::bad_alloc&) {
+		st = fail(PHX_ERR_OUT_OF_MEMORY, "phx_bvh_build: out of host memory");
+	}
+	if (st != PHX_OK) {
+		delete b;
+		return st;
+	}
+	registry().add(b);
+	*outBvh = b;
+	return ok();
+}
+
+phx_status phx_bvh_arrays(const PhxBVH* bvh, const PhxBVHNode** nodes, uint32_t* numNodes,
+                          const PhxTriangleVertices** triangles, const PhxTriangleData** datas,
+                          uint32_t* numTriangles)
+{
+	if (!bvh || !registry().has(bvh)) return fail(PHX_ERR_INVALID_HANDLE, "invalid BVH handle");
+	if (nodes) *nodes = bvh->bvh.nodes.data();
+	if (numNodes) *numNodes = uint32_t(bvh->bvh.nodes.size());
+	if (triangles) *triangles = bvh->bvh.triangles.data();
+	if (datas) *datas = bvh->bvh.datas.data();
+	if (numTriangles) *numTriangles = uint32_t(bvh->bvh.triangles.size());
+	return ok();
+}
+
+void phx_bvh_release(PhxBVH* bvh)
+{
+	if (bvh && registry().remove(bvh)) delete bvh;
+}
+
+phx_status phx_bvh_validate(const PhxBVHNode* nodes, uint32_t numNodes,
+                            const PhxTriangleVertices* triangles, uint32_t numTriangles,
+                            uint32_t* outDepth)
+{
+	return validateBVH(nodes, numNodes, triangles, numTriangles, outDepth);
+}
+
+phx_status phx_create(int device, PhxContext** outCtx)
+{
+	if (!outCtx) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_create: outCtx is NULL");
+	*outCtx = nullptr;
+	int count = 0;
+	hipError_t e = hipGetDeviceCount(&count);
+	if (e != hipSuccess || count == 0) return fail(PHX_ERR_NO_DEVICE, "phx_create: no HIP device");
+	if (device < 0 || device >= count)
+		return fail(PHX_ERR_NO_DEVICE, "phx_create: device %d out of range [0,%d)", device, count);
+	PHX_HIP(hipSetDevice(device), "hipSetDevice");
+	hipDeviceProp_t props;
+	PHX_HIP(hipGetDeviceProperties(&props, device), "hipGetDeviceProperties");
+	PhxContext* ctx = new (std::nothrow) PhxContext();
+	if (!ctx) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_create: out of host memory");
+	ctx->device = device;
+	ctx->numCUs = uint32_t(props.multiProcessorCount);
+	e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		delete ctx;
+		return hipFail(e, "hipStreamCreate");
+	}
+	registry().add(ctx);
+	*outCtx = ctx;
+	return ok();
+}
+
+void phx_destroy(PhxContext* ctx)
+{
+	if (!ctx || !registry().remove(ctx)) return;
+	hipSetDevice(ctx->device);
+	hipStreamSynchronize(ctx->stream);
+	hipFree(ctx->stageIn);
+	hipFree(ctx->stageOut);
+	hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+void* phx_context_stream(PhxContext* ctx)
+{
+	if (!ctx || !registry().has(ctx)) return nullptr;
+	return ctx->stream;
+}
+
+uint32_t phx_context_num_cus(PhxContext* ctx)
+{
+	if (!ctx || !registry().has(ctx)) return 0;
+	return ctx->numCUs;
+}
+
+phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!outPtr || bytes == 0) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_device_alloc: bad args");
+	*outPtr = nullptr;
+	hipError_t e = hipMalloc(outPtr, bytes);
+	if (e != hipSuccess) {
+		*outPtr = nullptr;
+		return fail(PHX_ERR_OUT_OF_MEMORY, "phx_device_alloc(%zu): %s", bytes, hipGetErrorString(e));
+	}
+	return ok();
+}
+
+phx_status phx_device_free(PhxContext* ctx, void* ptr)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!ptr) return ok();
+	PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_device_free");
+	PHX_HIP(hipFree(ptr), "hipFree");
+	return ok();
+}
+
+phx_status phx_copy_to_device(PhxContext* ctx, void* dst, const void* src, size_t bytes)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (bytes == 0) return ok();
+	if (!dst || !src) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_copy_to_device: null pointer");
+	PHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "copy H2D");
+	PHX_HIP(hipStreamSynchronize(ctx->stream), "copy H2D");
+	return ok();
+}
+
+phx_status phx_copy_to_host(PhxContext* ctx, void* dst, const void* src, size_t bytes)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (bytes == 0) return ok();
+	if (!dst || !src) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_copy_to_host: null pointer");
+	PHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy D2H");
+	PHX_HIP(hipStreamSynchronize(ctx->stream), "copy D2H");
+	return ok();
+}
+
+phx_status phx_synchronize(PhxContext* ctx)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_synchronize");
+	return ok();
+}
+
+phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t numNodes,
+                            const PhxTriangleVertices* triangles, const PhxTriangleData* datas,
+                            uint32_t numTriangles, const PhxMaterial* materials,
+                            uint32_t numMaterials, const PhxSphereLight* lights,
+                            uint32_t numLights, PhxScene** outScene)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!outScene) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_upload: outScene is NULL");
+	*outScene = nullptr;
+	if (numNodes >= kMaxSceneElements || numTriangles >= kMaxSceneElements)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_upload: more than 2^28 nodes/triangles");
+	if ((numMaterials && !materials) || (numLights && !lights))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_upload: null material/light array");
+	uint32_t depth = 0;
+	st = validateBVH(nodes, numNodes, triangles, numTriangles, &depth);
+	if (st != PHX_OK) return st;
+
+	PhxScene* s = new (std::nothrow) PhxScene();
+	if (!s) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_scene_upload: out of host memory");
+	s->ctx = ctx;
+	s->numNodes = numNodes;
+	s->numTris = numTriangles;
+	s->numMaterials = numMaterials;
+	s->numLights = numLights;
+	s->depth = depth;
+	st = uploadArray(ctx, s->nodes, nodes, numNodes, s->bytes);
+	if (st == PHX_OK) st = uploadArray(ctx, s->tris, triangles, numTriangles, s->bytes);
+	if (st == PHX_OK && datas) st = uploadArray(ctx, s->datas, datas, numTriangles, s->bytes);
+	if (st == PHX_OK) st = uploadArray(ctx, s->materials, materials, numMaterials, s->bytes);
+	if (st == PHX_OK) st = uploadArray(ctx, s->lights, lights, numLights, s->bytes);
+	if (st == PHX_OK) {
+		hipError_t e = hipStreamSynchronize(ctx->stream);
+		if (e != hipSuccess) st = hipFail(e, "phx_scene_upload");
+	}
+	if (st != PHX_OK) {
+		hipStreamSynchronize(ctx->stream);
+		freeScene(s);
+		return st;
+	}
+	registry().add(s);
+	*outScene = s;
+	return ok();
+}
+
+void phx_scene_release(PhxScene* scene)
+{
+	if (!scene || !registry().remove(scene)) return;
+	hipSetDevice(scene->ctx->device);
+	hipStreamSynchronize(scene->ctx->stream);
+	freeScene(scene);
+}
+
+size_t phx_scene_device_bytes(const PhxScene* scene)
+{
+	if (!scene || !registry().has(scene)) return 0;
+	return scene->bytes;
+}
+
+phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                            uint32_t numRays, PhxRayHit* hits, uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	const uint32_t mode = flags & PHX_TRAVERSAL_MASK;
+	if (mode != PHX_TRAVERSAL_REFERENCE_ORDER)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: unknown traversal mode 0x%x", mode);
+	SceneView sv = view(scene);
+	hipStream_t stream = ctx->stream;
+	return castCommon(
+		ctx, rays, numRays, hits, sizeof(PhxRayHit), flags,
+		[&](const PhxRayIn* r, void* out) {
+			return launchClosestReferenceOrder(sv, r, numRays, static_cast<PhxRayHit*>(out),
+			                                   nullptr, stream);
+		},
+		"phx_cast_closest");
+}
+
+phx_status phx_cast_anyhit(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                           uint32_t numRays, uint8_t* inLight, uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	SceneView sv = view(scene);
+	hipStream_t stream = ctx->stream;
+	return castCommon(
+		ctx, rays, numRays, inLight, 1, flags,
+		[&](const PhxRayIn* r, void* out) {
+			return launchAnyHit(sv, r, numRays, static_cast<uint8_t*>(out), stream);
+		},
+		"phx_cast_anyhit");
+}
+
+phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                               uint32_t numRays, uint32_t flags, uint64_t counters[5])
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (!counters) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: counters is NULL");
+	if ((flags & PHX_TRAVERSAL_MASK) != PHX_TRAVERSAL_REFERENCE_ORDER)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: unknown traversal mode");
+	if (flags & PHX_ASYNC) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: no PHX_ASYNC");
+	for (int k = 0; k < 5; k++) counters[k] = 0;
+	if (numRays == 0) return ok();
+	unsigned long long* dCounters = nullptr;
+	PHX_HIP(hipMalloc(&dCounters, 5 * sizeof(unsigned long long)), "phx_traversal_stats");
+	hipError_t e = hipMemsetAsync(dCounters, 0, 5 * sizeof(unsigned long long), ctx->stream);
+	PhxRayHit* dHits = nullptr;
+	if (e == hipSuccess) e = hipMalloc(&dHits, size_t(numRays) * sizeof(PhxRayHit));
+	SceneView sv = view(scene);
+	if (e == hipSuccess) {
+		st = castCommon(
+			ctx, rays, numRays, dHits, sizeof(PhxRayHit), (flags & PHX_MEM_DEVICE),
+			[&](const PhxRayIn* r, void*) {
+				return launchClosestReferenceOrder(sv, r, numRays, dHits, dCounters, ctx->stream);
+			},
+			"phx_traversal_stats");
+	}
+	unsigned long long host[5] = {0, 0, 0, 0, 0};
+	if (e == hipSuccess && st == PHX_OK)
+		e = hipMemcpy(host, dCounters, sizeof(host), hipMemcpyDeviceToHost);
+	hipFree(dHits);
+	hipFree(dCounters);
+	if (e != hipSuccess) return hipFail(e, "phx_traversal_stats");
+	if (st != PHX_OK) return st;
+	for (int k = 0; k < 5; k++) counters[k] = host[k];
+	return ok();
+}
+
+phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                               const PhxRayHit* hits, uint32_t numRays, uint64_t seed,
+                               uint32_t frame, PhxRayIn* outRays, uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (!scene->datas)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_bounce_rays: scene has no TriangleData");
+	if (numRays == 0) return ok();
+	if (!rays || !hits || !outRays)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_bounce_rays: null pointer");
+	const bool device = (flags & PHX_MEM_DEVICE) != 0;
+	const bool async = (flags & PHX_ASYNC) != 0;
+	if (async && !device)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_bounce_rays: PHX_ASYNC needs PHX_MEM_DEVICE");
+	if (device) {
+		PHX_HIP(launchGenBounceRays(rays, hits, scene->datas, numRays, seed, frame, outRays,
+		                            ctx->stream),
+		        "phx_gen_bounce_rays");
+		if (!async) PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_gen_bounce_rays");
+		return ok();
+	}
+	const size_t rb = size_t(numRays) * sizeof(PhxRayIn), hb = size_t(numRays) * sizeof(PhxRayHit);
+	void* tmp = nullptr;
+	PHX_HIP(hipMalloc(&tmp, 2 * rb + hb), "phx_gen_bounce_rays");
+	PhxRayIn* dr = static_cast<PhxRayIn*>(tmp);
+	PhxRayIn* dout = dr + numRays;
+	PhxRayHit* dh = reinterpret_cast<PhxRayHit*>(dout + numRays);
+	hipError_t e = hipMemcpyAsync(dr, rays, rb, hipMemcpyHostToDevice, ctx->stream);
+	if (e == hipSuccess) e = hipMemcpyAsync(dh, hits, hb, hipMemcpyHostToDevice, ctx->stream);
+	if (e == hipSuccess)
+		e = launchGenBounceRays(dr, dh, scene->datas, numRays, seed, frame, dout, ctx->stream);
+	if (e == hipSuccess) e = hipMemcpyAsync(outRays, dout, rb, hipMemcpyDeviceToHost, ctx->stream);
+	hipError_t e2 = hipStreamSynchronize(ctx->stream);
+	hipFree(tmp);
+	if (e != hipSuccess) return hipFail(e, "phx_gen_bounce_rays");
+	if (e2 != hipSuccess) return hipFail(e2, "phx_gen_bounce_rays");
+	return ok();
+}
+
+phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32_t width,
+                                uint32_t height, float tMin, float tMax, uint32_t order,
+                                PhxRayIn* rays, uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!cam || !rays || width == 0 || height == 0)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_primary_rays: bad arguments");
+	if (order != PHX_RAYS_ROW_MAJOR && order != PHX_RAYS_TILED_8X8)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_primary_rays: unknown order %u", order);
+	if (order == PHX_RAYS_TILED_8X8 && ((width % 8) || (height % 8)))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_primary_rays: tiled order needs w,h %% 8 == 0");
+	uint64_t n = uint64_t(width) * height;
+	if (n > 0xFFFFFFFFull) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_primary_rays: too many rays");
+	const bool device = (flags & PHX_MEM_DEVICE) != 0;
+	const bool async = (flags & PHX_ASYNC) != 0;
+	if (async && !device)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_primary_rays: PHX_ASYNC needs PHX_MEM_DEVICE");
+	PhxRayIn* dst = rays;
+	if (!device) {
+		st = growStage(ctx->stageOut, ctx->stageOutBytes, n * sizeof(PhxRayIn));
+		if (st != PHX_OK) return st;
+		dst = static_cast<PhxRayIn*>(ctx->stageOut);
+	}
+	PHX_HIP(launchGenPrimaryRays(*cam, width, height, tMin, tMax, order, dst, ctx->stream),
+	        "phx_gen_primary_rays");
+	if (!device)
+		PHX_HIP(hipMemcpyAsync(rays, dst, n * sizeof(PhxRayIn), hipMemcpyDeviceToHost, ctx->stream),
+		        "phx_gen_primary_rays");
+	if (!async) PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_gen_primary_rays");
+	return ok();
+}
+
+}  // extern "C"

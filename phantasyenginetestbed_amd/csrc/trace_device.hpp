@@ -1,0 +1,92 @@
+// Device-side arithmetic of the SBVH traversal, shared by the HIP kernels.
+//
+// Every function reproduces the reference CPU traversal's IEEE float operation sequence
+// (ray_tracer_common/Intersection.hpp:24-90, Ray.hpp:35-37, sfzCore Vector.inl:345-367 and
+// :425-470, :569-601). The translation units that include this header are compiled with
+// -ffp-contract=off and correctly rounded division, so a*b+c is never fused and 1/x is IEEE
+// division: hits, t, u and v are the CPU's bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace phx {
+namespace dev {
+
+// std::min / std::max: return the FIRST operand when the comparison is false (NaN, +-0 ties).
+// Deliberately not fminf/fmaxf (which drop NaNs) and not v_min3/v_max3.
+__device__ __forceinline__ float min_first(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float max_first(float a, float b) { return (a < b) ? b : a; }
+
+struct Ray {
+	float ox, oy, oz;
+	float dx, dy, dz;
+	float ix, iy, iz;  // 1 / dir, elementwise IEEE division (Ray::setDir, Ray.hpp:35-37)
+	float tMin, tMax;
+};
+
+__device__ __forceinline__ Ray make_ray(const float4 a, const float4 b)
+{
+	Ray r;
+	r.ox = a.x, r.oy = a.y, r.oz = a.z, r.tMin = a.w;
+	r.dx = b.x, r.dy = b.y, r.dz = b.z, r.tMax = b.w;
+	r.ix = 1.0f / b.x;
+	r.iy = 1.0f / b.y;
+	r.iz = 1.0f / b.z;
+	return r;
+}
+
+// Slab test of one AABB (Intersection.hpp:77-90): t1 = (min - o) * invDir, t2 = (max - o) *
+// invDir, tIn = maxElement(min(t1, t2)), tOut = minElement(max(t1, t2)); hit iff tOut >= tIn.
+__device__ __forceinline__ void slab(const Ray& r, float mnx, float mny, float mnz, float mxx,
+                                     float mxy, float mxz, float& tIn, float& tOut)
+{
+	float t1x = (mnx - r.ox) * r.ix;
+	float t1y = (mny - r.oy) * r.iy;
+	float t1z = (mnz - r.oz) * r.iz;
+	float t2x = (mxx - r.ox) * r.ix;
+	float t2y = (mxy - r.oy) * r.iy;
+	float t2z = (mxz - r.oz) * r.iz;
+	float lx = min_first(t1x, t2x), ly = min_first(t1y, t2y), lz = min_first(t1z, t2z);
+	float hx = max_first(t1x, t2x), hy = max_first(t1y, t2y), hz = max_first(t1z, t2z);
+	tIn = max_first(max_first(lx, ly), lz);
+	tOut = min_first(min_first(hx, hy), hz);
+}
+
+// Moller-Trumbore, Intersection.hpp:24-67 (RTR3 p. 750). Returns false on a miss. dot() is
+// (x*x' + y*y') + z*z' and cross() is (y*z'-z*y', z*x'-x*z', x*y'-y*x').
+__device__ __forceinline__ bool triangle(const Ray& r, const float4 p0, const float4 p1,
+                                         const float4 p2, float& t, float& u, float& v)
+{
+	const float EPS = 0.00001f;
+	float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+	float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+	float qx = r.dy * e2z - r.dz * e2y;
+	float qy = r.dz * e2x - r.dx * e2z;
+	float qz = r.dx * e2y - r.dy * e2x;
+	float a = e1x * qx + e1y * qy;
+	a = a + e1z * qz;
+	if (-EPS < a && a < EPS) return false;
+	float f = 1.0f / a;
+	float sx = r.ox - p0.x, sy = r.oy - p0.y, sz = r.oz - p0.z;
+	float sq = sx * qx + sy * qy;
+	sq = sq + sz * qz;
+	u = f * sq;
+	if (u < 0.0f) return false;
+	float rx = sy * e1z - sz * e1y;
+	float ry = sz * e1x - sx * e1z;
+	float rz = sx * e1y - sy * e1x;
+	float dr = r.dx * rx + r.dy * ry;
+	dr = dr + r.dz * rz;
+	v = f * dr;
+	if (v < 0.0f || (u + v) > 1.0f) return false;
+	float er = e2x * rx + e2y * ry;
+	er = er + e2z * rz;
+	t = f * er;
+	return true;
+}
+
+}  // namespace dev
+}  // namespace phx

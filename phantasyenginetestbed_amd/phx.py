@@ -1,0 +1,356 @@
+"""ctypes binding of libphx.so (the C ABI declared in include/phx.h) plus numpy dtypes of its
+structs. Loading fails loudly when the library has not been built; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libphx.so"
+
+# ---- numpy views of the C structs (byte layouts of the reference, see include/phx.h) --------
+NODE = np.dtype([("f", "<f4", (12,)), ("i", "<i4", (4,))])
+TRI = np.dtype([("v", "<f4", (3, 4))])
+TRIDATA = np.dtype([("n", "<f4", (3, 3)), ("uv", "<f4", (3, 2)), ("mat", "<u4")])
+RAY = np.dtype([("o", "<f4", (3,)), ("tmin", "<f4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
+HIT = np.dtype([("tri", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+MATERIAL = np.dtype([("itex", "<i4", (4,)), ("albedo", "<f4", (4,)), ("f2", "<f4", (4,))])
+LIGHT = np.dtype([("pos", "<f4", (3,)), ("radius", "<f4"), ("range", "<f4"),
+                  ("strength", "<f4", (3,)), ("static", "u1"), ("dynamic", "u1"),
+                  ("pad", "u1", (2,))])
+assert NODE.itemsize == 64 and TRI.itemsize == 48 and TRIDATA.itemsize == 64
+assert RAY.itemsize == 32 and HIT.itemsize == 16 and MATERIAL.itemsize == 48
+assert LIGHT.itemsize == 36
+
+MISS = 0xFFFFFFFF
+COUNTER_NAMES = ("nodesVisited", "trianglesIntersected", "triangleIntersectionTests",
+                 "aabbIntersectionTests", "aabbIntersections")
+FLT_MAX = float(np.finfo(np.float32).max)
+
+# status codes
+OK, ERR_INVALID_ARGUMENT, ERR_INVALID_HANDLE, ERR_OUT_OF_MEMORY, ERR_MALFORMED_BVH, ERR_HIP, \
+    ERR_NO_DEVICE, ERR_BUDGET, ERR_NOT_READY, ERR_CANCELLED, ERR_INVALID_SCENE = range(11)
+
+# flags
+MEM_DEVICE = 0x1
+ASYNC = 0x2
+TRAVERSAL_REFERENCE_ORDER = 0x00
+TRAVERSAL_NEAR_FIRST = 0x10
+RAYS_ROW_MAJOR = 0
+RAYS_TILED_8X8 = 1
+
+
+class CameraDef(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("dir", C.c_float * 3), ("dX", C.c_float * 3),
+                ("dY", C.c_float * 3)]
+
+    def as_array(self) -> np.ndarray:
+        return np.array(list(self.origin) + list(self.dir) + list(self.dX) + list(self.dY),
+                        dtype=np.float32)
+
+
+class PhxError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"phx status {status}: {message}")
+        self.status = status
+
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_F32 = C.c_float
+_PF = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "phx_last_error": (C.c_char_p, []),
+    "phx_version": (C.c_char_p, []),
+    "phx_bvh_build": (C.c_int, [_P, _P, _U32, _U32, C.POINTER(_P)]),
+    "phx_bvh_arrays": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_U32), C.POINTER(_P),
+                                 C.POINTER(_P), C.POINTER(_U32)]),
+    "phx_bvh_release": (None, [_P]),
+    "phx_bvh_validate": (C.c_int, [_P, _U32, _P, _U32, C.POINTER(_U32)]),
+    "phx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "phx_destroy": (None, [_P]),
+    "phx_context_stream": (_P, [_P]),
+    "phx_context_num_cus": (_U32, [_P]),
+    "phx_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "phx_device_free": (C.c_int, [_P, _P]),
+    "phx_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "phx_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "phx_synchronize": (C.c_int, [_P]),
+    "phx_scene_upload": (C.c_int, [_P, _P, _U32, _P, _P, _U32, _P, _U32, _P, _U32,
+                                   C.POINTER(_P)]),
+    "phx_scene_release": (None, [_P]),
+    "phx_scene_device_bytes": (C.c_size_t, [_P]),
+    "phx_cast_closest": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
+    "phx_cast_anyhit": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
+    "phx_traversal_stats": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(C.c_uint64)]),
+    "phx_gen_bounce_rays": (C.c_int, [_P, _P, _P, _P, _U32, C.c_uint64, _U32, _P, _U32]),
+    "phx_camera_def": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
+    "phx_camera_look_at": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
+    "phx_gen_primary_rays": (C.c_int, [_P, C.POINTER(CameraDef), _U32, _U32, _F32, _F32, _U32,
+                                       _P, _U32]),
+    "phx_scene_random_triangles": (C.c_int, [_U32, C.c_uint64, _F32, _F32, _P, _P]),
+    "phx_scene_heightfield_boxes_count": (_U32, [_U32, _U32]),
+    "phx_scene_heightfield_boxes": (C.c_int, [_U32, _U32, C.c_uint64, _F32, _F32, _F32, _U32,
+                                              _P, _P]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Loads libphx.so (once). Raises if it is missing: build it with
+    `python -m phantasyenginetestbed_amd.build`."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run python -m "
+                               "phantasyenginetestbed_amd.build")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise PhxError(status, lib().phx_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def _f3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+# ---- host helpers ---------------------------------------------------------------------------
+
+def random_triangles(n: int, seed: int, e_min: float = 0.002, e_max: float = 0.02,
+                     with_data: bool = True):
+    tris = np.zeros(n, TRI)
+    datas = np.zeros(n, TRIDATA) if with_data else None
+    check(lib().phx_scene_random_triangles(n, seed, e_min, e_max, _ptr(tris), _ptr(datas)))
+    return tris, datas
+
+
+def heightfield_boxes(grid: int, num_boxes: int, seed: int, amp: float = 0.1,
+                      box_mu: float = -3.0, box_sigma: float = 0.5, num_materials: int = 0,
+                      with_data: bool = True):
+    n = lib().phx_scene_heightfield_boxes_count(grid, num_boxes)
+    tris = np.zeros(n, TRI)
+    datas = np.zeros(n, TRIDATA) if with_data else None
+    check(lib().phx_scene_heightfield_boxes(grid, num_boxes, seed, amp, box_mu, box_sigma,
+                                            num_materials, _ptr(tris), _ptr(datas)))
+    return tris, datas
+
+
+class BVH:
+    """Host SBVH: buildStaticFrom + sanitizeBVH of the reference, arrays exposed as numpy."""
+
+    def __init__(self, tris: np.ndarray, datas: np.ndarray | None = None, threads: int = 0):
+        tris = np.ascontiguousarray(tris, TRI)
+        if datas is not None:
+            datas = np.ascontiguousarray(datas, TRIDATA)
+        h = C.c_void_p()
+        check(lib().phx_bvh_build(_ptr(tris), _ptr(datas), len(tris), threads, C.byref(h)))
+        self._h = h
+        pn, pt, pd = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nn, nt = C.c_uint32(), C.c_uint32()
+        check(lib().phx_bvh_arrays(h, C.byref(pn), C.byref(nn), C.byref(pt), C.byref(pd),
+                                   C.byref(nt)))
+        # copies, so the arrays outlive the handle
+        self.nodes = np.frombuffer((C.c_char * (nn.value * 64)).from_address(pn.value),
+                                   NODE).copy()
+        self.tris = np.frombuffer((C.c_char * (nt.value * 48)).from_address(pt.value),
+                                  TRI).copy()
+        self.datas = np.frombuffer((C.c_char * (nt.value * 64)).from_address(pd.value),
+                                   TRIDATA).copy()
+        lib().phx_bvh_release(h)
+        self._h = None
+
+    def depth(self) -> int:
+        return validate(self.nodes, self.tris)
+
+
+def validate(nodes: np.ndarray, tris: np.ndarray) -> int:
+    d = C.c_uint32()
+    check(lib().phx_bvh_validate(_ptr(nodes), len(nodes), _ptr(tris), len(tris), C.byref(d)))
+    return d.value
+
+
+def camera_def(pos, dir, up, fov_rad: float, res_x: float, res_y: float) -> CameraDef:
+    cam = CameraDef()
+    check(lib().phx_camera_def(_f3(pos), _f3(dir), _f3(up), fov_rad, res_x, res_y,
+                               C.byref(cam)))
+    return cam
+
+
+def camera_look_at(pos, target, up, fov_rad: float, res_x: float, res_y: float) -> CameraDef:
+    cam = CameraDef()
+    check(lib().phx_camera_look_at(_f3(pos), _f3(target), _f3(up), fov_rad, res_x, res_y,
+                                   C.byref(cam)))
+    return cam
+
+
+# ---- device side ----------------------------------------------------------------------------
+
+class DeviceBuffer:
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().phx_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().phx_copy_to_device(self.ctx.handle, self.ptr, _ptr(a), a.nbytes))
+        return self
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().phx_copy_to_host(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib().phx_device_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Scene:
+    def __init__(self, ctx: "Context", nodes, tris, datas=None, materials=None, lights=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        nm = 0 if materials is None else len(materials)
+        nl = 0 if lights is None else len(lights)
+        check(lib().phx_scene_upload(ctx.handle, _ptr(nodes), len(nodes), _ptr(tris),
+                                     _ptr(datas), len(tris), _ptr(materials), nm, _ptr(lights),
+                                     nl, C.byref(h)))
+        self.handle = h.value
+        self.num_nodes = len(nodes)
+        self.num_tris = len(tris)
+
+    def device_bytes(self) -> int:
+        return lib().phx_scene_device_bytes(self.handle)
+
+    def release(self) -> None:
+        if self.handle:
+            lib().phx_scene_release(self.handle)
+            self.handle = None
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().phx_create(device, C.byref(h)))
+        self.handle = h.value
+
+    @property
+    def stream(self) -> int:
+        return lib().phx_context_stream(self.handle)
+
+    @property
+    def num_cus(self) -> int:
+        return lib().phx_context_num_cus(self.handle)
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def upload_scene(self, nodes, tris, datas=None, materials=None, lights=None) -> Scene:
+        return Scene(self, nodes, tris, datas, materials, lights)
+
+    def cast_closest(self, scene: Scene, rays: np.ndarray, mode: int = TRAVERSAL_REFERENCE_ORDER
+                     ) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY)
+        hits = np.empty(len(rays), HIT)
+        check(lib().phx_cast_closest(self.handle, scene.handle, _ptr(rays), len(rays),
+                                     _ptr(hits), mode))
+        return hits
+
+    def cast_closest_device(self, scene: Scene, rays_ptr: int, n: int, hits_ptr: int,
+                            mode: int = TRAVERSAL_REFERENCE_ORDER, asynchronous: bool = False):
+        flags = MEM_DEVICE | mode | (ASYNC if asynchronous else 0)
+        check(lib().phx_cast_closest(self.handle, scene.handle, rays_ptr, n, hits_ptr, flags))
+
+    def cast_anyhit(self, scene: Scene, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY)
+        out = np.empty(len(rays), np.uint8)
+        check(lib().phx_cast_anyhit(self.handle, scene.handle, _ptr(rays), len(rays),
+                                    _ptr(out), 0))
+        return out
+
+    def cast_anyhit_device(self, scene: Scene, rays_ptr: int, n: int, out_ptr: int,
+                           asynchronous: bool = False):
+        flags = MEM_DEVICE | (ASYNC if asynchronous else 0)
+        check(lib().phx_cast_anyhit(self.handle, scene.handle, rays_ptr, n, out_ptr, flags))
+
+    def traversal_stats(self, scene: Scene, rays: np.ndarray, mode: int = 0) -> dict:
+        rays = np.ascontiguousarray(rays, RAY)
+        c = (C.c_uint64 * 5)()
+        check(lib().phx_traversal_stats(self.handle, scene.handle, _ptr(rays), len(rays), mode, c))
+        return dict(zip(COUNTER_NAMES, list(c)))
+
+    def traversal_stats_device(self, scene: Scene, rays_ptr: int, n: int, mode: int = 0) -> dict:
+        c = (C.c_uint64 * 5)()
+        check(lib().phx_traversal_stats(self.handle, scene.handle, rays_ptr, n,
+                                        mode | MEM_DEVICE, c))
+        return dict(zip(COUNTER_NAMES, list(c)))
+
+    def gen_bounce_rays(self, scene: Scene, rays: np.ndarray, hits: np.ndarray, seed: int,
+                        frame: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY)
+        hits = np.ascontiguousarray(hits, HIT)
+        out = np.empty(len(rays), RAY)
+        check(lib().phx_gen_bounce_rays(self.handle, scene.handle, _ptr(rays), _ptr(hits),
+                                        len(rays), seed, frame, _ptr(out), 0))
+        return out
+
+    def gen_bounce_rays_device(self, scene: Scene, rays_ptr: int, hits_ptr: int, n: int,
+                               seed: int, frame: int, out_ptr: int):
+        check(lib().phx_gen_bounce_rays(self.handle, scene.handle, rays_ptr, hits_ptr, n, seed,
+                                        frame, out_ptr, MEM_DEVICE))
+
+    def gen_primary_rays(self, cam: CameraDef, w: int, h: int, t_min: float = 1e-4,
+                         t_max: float = FLT_MAX, order: int = RAYS_ROW_MAJOR) -> np.ndarray:
+        rays = np.empty(w * h, RAY)
+        check(lib().phx_gen_primary_rays(self.handle, C.byref(cam), w, h, t_min, t_max, order,
+                                         _ptr(rays), 0))
+        return rays
+
+    def gen_primary_rays_device(self, cam: CameraDef, w: int, h: int, out_ptr: int,
+                                t_min: float = 1e-4, t_max: float = FLT_MAX,
+                                order: int = RAYS_ROW_MAJOR):
+        check(lib().phx_gen_primary_rays(self.handle, C.byref(cam), w, h, t_min, t_max, order,
+                                         out_ptr, MEM_DEVICE))
+
+    def synchronize(self) -> None:
+        check(lib().phx_synchronize(self.handle))
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().phx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
