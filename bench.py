@@ -115,6 +115,10 @@ def main() -> int:
     ap.add_argument("--grid", type=int, default=700)
     ap.add_argument("--boxes", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane"], default="persistent")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB first (a plain copy kernel "
+                         "of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard with "
@@ -158,16 +162,25 @@ def main() -> int:
     log(f"per ray: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} triangle tests, "
         f"{bytes_per_ray:.0f} algorithmic bytes")
 
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_on_device(dst.ptr, src.ptr, nb)
+        src.free()
+        dst.free()
+        log(f"calibration copy of {args.calibrate_copy} MiB done")
+
+    mode = phx.KERNEL_ONE_RAY_PER_LANE if args.kernel == "one-ray-per-lane" else 0
     ev = HipEvents()
     stream = ctx.stream
     for _ in range(args.warmup):
-        ctx.cast_closest_device(scene, rays.ptr, n, out.ptr, asynchronous=True)
+        ctx.cast_closest_device(scene, rays.ptr, n, out.ptr, mode=mode, asynchronous=True)
     ctx.synchronize()
     evs = ev.create(args.steps + 1)
     t0 = time.perf_counter()
     ev.record(evs[0], stream)
     for k in range(args.steps):
-        ctx.cast_closest_device(scene, rays.ptr, n, out.ptr, asynchronous=True)
+        ctx.cast_closest_device(scene, rays.ptr, n, out.ptr, mode=mode, asynchronous=True)
         ev.record(evs[k + 1], stream)
     ctx.synchronize()
     wall = time.perf_counter() - t0
@@ -190,7 +203,8 @@ def main() -> int:
                                "SBVH, rays from 1920x1080 primary hits",
                    "scene_tris": int(len(tris)), "sbvh_nodes": int(len(bvh.nodes)),
                    "sbvh_refs": int(len(bvh.tris)), "rays_per_step": int(n),
-                   "traversal": "reference-order (bit-identical to castRay)"},
+                   "traversal": "reference-order (bit-identical to castRay)",
+                   "kernel": args.kernel},
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -141,6 +141,12 @@ typedef struct PhxCameraDef {
 #define PHX_TRAVERSAL_NEAR_FIRST 0x10u      /* nearer child first; ties broken by castRay order  */
 #define PHX_TRAVERSAL_MASK 0xF0u
 
+/* Launch model of phx_cast_closest / phx_cast_anyhit / phx_traversal_stats. Default: persistent
+ * wavefronts refilled from a shared ray counter (rayCastKernel's model, RayCastKernel.cu:179-411).
+ * This flag selects one ray per lane (rayCastNoPersistenceKernel, RayCastKernel.cu:416-572).
+ * Results are identical either way. */
+#define PHX_KERNEL_ONE_RAY_PER_LANE 0x100u
+
 /* Ray order of phx_gen_primary_rays. */
 #define PHX_RAYS_ROW_MAJOR 0u  /* id = y * w + x (CPUTracerRenderer.cpp:84-97)                   */
 #define PHX_RAYS_TILED_8X8 1u  /* 8x8-tile order of getRayIdx (CudaDeviceHelpers.cuh:49-62)       */
@@ -194,6 +200,7 @@ phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr);
 phx_status phx_device_free(PhxContext* ctx, void* ptr);
 phx_status phx_copy_to_device(PhxContext* ctx, void* dst, const void* src, size_t bytes);
 phx_status phx_copy_to_host(PhxContext* ctx, void* dst, const void* src, size_t bytes);
+phx_status phx_copy_on_device(PhxContext* ctx, void* dst, const void* src, size_t bytes);
 phx_status phx_synchronize(PhxContext* ctx);
 
 /* ------------------------------------------------------------------------------------------ */

@@ -87,6 +87,13 @@ struct PhxContext {
 	int device = 0;
 	hipStream_t stream = nullptr;
 	uint32_t numCUs = 0;
+	// Persistent-kernel state of this context's stream: the ray counter pair (left at zero by
+	// every launch), the spill area of the LDS stacks, and the launch shape.
+	uint32_t* work = nullptr;
+	uint32_t* spill = nullptr;
+	uint32_t gridClosest = 0;
+	uint32_t gridAnyhit = 0;
+	uint32_t refillMin = 8;
 	// Staging buffers for calls that pass host pointers, grown on demand.
 	void* stageIn = nullptr;
 	size_t stageInBytes = 0;
@@ -199,6 +206,34 @@ phx_status castCommon(PhxContext* ctx, const PhxRayIn* rays, uint32_t n, void* o
 
 SceneView view(const PhxScene* s) { return SceneView{s->nodes, s->tris, s->numNodes, s->numTris}; }
 
+TraceLaunch traceLaunch(const PhxContext* ctx, uint32_t flags, bool anyhit)
+{
+	TraceLaunch L;
+	L.kernel = (flags & PHX_KERNEL_ONE_RAY_PER_LANE) ? TraceKernel::Simple : TraceKernel::Persistent;
+	L.gridBlocks = anyhit ? ctx->gridAnyhit : ctx->gridClosest;
+	L.refillMin = ctx->refillMin;
+	L.work = ctx->work;
+	L.spill = ctx->spill;
+	return L;
+}
+
+// Triangles as (p0, e1 = p1 - p0, e2 = p2 - p0); p0.w keeps the end marker. The subtractions are
+// the IEEE float operations Intersection.hpp:33-34 performs per test, done once here.
+std::vector<PhxTriangleVertices> pretransform(const PhxTriangleVertices* t, uint32_t n)
+{
+	std::vector<PhxTriangleVertices> out(n);
+	for (uint32_t i = 0; i < n; i++) {
+		for (int k = 0; k < 4; k++) out[i].v0[k] = t[i].v0[k];
+		for (int k = 0; k < 3; k++) {
+			out[i].v1[k] = t[i].v1[k] - t[i].v0[k];
+			out[i].v2[k] = t[i].v2[k] - t[i].v0[k];
+		}
+		out[i].v1[3] = 0.0f;
+		out[i].v2[3] = 0.0f;
+	}
+	return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -275,6 +310,22 @@ phx_status phx_create(int device, PhxContext** outCtx)
 		delete ctx;
 		return hipFail(e, "hipStreamCreate");
 	}
+	uint32_t bc = 1, ba = 1;
+	e = persistentBlocksPerCU(&bc, &ba);
+	ctx->gridClosest = ctx->numCUs * bc;
+	ctx->gridAnyhit = ctx->numCUs * ba;
+	const uint32_t maxGrid = std::max(ctx->gridClosest, ctx->gridAnyhit);
+	if (e == hipSuccess) e = hipMalloc(&ctx->work, 64);
+	if (e == hipSuccess) e = hipMemset(ctx->work, 0, 64);
+	if (e == hipSuccess)
+		e = hipMalloc(&ctx->spill, size_t(persistentSpillWords(maxGrid)) * sizeof(uint32_t));
+	if (e != hipSuccess) {
+		hipFree(ctx->work);
+		hipFree(ctx->spill);
+		hipStreamDestroy(ctx->stream);
+		delete ctx;
+		return hipFail(e, "phx_create: persistent-kernel state");
+	}
 	registry().add(ctx);
 	*outCtx = ctx;
 	return ok();
@@ -287,6 +338,8 @@ void phx_destroy(PhxContext* ctx)
 	hipStreamSynchronize(ctx->stream);
 	hipFree(ctx->stageIn);
 	hipFree(ctx->stageOut);
+	hipFree(ctx->work);
+	hipFree(ctx->spill);
 	hipStreamDestroy(ctx->stream);
 	delete ctx;
 }
@@ -349,6 +402,17 @@ phx_status phx_copy_to_host(PhxContext* ctx, void* dst, const void* src, size_t 
 	return ok();
 }
 
+phx_status phx_copy_on_device(PhxContext* ctx, void* dst, const void* src, size_t bytes)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (bytes == 0) return ok();
+	if (!dst || !src) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_copy_on_device: null pointer");
+	PHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream), "copy D2D");
+	PHX_HIP(hipStreamSynchronize(ctx->stream), "copy D2D");
+	return ok();
+}
+
 phx_status phx_synchronize(PhxContext* ctx)
 {
 	phx_status st = checkCtx(ctx);
@@ -384,7 +448,12 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 	s->numLights = numLights;
 	s->depth = depth;
 	st = uploadArray(ctx, s->nodes, nodes, numNodes, s->bytes);
-	if (st == PHX_OK) st = uploadArray(ctx, s->tris, triangles, numTriangles, s->bytes);
+	if (st == PHX_OK) {
+		std::vector<PhxTriangleVertices> pre = pretransform(triangles, numTriangles);
+		st = uploadArray(ctx, s->tris, pre.data(), numTriangles, s->bytes);
+		if (st == PHX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+			st = fail(PHX_ERR_HIP, "phx_scene_upload: triangle upload failed");
+	}
 	if (st == PHX_OK && datas) st = uploadArray(ctx, s->datas, datas, numTriangles, s->bytes);
 	if (st == PHX_OK) st = uploadArray(ctx, s->materials, materials, numMaterials, s->bytes);
 	if (st == PHX_OK) st = uploadArray(ctx, s->lights, lights, numLights, s->bytes);
@@ -428,11 +497,11 @@ phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRay
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: unknown traversal mode 0x%x", mode);
 	SceneView sv = view(scene);
 	hipStream_t stream = ctx->stream;
+	const TraceLaunch L = traceLaunch(ctx, flags, false);
 	return castCommon(
 		ctx, rays, numRays, hits, sizeof(PhxRayHit), flags,
 		[&](const PhxRayIn* r, void* out) {
-			return launchClosestReferenceOrder(sv, r, numRays, static_cast<PhxRayHit*>(out),
-			                                   nullptr, stream);
+			return launchClosest(sv, r, numRays, static_cast<PhxRayHit*>(out), L, nullptr, stream);
 		},
 		"phx_cast_closest");
 }
@@ -446,10 +515,11 @@ phx_status phx_cast_anyhit(PhxContext* ctx, const PhxScene* scene, const PhxRayI
 	if (st != PHX_OK) return st;
 	SceneView sv = view(scene);
 	hipStream_t stream = ctx->stream;
+	const TraceLaunch L = traceLaunch(ctx, flags, true);
 	return castCommon(
 		ctx, rays, numRays, inLight, 1, flags,
 		[&](const PhxRayIn* r, void* out) {
-			return launchAnyHit(sv, r, numRays, static_cast<uint8_t*>(out), stream);
+			return launchAnyHit(sv, r, numRays, static_cast<uint8_t*>(out), L, stream);
 		},
 		"phx_cast_anyhit");
 }
@@ -473,11 +543,12 @@ phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const Phx
 	PhxRayHit* dHits = nullptr;
 	if (e == hipSuccess) e = hipMalloc(&dHits, size_t(numRays) * sizeof(PhxRayHit));
 	SceneView sv = view(scene);
+	const TraceLaunch L = traceLaunch(ctx, flags, false);
 	if (e == hipSuccess) {
 		st = castCommon(
 			ctx, rays, numRays, dHits, sizeof(PhxRayHit), (flags & PHX_MEM_DEVICE),
 			[&](const PhxRayIn* r, void*) {
-				return launchClosestReferenceOrder(sv, r, numRays, dHits, dCounters, ctx->stream);
+				return launchClosest(sv, r, numRays, dHits, L, dCounters, ctx->stream);
 			},
 			"phx_traversal_stats");
 	}

@@ -9,23 +9,40 @@
 
 namespace phx {
 
-// Device view of an uploaded scene.
+// Device view of an uploaded scene. trisPre holds each triangle as (p0, e1 = p1 - p0,
+// e2 = p2 - p0) with the end marker in p0.w.
 struct SceneView {
 	const PhxBVHNode* nodes;
-	const PhxTriangleVertices* tris;
+	const PhxTriangleVertices* trisPre;
 	uint32_t numNodes;
 	uint32_t numTris;
 };
 
-// Closest hit in the CPU castRay visiting order; one ray per lane. If `counters` is not null the
-// castDebugRay counters of the batch are added to counters[0..4] (device memory).
-hipError_t launchClosestReferenceOrder(const SceneView& s, const PhxRayIn* rays, uint32_t n,
-                                       PhxRayHit* hits, unsigned long long* counters,
-                                       hipStream_t stream);
+enum class TraceKernel { Persistent, Simple };
 
-// Any hit (shadow); one ray per lane.
+// Launch parameters of the traversal kernels. `work` points at two zeroed uint32 words owned by
+// the launching stream (the persistent kernels leave them zeroed); `spill` holds
+// persistentSpillWords(gridBlocks) words.
+struct TraceLaunch {
+	TraceKernel kernel = TraceKernel::Persistent;
+	uint32_t gridBlocks = 0;   // persistent grid (blocks of 256 threads)
+	uint32_t refillMin = 8;    // refill a wave once this many of its lanes are idle
+	uint32_t* work = nullptr;
+	uint32_t* spill = nullptr;
+};
+
+uint32_t persistentSpillWords(uint32_t gridBlocks);
+hipError_t persistentBlocksPerCU(uint32_t* closestBlocks, uint32_t* anyhitBlocks);
+
+// Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
+// counters of the batch are added to counters[0..4] (device memory).
+hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, PhxRayHit* hits,
+                         const TraceLaunch& launch, unsigned long long* counters,
+                         hipStream_t stream);
+
+// Any hit (shadow rays).
 hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, uint8_t* inLight,
-                        hipStream_t stream);
+                        const TraceLaunch& launch, hipStream_t stream);
 
 // Pinhole primary rays.
 hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float tMin,

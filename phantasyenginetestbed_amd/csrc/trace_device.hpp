@@ -90,3 +90,41 @@ __device__ __forceinline__ bool triangle(const Ray& r, const float4 p0, const fl
 
 }  // namespace dev
 }  // namespace phx
+
+namespace phx {
+namespace dev {
+
+// Moller-Trumbore on a pre-transformed triangle (p0, e1 = p1 - p0, e2 = p2 - p0). e1 and e2 are
+// computed once at upload with the same IEEE subtraction the CPU performs per test, so every
+// subsequent operation -- and the result -- is bit-identical to triangle().
+__device__ __forceinline__ bool triangle_pre(const Ray& r, const float4 p0, const float4 e1,
+                                             const float4 e2, float& t, float& u, float& v)
+{
+	const float EPS = 0.00001f;
+	float qx = r.dy * e2.z - r.dz * e2.y;
+	float qy = r.dz * e2.x - r.dx * e2.z;
+	float qz = r.dx * e2.y - r.dy * e2.x;
+	float a = e1.x * qx + e1.y * qy;
+	a = a + e1.z * qz;
+	if (-EPS < a && a < EPS) return false;
+	float f = 1.0f / a;
+	float sx = r.ox - p0.x, sy = r.oy - p0.y, sz = r.oz - p0.z;
+	float sq = sx * qx + sy * qy;
+	sq = sq + sz * qz;
+	u = f * sq;
+	if (u < 0.0f) return false;
+	float rx = sy * e1.z - sz * e1.y;
+	float ry = sz * e1.x - sx * e1.z;
+	float rz = sx * e1.y - sy * e1.x;
+	float dr = r.dx * rx + r.dy * ry;
+	dr = dr + r.dz * rz;
+	v = f * dr;
+	if (v < 0.0f || (u + v) > 1.0f) return false;
+	float er = e2.x * rx + e2.y * ry;
+	er = er + e2.z * rz;
+	t = f * er;
+	return true;
+}
+
+}  // namespace dev
+}  // namespace phx

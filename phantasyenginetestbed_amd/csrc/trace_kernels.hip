@@ -1,9 +1,16 @@
 // SBVH traversal kernels for gfx950 (wave64).
 //
-// Node layout (64 B, BVHNode.hpp:23-149): four 16-B words
+// Node layout (64 B, BVHNode.hpp:23-149): four 16-B words read as four 128-bit loads
 //   w0 = lc.min.xyz, lc.max.x   w1 = lc.max.yz, rc.min.xy   w2 = rc.min.z, rc.max.xyz
 //   w3 = (lcIndex, rcIndex, lcNumTriangles, rcNumTriangles)
-// read as four 128-bit loads; a triangle (48 B) is three 128-bit loads.
+// Triangles are the pre-transformed copy made at upload: (p0, e1 = p1 - p0, e2 = p2 - p0), three
+// 128-bit loads per triangle.
+//
+// Visiting order of every closest-hit kernel here is castRay's (BVHTraversal.hpp:37-128): pop a
+// node, decide both children against min(tMax, closest.t) taken before either is processed,
+// handle the left child (push if inner, test its triangles if a leaf), then the right child; the
+// stack is LIFO. The persistent kernels keep the node castRay would pop next in a register
+// ("next") and the rest of castRay's stack in LDS, so the order -- and every hit -- is unchanged.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -12,9 +19,9 @@
 namespace phx {
 namespace {
 
-// Deepest stack the reference-order traversal can need: one pending entry per level of a tree of
-// depth <= 64 (enforced at upload) plus the two children of the node being expanded.
-constexpr int kStackSize = 72;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
 
 struct NodeWords {
 	float4 w0, w1, w2;
@@ -37,180 +44,470 @@ struct Closest {
 	float t, u, v;
 };
 
+struct Counts {
+	uint32_t nodes = 0, accepted = 0, triTests = 0, boxHits = 0;
+};
+
 // Tests the `count` triangles of a leaf starting at `first` (BVHTraversal.hpp:78-93).
-__device__ __forceinline__ void test_leaf(const dev::Ray& r, const float4* __restrict__ tris,
-                                          uint32_t first, int32_t count, Closest& c,
-                                          uint32_t* accepted)
+template <bool kCount>
+__device__ __forceinline__ void closest_leaf(const dev::Ray& r, const float4* __restrict__ tris,
+                                             uint32_t first, int32_t count, Closest& c, Counts& k)
 {
-	for (int32_t k = 0; k < count; k++) {
-		const float4* p = tris + 3u * (first + uint32_t(k));
-		float4 p0 = p[0], p1 = p[1], p2 = p[2];
+	for (int32_t j = 0; j < count; j++) {
+		const float4* p = tris + 3u * (first + uint32_t(j));
+		const float4 p0 = p[0], e1 = p[1], e2 = p[2];
 		float t, u, v;
-		if (dev::triangle(r, p0, p1, p2, t, u, v)) {
-			if (t < c.t && r.tMin <= t && t <= r.tMax) {
-				c.tri = first + uint32_t(k);
-				if (accepted) (*accepted)++;
-				c.t = t;
-				c.u = u;
-				c.v = v;
-			}
+		if (dev::triangle_pre(r, p0, e1, e2, t, u, v) && t < c.t && r.tMin <= t && t <= r.tMax) {
+			c.tri = first + uint32_t(j);
+			c.t = t;
+			c.u = u;
+			c.v = v;
+			if (kCount) k.accepted++;
 		}
 	}
+	if (kCount) k.triTests += uint32_t(count);
 }
 
-// Closest hit with castRay's visiting order (BVHTraversal.hpp:37-128): pop a node, decide both
-// children against min(tMax, closest.t) taken before either child is processed, then handle the
-// left child (push it if inner, test its triangles if a leaf) and then the right child. The
-// stack is LIFO, so the right subtree is explored before the left one.
-// kCount adds castDebugRay's counters (BVHTraversal.hpp:130-235), summed over the batch into
-// counters[0..4] = nodesVisited, trianglesIntersected, triangleIntersectionTests,
-// aabbIntersectionTests, aabbIntersections.
+// Any-hit leaf: true iff a triangle has tMin <= t < tMax (RayCastKernel.cu:759-765).
+__device__ __forceinline__ bool anyhit_leaf(const dev::Ray& r, const float4* __restrict__ tris,
+                                            uint32_t first, int32_t count)
+{
+	for (int32_t j = 0; j < count; j++) {
+		const float4* p = tris + 3u * (first + uint32_t(j));
+		float t, u, v;
+		if (dev::triangle_pre(r, p[0], p[1], p[2], t, u, v) && r.tMin <= t && t < r.tMax)
+			return true;
+	}
+	return false;
+}
+
+__device__ __forceinline__ uint32_t child_index(int32_t raw)
+{
+	return uint32_t(raw < 0 ? ~raw : raw);
+}
+
+__device__ __forceinline__ void add_counts(unsigned long long* counters, const Counts& k)
+{
+	atomicAdd(&counters[0], (unsigned long long)k.nodes);
+	atomicAdd(&counters[1], (unsigned long long)k.accepted);
+	atomicAdd(&counters[2], (unsigned long long)k.triTests);
+	atomicAdd(&counters[3], 2ull * k.nodes);
+	atomicAdd(&counters[4], (unsigned long long)k.boxHits);
+}
+
+// ---- one ray per lane -------------------------------------------------------------------------
+
+// Deepest castRay stack for a tree of depth <= 64 (enforced at upload): one pending entry per
+// level plus the two children of the node being expanded.
+constexpr int kSimpleStack = 72;
+
 template <bool kCount>
-__global__ __launch_bounds__(256) void k_closest_reference_order(
+__global__ __launch_bounds__(kBlock) void k_closest_simple(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
 	unsigned long long* __restrict__ counters)
 {
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
-
-	uint32_t stack[kStackSize];
+	uint32_t stack[kSimpleStack];
 	stack[0] = 0u;
 	int sp = 1;
-	Closest c = {~0u, FLT_MAX, FLT_MAX, FLT_MAX};
-	uint32_t nNodes = 0, nAccepted = 0, nTriTests = 0, nBoxHits = 0;
-
+	Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+	Counts k;
 	while (sp > 0) {
 		sp--;
 		const NodeWords nd = load_node(nodes, stack[sp]);
-		if (kCount) nNodes++;
+		if (kCount) k.nodes++;
 		float inL, outL, inR, outR;
 		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
 		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
 		const float tCurrMax = dev::min_first(r.tMax, c.t);
 		const bool visitL = (outL >= inL) && outL > r.tMin && inL < tCurrMax;
 		const bool visitR = (outR >= inR) && outR > r.tMin && inR < tCurrMax;
-
 		if (visitL) {
-			if (kCount) nBoxHits++;
-			const int32_t raw = nd.w3.x;
-			const uint32_t idx = uint32_t(raw < 0 ? ~raw : raw);
+			if (kCount) k.boxHits++;
 			if (nd.w3.z == 0) {
-				if (sp < kStackSize) stack[sp++] = idx;
+				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.x);
 			} else {
-				test_leaf(r, tris, idx, nd.w3.z, c, kCount ? &nAccepted : nullptr);
-				if (kCount) nTriTests += uint32_t(nd.w3.z);
+				closest_leaf<kCount>(r, tris, child_index(nd.w3.x), nd.w3.z, c, k);
 			}
 		}
 		if (visitR) {
-			if (kCount) nBoxHits++;
-			const int32_t raw = nd.w3.y;
-			const uint32_t idx = uint32_t(raw < 0 ? ~raw : raw);
+			if (kCount) k.boxHits++;
 			if (nd.w3.w == 0) {
-				if (sp < kStackSize) stack[sp++] = idx;
+				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.y);
 			} else {
-				test_leaf(r, tris, idx, nd.w3.w, c, kCount ? &nAccepted : nullptr);
-				if (kCount) nTriTests += uint32_t(nd.w3.w);
+				closest_leaf<kCount>(r, tris, child_index(nd.w3.y), nd.w3.w, c, k);
 			}
 		}
 	}
-	uint4 out;
-	out.x = c.tri;
-	out.y = __float_as_uint(c.t);
-	out.z = __float_as_uint(c.u);
-	out.w = __float_as_uint(c.v);
-	hits[i] = out;
-	if (kCount) {
-		atomicAdd(&counters[0], (unsigned long long)nNodes);
-		atomicAdd(&counters[1], (unsigned long long)nAccepted);
-		atomicAdd(&counters[2], (unsigned long long)nTriTests);
-		atomicAdd(&counters[3], 2ull * nNodes);
-		atomicAdd(&counters[4], (unsigned long long)nBoxHits);
-	}
+	hits[i] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
+	if (kCount) add_counts(counters, k);
 }
 
-// Any hit: occluded iff some triangle has tMin <= t < tMax (the reference's shadow predicate,
-// RayCastKernel.cu:759-765), with boxes culled by castRay's predicate (hit, tOut > tMin,
-// tIn < tMax). The answer does not depend on the visiting order.
-__global__ __launch_bounds__(256) void k_anyhit(const float4* __restrict__ nodes,
-                                                const float4* __restrict__ tris,
-                                                const float4* __restrict__ rays,
-                                                uint8_t* __restrict__ inLight, uint32_t n)
+__global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restrict__ nodes,
+                                                          const float4* __restrict__ tris,
+                                                          const float4* __restrict__ rays,
+                                                          uint8_t* __restrict__ inLight, uint32_t n)
 {
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
-
-	uint32_t stack[kStackSize];
+	uint32_t stack[kSimpleStack];
 	stack[0] = 0u;
 	int sp = 1;
 	bool occluded = false;
-
 	while (sp > 0 && !occluded) {
 		sp--;
 		const NodeWords nd = load_node(nodes, stack[sp]);
 		float inL, outL, inR, outR;
 		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
 		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
-		const bool visit[2] = {(outL >= inL) && outL > r.tMin && inL < r.tMax,
-		                       (outR >= inR) && outR > r.tMin && inR < r.tMax};
-		for (int side = 0; side < 2 && !occluded; side++) {
-			if (!visit[side]) continue;
-			const int32_t raw = side == 0 ? nd.w3.x : nd.w3.y;
-			const int32_t count = side == 0 ? nd.w3.z : nd.w3.w;
-			const uint32_t idx = uint32_t(raw < 0 ? ~raw : raw);
-			if (count == 0) {
-				if (sp < kStackSize) stack[sp++] = idx;
-				continue;
+		const bool visitL = (outL >= inL) && outL > r.tMin && inL < r.tMax;
+		const bool visitR = (outR >= inR) && outR > r.tMin && inR < r.tMax;
+		if (visitL) {
+			if (nd.w3.z == 0) {
+				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.x);
+			} else {
+				occluded = anyhit_leaf(r, tris, child_index(nd.w3.x), nd.w3.z);
 			}
-			for (int32_t k = 0; k < count; k++) {
-				const float4* p = tris + 3u * (idx + uint32_t(k));
-				float t, u, v;
-				if (dev::triangle(r, p[0], p[1], p[2], t, u, v) && r.tMin <= t && t < r.tMax) {
-					occluded = true;
-					break;
-				}
+		}
+		if (visitR && !occluded) {
+			if (nd.w3.w == 0) {
+				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.y);
+			} else {
+				occluded = anyhit_leaf(r, tris, child_index(nd.w3.y), nd.w3.w);
 			}
 		}
 	}
 	inLight[i] = occluded ? 0 : 1;
 }
 
+// ---- persistent wavefronts ----------------------------------------------------------------------
+//
+// A fixed grid of waves pulls rays from a shared counter, one atomic per wavefront for a window
+// of kWindow rays. Every iteration, lanes without a ray are refilled from the window when at least
+// `refillMin` of the wave's 64 lanes are idle (always when all are): the ray loads of the
+// refilled lanes are issued together with the node loads of the busy lanes, and the root node --
+// the first node of every ray -- is held in registers, so a refill costs no extra round trip.
+// The last wave to finish resets the counter, so every launch starts from zero without a memset.
+//
+// castRay's pending stack lives in LDS, laid out [wave][entry][lane] (one 4-B column per lane,
+// so each push/pop is a conflict-free 256-B row), kLdsDepth entries deep; deeper entries spill to
+// a global array laid out [entry][lane] (coalesced). Trees are at most 64 levels deep (checked at
+// upload), so the pending stack never exceeds kPersistentStack entries.
+
+constexpr int kLdsDepth = 16;
+constexpr int kPersistentStack = 72;
+constexpr uint32_t kWindow = 64;
+
+struct WaveQueue {
+	uint32_t base = 0;   // next unclaimed ray of this wave's window
+	uint32_t left = 0;   // rays left in the window
+	bool exhausted = false;
+};
+
+// Assigns ray indices to the idle lanes of the wave (`idle` = ballot of idle lanes). Returns the
+// lane's new ray index, or kInvalid. Wave-uniform state lives in `q`.
+__device__ __forceinline__ uint32_t refill(WaveQueue& q, uint64_t idle, uint32_t* work,
+                                           uint32_t n, bool isIdle)
+{
+	const uint32_t need = uint32_t(__popcll(idle));
+	const uint32_t take1 = need < q.left ? need : q.left;
+	const uint32_t base1 = q.base;
+	q.base += take1;
+	q.left -= take1;
+	uint32_t take2 = 0, base2 = 0;
+	if (need > take1) {
+		const int leader = __ffsll((unsigned long long)idle) - 1;
+		uint32_t b = 0;
+		if (int(__lane_id()) == leader) b = atomicAdd(work, kWindow);
+		b = __shfl(b, leader);
+		if (b >= n) {
+			q.exhausted = true;
+			q.left = 0;
+		} else {
+			const uint32_t avail = (n - b) < kWindow ? (n - b) : kWindow;
+			take2 = (need - take1) < avail ? (need - take1) : avail;
+			base2 = b;
+			q.base = b + take2;
+			q.left = avail - take2;
+		}
+	}
+	if (!isIdle) return kInvalid;
+	const uint32_t rank = uint32_t(__popcll(idle & __lanemask_lt()));
+	if (rank < take1) return base1 + rank;
+	if (rank < take1 + take2) return base2 + (rank - take1);
+	return kInvalid;
+}
+
+// Called by every wave as it exits: the last one resets the shared counters to zero.
+__device__ __forceinline__ void retire_wave(uint32_t* work)
+{
+	if (__lane_id() == 0) {
+		__threadfence();
+		const uint32_t totalWaves = gridDim.x * (blockDim.x / 64u);
+		if (atomicAdd(&work[1], 1u) == totalWaves - 1u) {
+			atomicExch(&work[0], 0u);
+			atomicExch(&work[1], 0u);
+		}
+	}
+}
+
+struct LaneStack {
+	uint32_t (*lds)[64];  // [entry][lane] of this wave
+	uint32_t* spill;      // this lane's spill column
+	uint32_t stride;      // spill row stride (lanes in the grid)
+	uint32_t lane;
+	int sp;
+
+	__device__ __forceinline__ void push(uint32_t v)
+	{
+		if (sp < kLdsDepth)
+			lds[sp][lane] = v;
+		else if (sp < kPersistentStack)
+			spill[size_t(sp - kLdsDepth) * stride] = v;
+		sp++;
+	}
+	__device__ __forceinline__ uint32_t pop()
+	{
+		sp--;
+		if (sp < kLdsDepth) return lds[sp][lane];
+		if (sp < kPersistentStack) return spill[size_t(sp - kLdsDepth) * stride];
+		return 0u;
+	}
+};
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_closest_persistent(
+	const float4* __restrict__ nodes, const float4* __restrict__ tris,
+	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
+	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
+	unsigned long long* __restrict__ counters)
+{
+	__shared__ uint32_t ldsStack[kWavesPerBlock][kLdsDepth][64];
+	const uint32_t lane = __lane_id();
+	const uint32_t wave = threadIdx.x >> 6;
+	LaneStack st;
+	st.lds = ldsStack[wave];
+	st.stride = gridDim.x * blockDim.x;
+	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
+	st.lane = lane;
+	st.sp = 0;
+
+	const NodeWords root = load_node(nodes, 0);
+	WaveQueue q;
+	uint32_t rayIdx = kInvalid;
+	int32_t next = -1;
+	dev::Ray r;
+	Closest c;
+	Counts k;
+
+	while (true) {
+		const uint64_t idle = __ballot(rayIdx == kInvalid);
+		bool fresh = false;
+		if (!q.exhausted && idle != 0 &&
+		    (uint32_t(__popcll(idle)) >= refillMin || idle == ~0ull)) {
+			const uint32_t got = refill(q, idle, work, n, rayIdx == kInvalid);
+			if (got != kInvalid) {
+				rayIdx = got;
+				fresh = true;
+			}
+		}
+		if (q.exhausted && __ballot(rayIdx != kInvalid) == 0) break;
+		if (rayIdx == kInvalid) continue;
+
+		NodeWords nd;
+		float4 ra, rb;
+		if (fresh) {
+			ra = rays[2 * size_t(rayIdx)];
+			rb = rays[2 * size_t(rayIdx) + 1];
+		} else {
+			nd = load_node(nodes, uint32_t(next));
+		}
+		if (fresh) {
+			r = dev::make_ray(ra, rb);
+			c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+			st.sp = 0;
+			nd = root;
+		}
+
+		if (kCount) k.nodes++;
+		float inL, outL, inR, outR;
+		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
+		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
+		const float tCurrMax = dev::min_first(r.tMax, c.t);
+		const bool visitL = (outL >= inL) && outL > r.tMin && inL < tCurrMax;
+		const bool visitR = (outR >= inR) && outR > r.tMin && inR < tCurrMax;
+		const bool innerL = visitL && nd.w3.z == 0;
+		const bool innerR = visitR && nd.w3.w == 0;
+		if (kCount) k.boxHits += uint32_t(visitL) + uint32_t(visitR);
+		if (visitL && !innerL) closest_leaf<kCount>(r, tris, child_index(nd.w3.x), nd.w3.z, c, k);
+		if (visitR && !innerR) closest_leaf<kCount>(r, tris, child_index(nd.w3.y), nd.w3.w, c, k);
+		if (innerL && innerR) {
+			st.push(child_index(nd.w3.x));
+			next = int32_t(child_index(nd.w3.y));
+		} else if (innerR) {
+			next = int32_t(child_index(nd.w3.y));
+		} else if (innerL) {
+			next = int32_t(child_index(nd.w3.x));
+		} else {
+			next = st.sp > 0 ? int32_t(st.pop()) : -1;
+		}
+		if (next < 0) {
+			hits[rayIdx] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u),
+			                          __float_as_uint(c.v));
+			rayIdx = kInvalid;
+		}
+	}
+	if (kCount) add_counts(counters, k);
+	retire_wave(work);
+}
+
+__global__ __launch_bounds__(kBlock) void k_anyhit_persistent(
+	const float4* __restrict__ nodes, const float4* __restrict__ tris,
+	const float4* __restrict__ rays, uint8_t* __restrict__ inLight, uint32_t n,
+	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin)
+{
+	__shared__ uint32_t ldsStack[kWavesPerBlock][kLdsDepth][64];
+	const uint32_t lane = __lane_id();
+	const uint32_t wave = threadIdx.x >> 6;
+	LaneStack st;
+	st.lds = ldsStack[wave];
+	st.stride = gridDim.x * blockDim.x;
+	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
+	st.lane = lane;
+	st.sp = 0;
+
+	const NodeWords root = load_node(nodes, 0);
+	WaveQueue q;
+	uint32_t rayIdx = kInvalid;
+	int32_t next = -1;
+	dev::Ray r;
+
+	while (true) {
+		const uint64_t idle = __ballot(rayIdx == kInvalid);
+		bool fresh = false;
+		if (!q.exhausted && idle != 0 &&
+		    (uint32_t(__popcll(idle)) >= refillMin || idle == ~0ull)) {
+			const uint32_t got = refill(q, idle, work, n, rayIdx == kInvalid);
+			if (got != kInvalid) {
+				rayIdx = got;
+				fresh = true;
+			}
+		}
+		if (q.exhausted && __ballot(rayIdx != kInvalid) == 0) break;
+		if (rayIdx == kInvalid) continue;
+
+		NodeWords nd;
+		float4 ra, rb;
+		if (fresh) {
+			ra = rays[2 * size_t(rayIdx)];
+			rb = rays[2 * size_t(rayIdx) + 1];
+		} else {
+			nd = load_node(nodes, uint32_t(next));
+		}
+		if (fresh) {
+			r = dev::make_ray(ra, rb);
+			st.sp = 0;
+			nd = root;
+		}
+		float inL, outL, inR, outR;
+		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
+		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
+		const bool visitL = (outL >= inL) && outL > r.tMin && inL < r.tMax;
+		const bool visitR = (outR >= inR) && outR > r.tMin && inR < r.tMax;
+		const bool innerL = visitL && nd.w3.z == 0;
+		const bool innerR = visitR && nd.w3.w == 0;
+		bool occluded = false;
+		if (visitL && !innerL) occluded = anyhit_leaf(r, tris, child_index(nd.w3.x), nd.w3.z);
+		if (!occluded && visitR && !innerR)
+			occluded = anyhit_leaf(r, tris, child_index(nd.w3.y), nd.w3.w);
+		if (occluded) {
+			next = -1;
+		} else if (innerL && innerR) {
+			st.push(child_index(nd.w3.x));
+			next = int32_t(child_index(nd.w3.y));
+		} else if (innerR) {
+			next = int32_t(child_index(nd.w3.y));
+		} else if (innerL) {
+			next = int32_t(child_index(nd.w3.x));
+		} else {
+			next = st.sp > 0 ? int32_t(st.pop()) : -1;
+		}
+		if (next < 0) {
+			inLight[rayIdx] = occluded ? 0 : 1;
+			rayIdx = kInvalid;
+		}
+	}
+	retire_wave(work);
+}
+
+uint32_t gridFor(uint32_t n) { return n / kBlock + (n % kBlock != 0u); }
+
 }  // namespace
 
-hipError_t launchClosestReferenceOrder(const SceneView& s, const PhxRayIn* rays, uint32_t n,
-                                       PhxRayHit* hits, unsigned long long* counters,
-                                       hipStream_t stream)
+uint32_t persistentSpillWords(uint32_t gridBlocks)
+{
+	return uint32_t(kPersistentStack - kLdsDepth) * gridBlocks * uint32_t(kBlock);
+}
+
+hipError_t persistentBlocksPerCU(uint32_t* closestBlocks, uint32_t* anyhitBlocks)
+{
+	int a = 0, b = 0;
+	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+		&a, reinterpret_cast<const void*>(k_closest_persistent<false>), kBlock, 0);
+	if (e != hipSuccess) return e;
+	e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+		&b, reinterpret_cast<const void*>(k_anyhit_persistent), kBlock, 0);
+	*closestBlocks = uint32_t(a > 0 ? a : 1);
+	*anyhitBlocks = uint32_t(b > 0 ? b : 1);
+	return e;
+}
+
+hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, PhxRayHit* hits,
+                         const TraceLaunch& L, unsigned long long* counters, hipStream_t stream)
 {
 	if (n == 0) return hipSuccess;
-	const uint32_t block = 256;
-	const uint32_t grid = n / block + (n % block != 0u);
+	const float4* nodes = reinterpret_cast<const float4*>(s.nodes);
+	const float4* tris = reinterpret_cast<const float4*>(s.trisPre);
+	const float4* r4 = reinterpret_cast<const float4*>(rays);
+	uint4* h4 = reinterpret_cast<uint4*>(hits);
+	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
+		if (counters)
+			hipLaunchKernelGGL(k_closest_simple<true>, dim3(gridFor(n)), dim3(kBlock), 0, stream,
+			                   nodes, tris, r4, h4, n, counters);
+		else
+			hipLaunchKernelGGL(k_closest_simple<false>, dim3(gridFor(n)), dim3(kBlock), 0, stream,
+			                   nodes, tris, r4, h4, n, counters);
+		return hipGetLastError();
+	}
+	uint32_t grid = L.gridBlocks < gridFor(n) ? L.gridBlocks : gridFor(n);
 	if (counters)
-		hipLaunchKernelGGL(k_closest_reference_order<true>, dim3(grid), dim3(block), 0, stream,
-		                   reinterpret_cast<const float4*>(s.nodes),
-		                   reinterpret_cast<const float4*>(s.tris),
-		                   reinterpret_cast<const float4*>(rays), reinterpret_cast<uint4*>(hits),
-		                   n, counters);
+		hipLaunchKernelGGL(k_closest_persistent<true>, dim3(grid), dim3(kBlock), 0, stream, nodes,
+		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, counters);
 	else
-		hipLaunchKernelGGL(k_closest_reference_order<false>, dim3(grid), dim3(block), 0, stream,
-		                   reinterpret_cast<const float4*>(s.nodes),
-		                   reinterpret_cast<const float4*>(s.tris),
-		                   reinterpret_cast<const float4*>(rays), reinterpret_cast<uint4*>(hits),
-		                   n, counters);
+		hipLaunchKernelGGL(k_closest_persistent<false>, dim3(grid), dim3(kBlock), 0, stream, nodes,
+		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, counters);
 	return hipGetLastError();
 }
 
 hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, uint8_t* inLight,
-                        hipStream_t stream)
+                        const TraceLaunch& L, hipStream_t stream)
 {
 	if (n == 0) return hipSuccess;
-	const uint32_t block = 256;
-	const uint32_t grid = n / block + (n % block != 0u);
-	hipLaunchKernelGGL(k_anyhit, dim3(grid), dim3(block), 0, stream,
-	                   reinterpret_cast<const float4*>(s.nodes),
-	                   reinterpret_cast<const float4*>(s.tris),
-	                   reinterpret_cast<const float4*>(rays), inLight, n);
+	const float4* nodes = reinterpret_cast<const float4*>(s.nodes);
+	const float4* tris = reinterpret_cast<const float4*>(s.trisPre);
+	const float4* r4 = reinterpret_cast<const float4*>(rays);
+	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
+		hipLaunchKernelGGL(k_anyhit_simple, dim3(gridFor(n)), dim3(kBlock), 0, stream, nodes, tris,
+		                   r4, inLight, n);
+		return hipGetLastError();
+	}
+	uint32_t grid = L.gridBlocks < gridFor(n) ? L.gridBlocks : gridFor(n);
+	hipLaunchKernelGGL(k_anyhit_persistent, dim3(grid), dim3(kBlock), 0, stream, nodes, tris, r4,
+	                   inLight, n, L.work, L.spill, L.refillMin);
 	return hipGetLastError();
 }
 

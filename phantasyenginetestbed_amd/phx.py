@@ -38,6 +38,7 @@ MEM_DEVICE = 0x1
 ASYNC = 0x2
 TRAVERSAL_REFERENCE_ORDER = 0x00
 TRAVERSAL_NEAR_FIRST = 0x10
+KERNEL_ONE_RAY_PER_LANE = 0x100
 RAYS_ROW_MAJOR = 0
 RAYS_TILED_8X8 = 1
 
@@ -79,6 +80,7 @@ _SIGNATURES = {
     "phx_device_free": (C.c_int, [_P, _P]),
     "phx_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "phx_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "phx_copy_on_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "phx_synchronize": (C.c_int, [_P]),
     "phx_scene_upload": (C.c_int, [_P, _P, _U32, _P, _P, _U32, _P, _U32, _P, _U32,
                                    C.POINTER(_P)]),
@@ -290,16 +292,16 @@ class Context:
         flags = MEM_DEVICE | mode | (ASYNC if asynchronous else 0)
         check(lib().phx_cast_closest(self.handle, scene.handle, rays_ptr, n, hits_ptr, flags))
 
-    def cast_anyhit(self, scene: Scene, rays: np.ndarray) -> np.ndarray:
+    def cast_anyhit(self, scene: Scene, rays: np.ndarray, mode: int = 0) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY)
         out = np.empty(len(rays), np.uint8)
         check(lib().phx_cast_anyhit(self.handle, scene.handle, _ptr(rays), len(rays),
-                                    _ptr(out), 0))
+                                    _ptr(out), mode))
         return out
 
     def cast_anyhit_device(self, scene: Scene, rays_ptr: int, n: int, out_ptr: int,
-                           asynchronous: bool = False):
-        flags = MEM_DEVICE | (ASYNC if asynchronous else 0)
+                           asynchronous: bool = False, mode: int = 0):
+        flags = MEM_DEVICE | mode | (ASYNC if asynchronous else 0)
         check(lib().phx_cast_anyhit(self.handle, scene.handle, rays_ptr, n, out_ptr, flags))
 
     def traversal_stats(self, scene: Scene, rays: np.ndarray, mode: int = 0) -> dict:
@@ -340,6 +342,9 @@ class Context:
                                 order: int = RAYS_ROW_MAJOR):
         check(lib().phx_gen_primary_rays(self.handle, C.byref(cam), w, h, t_min, t_max, order,
                                          out_ptr, MEM_DEVICE))
+
+    def copy_on_device(self, dst: int, src: int, nbytes: int) -> None:
+        check(lib().phx_copy_on_device(self.handle, dst, src, nbytes))
 
     def synchronize(self) -> None:
         check(lib().phx_synchronize(self.handle))

@@ -51,6 +51,10 @@ def c2(ctx):
     s.release()
 
 
+KERNELS = [0, phx.KERNEL_ONE_RAY_PER_LANE]
+KERNEL_IDS = ["persistent", "one-ray-per-lane"]
+
+
 def assert_same_hits(gpu, ref, what):
     if gpu.tobytes() != ref.tobytes():
         bad = np.nonzero((gpu.view(np.uint8).reshape(-1, 16) !=
@@ -59,30 +63,32 @@ def assert_same_hits(gpu, ref, what):
                              f"{bad[:5].tolist()}: gpu {gpu[bad[:3]]} ref {ref[bad[:3]]}")
 
 
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
 @pytest.mark.parametrize("name", ["rand2k", "hf32", "edges"])
-def test_closest_matches_reference_fixtures(ctx, name):
-    say(f"[gpu] fixture {name}")
+def test_closest_matches_reference_fixtures(ctx, name, kernel):
+    say(f"[gpu] fixture {name} kernel {kernel}")
     fx = load_fixture(name)
     s = ctx.upload_scene(fx["nodes"], fx["tris"])
     try:
-        hits = ctx.cast_closest(s, fx["rays"])
+        hits = ctx.cast_closest(s, fx["rays"], kernel)
         assert_same_hits(hits, fx["hits"], name)
-        st = ctx.traversal_stats(s, fx["rays"])
+        st = ctx.traversal_stats(s, fx["rays"], kernel)
         c = fx["counters"].astype(np.uint64).sum(axis=0)
         assert [st[k] for k in phx.COUNTER_NAMES] == c.tolist()
     finally:
         s.release()
 
 
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
 @pytest.mark.parametrize("name", ["rand2k", "hf32", "edges"])
-def test_anyhit_matches_oracle_on_fixtures(ctx, name):
+def test_anyhit_matches_oracle_on_fixtures(ctx, name, kernel):
     fx = load_fixture(name)
     s = ctx.upload_scene(fx["nodes"], fx["tris"])
     try:
         for tmax in (np.float32(0.3), np.float32(phx.FLT_MAX)):
             rays = fx["rays"].copy()
             rays["tmax"] = tmax
-            lit = ctx.cast_anyhit(s, rays)
+            lit = ctx.cast_anyhit(s, rays, kernel)
             ref = O.cast_anyhit(fx["nodes"], fx["tris"], rays)
             assert (lit == ref).all(), f"{(lit != ref).sum()} rays differ"
     finally:
@@ -140,8 +146,10 @@ def test_c2_primary_bounce_shadow_bit_identical(ctx, c2, golden):
 
     say("[gpu] C2 diffuse bounce")
     bounce = ctx.gen_bounce_rays(s, rays, hits, seed=0xC0FFEE02, frame=0)
-    bhits = ctx.cast_closest(s, bounce)
-    assert_same_hits(bhits, O.cast_closest(b.nodes, b.tris, bounce), "C2 bounce")
+    ref_bounce = O.cast_closest(b.nodes, b.tris, bounce)
+    for kernel in KERNELS:
+        bhits = ctx.cast_closest(s, bounce, kernel)
+        assert_same_hits(bhits, ref_bounce, f"C2 bounce kernel {kernel}")
     # bounce rays: unit length, leave the surface on the side the ray came from
     d = bounce["d"][hitmask].astype(np.float64)
     assert np.abs(np.linalg.norm(d, axis=1) - 1) .max() < 1e-5
@@ -159,23 +167,27 @@ def test_c2_primary_bounce_shadow_bit_identical(ctx, c2, golden):
     sh = np.zeros(len(pos), phx.RAY)
     sh["o"], sh["d"] = pos, (to_l / dist[:, None]).astype(np.float32)
     sh["tmin"], sh["tmax"] = np.float32(1e-3), dist - np.float32(1e-3)
-    lit = ctx.cast_anyhit(s, sh)
-    assert (lit == O.cast_anyhit(b.nodes, b.tris, sh)).all()
+    ref_lit = O.cast_anyhit(b.nodes, b.tris, sh)
+    for kernel in KERNELS:
+        lit = ctx.cast_anyhit(s, sh, kernel)
+        assert (lit == ref_lit).all()
     assert 0.05 < lit.mean() < 0.99
 
 
-def test_counters_match_oracle_on_c2_bounce(ctx, c2):
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
+def test_counters_match_oracle_on_c2_bounce(ctx, c2, kernel):
     b, s = c2
     cam = phx.camera_look_at(*C2_CAM, math.radians(60), 1920, 1080)
     rays = ctx.gen_primary_rays(cam, 480, 270)
     hits = ctx.cast_closest(s, rays)
     bounce = ctx.gen_bounce_rays(s, rays, hits, seed=5, frame=1)
-    st = ctx.traversal_stats(s, bounce)
+    st = ctx.traversal_stats(s, bounce, kernel)
     _, tot, _ = O.cast_debug(b.nodes, b.tris, bounce)
     assert st == tot
 
 
-def test_batch_edges(ctx, c1):
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
+def test_batch_edges(ctx, c1, kernel):
     b, s = c1
     rng = np.random.default_rng(3)
     for n in (0, 1, 63, 65, 1000003):
@@ -184,10 +196,13 @@ def test_batch_edges(ctx, c1):
         d = rng.normal(size=(n, 3))
         rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
         rays["tmin"], rays["tmax"] = np.float32(1e-4), np.float32(phx.FLT_MAX)
-        hits = ctx.cast_closest(s, rays)
+        hits = ctx.cast_closest(s, rays, kernel)
         assert len(hits) == n
         if n:
             assert_same_hits(hits, O.cast_closest(b.nodes, b.tris, rays), f"n={n}")
+        # repeated launches reuse the self-resetting work counter
+        if n:
+            assert ctx.cast_closest(s, rays, kernel).tobytes() == hits.tobytes()
     # empty batch through the device path is a no-op
     ctx.cast_closest_device(s, 0, 0, 0)
     # async launch + explicit synchronisation gives the same result
