@@ -195,6 +195,26 @@ void* phx_context_stream(PhxContext* ctx);
 /* Number of compute units of the context's device. */
 uint32_t phx_context_num_cus(PhxContext* ctx);
 
+/* Launch shape of the persistent traversal kernels on this context: grid size in blocks for
+ * closest-hit and any-hit, block size (threads, a multiple of 64 up to 1024), the number of top
+ * tree nodes each block keeps in LDS (0..2048; the upload lays out the top of the tree breadth
+ * first), and the refill threshold (a wave refills its idle lanes once at least `refillMin` of its
+ * 64 lanes are idle; 1..64). phx_create picks defaults from the device's occupancy; phx_autotune
+ * picks them per scene and ray mix. In phx_set_launch_params a zero field (0xFFFFFFFF for
+ * topNodes) keeps the current value; combinations that do not fit a CU are rejected. Results do
+ * not depend on these parameters. */
+typedef struct PhxLaunchParams {
+	uint32_t gridBlocksClosest;
+	uint32_t gridBlocksAnyhit;
+	uint32_t refillMin;
+	uint32_t blockThreads;
+	uint32_t topNodes;
+	uint32_t residentGridBlocks; /* read only: blocks of this shape the device holds at once */
+} PhxLaunchParams;
+
+phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out);
+phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* params);
+
 /* Device buffers (CudaBuffer.hpp:93-108 equivalents). Allocations are 256-B aligned. */
 phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr);
 phx_status phx_device_free(PhxContext* ctx, void* ptr);

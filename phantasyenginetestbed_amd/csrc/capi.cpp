@@ -2,6 +2,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -93,7 +94,10 @@ struct PhxContext {
 	uint32_t* spill = nullptr;
 	uint32_t gridClosest = 0;
 	uint32_t gridAnyhit = 0;
+	uint32_t blockThreads = 256;
+	uint32_t topNodes = 1024;
 	uint32_t refillMin = 8;
+	uint32_t spillLanes = 0;
 	// Staging buffers for calls that pass host pointers, grown on demand.
 	void* stageIn = nullptr;
 	size_t stageInBytes = 0;
@@ -110,6 +114,8 @@ struct PhxScene {
 	PhxSphereLight* lights = nullptr;
 	uint32_t numNodes = 0, numTris = 0, numMaterials = 0, numLights = 0;
 	uint32_t depth = 0;
+	uint32_t finiteBounds = 0;
+	uint32_t topCount = 0;
 	size_t bytes = 0;
 };
 
@@ -146,13 +152,18 @@ phx_status growStage(void*& buf, size_t& have, size_t need)
 	return PHX_OK;
 }
 
+// Device copy of `count` records followed by `padBytes` zero bytes (kernels that read whole
+// 64-B words may touch up to 64 B past the last triangle).
 template <class T>
-phx_status uploadArray(PhxContext* ctx, T*& dst, const T* src, size_t count, size_t& total)
+phx_status uploadArray(PhxContext* ctx, T*& dst, const T* src, size_t count, size_t& total,
+                       size_t padBytes = 0)
 {
 	dst = nullptr;
 	if (count == 0) return PHX_OK;
 	size_t bytes = count * sizeof(T);
-	hipError_t e = hipMalloc(reinterpret_cast<void**>(&dst), bytes);
+	hipError_t e = hipMalloc(reinterpret_cast<void**>(&dst), bytes + padBytes);
+	if (e == hipSuccess && padBytes)
+		e = hipMemsetAsync(reinterpret_cast<char*>(dst) + bytes, 0, padBytes, ctx->stream);
 	if (e != hipSuccess) {
 		dst = nullptr;
 		return fail(PHX_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes failed", bytes);
@@ -204,17 +215,54 @@ phx_status castCommon(PhxContext* ctx, const PhxRayIn* rays, uint32_t n, void* o
 	return ok();
 }
 
-SceneView view(const PhxScene* s) { return SceneView{s->nodes, s->tris, s->numNodes, s->numTris}; }
+SceneView view(const PhxScene* s)
+{
+	return SceneView{s->nodes, s->tris, s->numNodes, s->numTris, s->finiteBounds, s->topCount};
+}
 
 TraceLaunch traceLaunch(const PhxContext* ctx, uint32_t flags, bool anyhit)
 {
 	TraceLaunch L;
 	L.kernel = (flags & PHX_KERNEL_ONE_RAY_PER_LANE) ? TraceKernel::Simple : TraceKernel::Persistent;
 	L.gridBlocks = anyhit ? ctx->gridAnyhit : ctx->gridClosest;
+	L.blockThreads = ctx->blockThreads;
+	L.topNodes = ctx->topNodes;
 	L.refillMin = ctx->refillMin;
+	L.spillLanes = ctx->spillLanes;
 	L.work = ctx->work;
 	L.spill = ctx->spill;
 	return L;
+}
+
+// Most node visits of incoherent rays happen in the top levels of the tree, which the persistent
+// kernels keep in LDS. The device copy therefore starts with the first kTopCap nodes in
+// breadth-first order (the root stays node 0) followed by the remaining nodes in their original
+// depth-first order; inner-child indices are remapped, leaf (triangle) indices are unchanged, so
+// every traversal decision -- and every hit -- is the same.
+constexpr uint32_t kTopCap = 2048;
+
+std::vector<PhxBVHNode> relayoutTop(const PhxBVHNode* nodes, uint32_t n, uint32_t* topCount)
+{
+	std::vector<uint32_t> bfs;
+	bfs.reserve(std::min(n, kTopCap));
+	bfs.push_back(0);
+	for (size_t h = 0; h < bfs.size() && bfs.size() < kTopCap; h++)
+		for (int side = 0; side < 2 && bfs.size() < kTopCap; side++)
+			if (nodes[bfs[h]].iData[2 + side] == 0) bfs.push_back(uint32_t(nodes[bfs[h]].iData[side]));
+	std::vector<uint32_t> newIndex(n, 0xFFFFFFFFu);
+	for (uint32_t i = 0; i < bfs.size(); i++) newIndex[bfs[i]] = i;
+	uint32_t next = uint32_t(bfs.size());
+	for (uint32_t i = 0; i < n; i++)
+		if (newIndex[i] == 0xFFFFFFFFu) newIndex[i] = next++;
+	std::vector<PhxBVHNode> out(n);
+	for (uint32_t i = 0; i < n; i++) {
+		PhxBVHNode nd = nodes[i];
+		for (int side = 0; side < 2; side++)
+			if (nd.iData[2 + side] == 0) nd.iData[side] = int32_t(newIndex[uint32_t(nd.iData[side])]);
+		out[newIndex[i]] = nd;
+	}
+	*topCount = uint32_t(bfs.size());
+	return out;
 }
 
 // Triangles as (p0, e1 = p1 - p0, e2 = p2 - p0); p0.w keeps the end marker. The subtractions are
@@ -310,15 +358,17 @@ phx_status phx_create(int device, PhxContext** outCtx)
 		delete ctx;
 		return hipFail(e, "hipStreamCreate");
 	}
-	uint32_t bc = 1, ba = 1;
-	e = persistentBlocksPerCU(&bc, &ba);
-	ctx->gridClosest = ctx->numCUs * bc;
-	ctx->gridAnyhit = ctx->numCUs * ba;
-	const uint32_t maxGrid = std::max(ctx->gridClosest, ctx->gridAnyhit);
+	uint32_t bpc = 0;
+	e = persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, &bpc);
+	if (bpc == 0) bpc = 1;
+	ctx->gridClosest = ctx->numCUs * bpc;
+	ctx->gridAnyhit = ctx->numCUs * bpc;
+	// The spill area covers every lane the device can hold (2048 per CU).
+	ctx->spillLanes = ctx->numCUs * 2048u;
 	if (e == hipSuccess) e = hipMalloc(&ctx->work, 64);
 	if (e == hipSuccess) e = hipMemset(ctx->work, 0, 64);
 	if (e == hipSuccess)
-		e = hipMalloc(&ctx->spill, size_t(persistentSpillWords(maxGrid)) * sizeof(uint32_t));
+		e = hipMalloc(&ctx->spill, size_t(persistentSpillWords(ctx->spillLanes)) * sizeof(uint32_t));
 	if (e != hipSuccess) {
 		hipFree(ctx->work);
 		hipFree(ctx->spill);
@@ -354,6 +404,62 @@ uint32_t phx_context_num_cus(PhxContext* ctx)
 {
 	if (!ctx || !registry().has(ctx)) return 0;
 	return ctx->numCUs;
+}
+
+phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!out) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_get_launch_params: out is NULL");
+	out->gridBlocksClosest = ctx->gridClosest;
+	out->gridBlocksAnyhit = ctx->gridAnyhit;
+	out->refillMin = ctx->refillMin;
+	out->blockThreads = ctx->blockThreads;
+	out->topNodes = ctx->topNodes;
+	uint32_t bpc = 0;
+	persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, &bpc);
+	out->residentGridBlocks = ctx->numCUs * bpc;
+	return ok();
+}
+
+phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!p) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: params is NULL");
+	const uint32_t block = p->blockThreads ? p->blockThreads : ctx->blockThreads;
+	const uint32_t top = p->topNodes != 0xFFFFFFFFu ? p->topNodes : ctx->topNodes;
+	if (block % 64 != 0 || block > 1024)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: blockThreads %u", block);
+	if (top > kTopCap)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: topNodes above %u", kTopCap);
+	uint32_t bpc = 0;
+	persistentBlocksPerCU(block, top, &bpc);
+	if (bpc == 0)
+		return fail(PHX_ERR_INVALID_ARGUMENT,
+		            "phx_set_launch_params: %u threads with a %u-node treelet do not fit a CU",
+		            block, top);
+	const uint32_t maxGrid = ctx->spillLanes / block;
+	if (p->gridBlocksClosest > maxGrid || p->gridBlocksAnyhit > maxGrid)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: grid above %u blocks",
+		            maxGrid);
+	if (p->refillMin > 64)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: refillMin above 64");
+	// A new block or treelet size without an explicit grid gets the resident grid of that shape.
+	const bool reshaped = block != ctx->blockThreads || top != ctx->topNodes;
+	const uint32_t resident = std::min(ctx->numCUs * bpc, maxGrid);
+	ctx->blockThreads = block;
+	ctx->topNodes = top;
+	if (p->gridBlocksClosest)
+		ctx->gridClosest = p->gridBlocksClosest;
+	else if (reshaped)
+		ctx->gridClosest = resident;
+	if (p->gridBlocksAnyhit)
+		ctx->gridAnyhit = p->gridBlocksAnyhit;
+	else if (reshaped)
+		ctx->gridAnyhit = resident;
+	if (p->refillMin) ctx->refillMin = p->refillMin;
+	return ok();
 }
 
 phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr)
@@ -447,10 +553,22 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 	s->numMaterials = numMaterials;
 	s->numLights = numLights;
 	s->depth = depth;
-	st = uploadArray(ctx, s->nodes, nodes, numNodes, s->bytes);
+	s->finiteBounds = 1;
+	for (uint32_t i = 0; i < numNodes && s->finiteBounds; i++)
+		for (int k = 0; k < 12; k++)
+			if (!std::isfinite(nodes[i].fData[k])) {
+				s->finiteBounds = 0;
+				break;
+			}
+	{
+		std::vector<PhxBVHNode> laid = relayoutTop(nodes, numNodes, &s->topCount);
+		st = uploadArray(ctx, s->nodes, laid.data(), numNodes, s->bytes);
+		if (st == PHX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+			st = fail(PHX_ERR_HIP, "phx_scene_upload: node upload failed");
+	}
 	if (st == PHX_OK) {
 		std::vector<PhxTriangleVertices> pre = pretransform(triangles, numTriangles);
-		st = uploadArray(ctx, s->tris, pre.data(), numTriangles, s->bytes);
+		st = uploadArray(ctx, s->tris, pre.data(), numTriangles, s->bytes, 64);
 		if (st == PHX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
 			st = fail(PHX_ERR_HIP, "phx_scene_upload: triangle upload failed");
 	}

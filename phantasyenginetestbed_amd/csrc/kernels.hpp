@@ -16,6 +16,8 @@ struct SceneView {
 	const PhxTriangleVertices* trisPre;
 	uint32_t numNodes;
 	uint32_t numTris;
+	uint32_t finiteBounds;  // every node bound is finite (enables the NaN-free slab path)
+	uint32_t topCount;      // nodes [0, topCount) are the top levels in breadth-first order
 };
 
 enum class TraceKernel { Persistent, Simple };
@@ -25,14 +27,21 @@ enum class TraceKernel { Persistent, Simple };
 // persistentSpillWords(gridBlocks) words.
 struct TraceLaunch {
 	TraceKernel kernel = TraceKernel::Persistent;
-	uint32_t gridBlocks = 0;   // persistent grid (blocks of 256 threads)
-	uint32_t refillMin = 8;    // refill a wave once this many of its lanes are idle
+	uint32_t gridBlocks = 0;     // persistent grid
+	uint32_t blockThreads = 256; // persistent block size (multiple of 64, <= 1024)
+	uint32_t topNodes = 0;       // treelet nodes held in LDS (capped by the scene's topCount)
+	uint32_t refillMin = 8;      // refill a wave once this many of its lanes are idle
+	uint32_t spillLanes = 0;     // lanes the spill area is sized for
 	uint32_t* work = nullptr;
 	uint32_t* spill = nullptr;
 };
 
-uint32_t persistentSpillWords(uint32_t gridBlocks);
-hipError_t persistentBlocksPerCU(uint32_t* closestBlocks, uint32_t* anyhitBlocks);
+// Words of the stack spill area for `lanes` resident lanes.
+uint32_t persistentSpillWords(uint32_t lanes);
+// Dynamic LDS bytes of a persistent block.
+uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes);
+// Resident persistent blocks per CU for a block size and treelet size (0 if it does not fit).
+hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t* blocks);
 
 // Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
 // counters of the batch are added to counters[0..4] (device memory).

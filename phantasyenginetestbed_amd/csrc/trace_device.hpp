@@ -25,6 +25,7 @@ struct Ray {
 	float dx, dy, dz;
 	float ix, iy, iz;  // 1 / dir, elementwise IEEE division (Ray::setDir, Ray.hpp:35-37)
 	float tMin, tMax;
+	bool nanFree;      // no slab value of this ray can be NaN (see slab_fast)
 };
 
 __device__ __forceinline__ Ray make_ray(const float4 a, const float4 b)
@@ -35,6 +36,8 @@ __device__ __forceinline__ Ray make_ray(const float4 a, const float4 b)
 	r.ix = 1.0f / b.x;
 	r.iy = 1.0f / b.y;
 	r.iz = 1.0f / b.z;
+	r.nanFree = __builtin_isfinite(r.ix) && __builtin_isfinite(r.iy) && __builtin_isfinite(r.iz) &&
+	            __builtin_isfinite(r.ox) && __builtin_isfinite(r.oy) && __builtin_isfinite(r.oz);
 	return r;
 }
 
@@ -53,6 +56,52 @@ __device__ __forceinline__ void slab(const Ray& r, float mnx, float mny, float m
 	float hx = max_first(t1x, t2x), hy = max_first(t1y, t2y), hz = max_first(t1z, t2z);
 	tIn = max_first(max_first(lx, ly), lz);
 	tOut = min_first(min_first(hx, hy), hz);
+}
+
+// v_min_f32 / v_max_f32 / v_min3_f32 / v_max3_f32 on values known not to be NaN. Written out
+// because the minnum/maxnum intrinsics first canonicalise their inputs (one extra v_max per
+// operand) when the compiler cannot prove them canonical. ALU only: no memory access in asm.
+__device__ __forceinline__ float vmin(float a, float b)
+{
+	float r;
+	asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+	return r;
+}
+__device__ __forceinline__ float vmax(float a, float b)
+{
+	float r;
+	asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+	return r;
+}
+__device__ __forceinline__ float vmin3(float a, float b, float c)
+{
+	float r;
+	asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+	return r;
+}
+__device__ __forceinline__ float vmax3(float a, float b, float c)
+{
+	float r;
+	asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+	return r;
+}
+
+// The same slab test for a ray whose 1/dir and origin are finite. Box bounds are finite (checked at
+// upload), so (b - o) is finite or +-inf and 1/dir is finite and non-zero: no slab value is NaN.
+// Without NaNs, std::min/std::max and the hardware min/max can differ only in the sign of a zero
+// result, and tIn/tOut are only ever compared (-0 == +0), so every visit decision equals
+// slab()'s.
+__device__ __forceinline__ void slab_fast(const Ray& r, float mnx, float mny, float mnz, float mxx,
+                                          float mxy, float mxz, float& tIn, float& tOut)
+{
+	float t1x = (mnx - r.ox) * r.ix;
+	float t1y = (mny - r.oy) * r.iy;
+	float t1z = (mnz - r.oz) * r.iz;
+	float t2x = (mxx - r.ox) * r.ix;
+	float t2y = (mxy - r.oy) * r.iy;
+	float t2z = (mxz - r.oz) * r.iz;
+	tIn = vmax3(vmin(t1x, t2x), vmin(t1y, t2y), vmin(t1z, t2z));
+	tOut = vmin3(vmax(t1x, t2x), vmax(t1y, t2y), vmax(t1z, t2z));
 }
 
 // Moller-Trumbore, Intersection.hpp:24-67 (RTR3 p. 750). Returns false on a miss. dot() is

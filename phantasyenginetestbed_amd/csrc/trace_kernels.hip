@@ -188,19 +188,22 @@ __global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restri
 //
 // A fixed grid of waves pulls rays from a shared counter, one atomic per wavefront for a window
 // of kWindow rays. Every iteration, lanes without a ray are refilled from the window when at least
-// `refillMin` of the wave's 64 lanes are idle (always when all are): the ray loads of the
-// refilled lanes are issued together with the node loads of the busy lanes, and the root node --
-// the first node of every ray -- is held in registers, so a refill costs no extra round trip.
-// The last wave to finish resets the counter, so every launch starts from zero without a memset.
+// `refillMin` of the wave's 64 lanes are idle (always when all are). The last wave to finish resets
+// the counter, so every launch starts from zero without a memset.
 //
-// castRay's pending stack lives in LDS, laid out [wave][entry][lane] (one 4-B column per lane,
-// so each push/pop is a conflict-free 256-B row), kLdsDepth entries deep; deeper entries spill to
-// a global array laid out [entry][lane] (coalesced). Trees are at most 64 levels deep (checked at
-// upload), so the pending stack never exceeds kPersistentStack entries.
+// LDS (dynamic) holds, per block:
+//   - castRay's pending stack, [wave][entry][lane] (one 4-B column per lane, so each push/pop is a
+//     conflict-free 256-B row), kLdsDepth entries deep; deeper entries spill to a global array laid
+//     out [entry][lane] (coalesced). Trees are at most 64 levels deep (checked at upload), so the
+//     pending stack never exceeds kPersistentStack entries;
+//   - the top treelet: nodes [0, topCount) of the device node array, which the upload lays out in
+//     breadth-first order so they are the top levels of the tree -- where most node visits of
+//     incoherent rays happen. They are copied once per block and read from LDS afterwards.
 
 constexpr int kLdsDepth = 16;
 constexpr int kPersistentStack = 72;
 constexpr uint32_t kWindow = 64;
+constexpr int kMaxBlock = 1024;
 
 struct WaveQueue {
 	uint32_t base = 0;   // next unclaimed ray of this wave's window
@@ -256,16 +259,16 @@ __device__ __forceinline__ void retire_wave(uint32_t* work)
 }
 
 struct LaneStack {
-	uint32_t (*lds)[64];  // [entry][lane] of this wave
-	uint32_t* spill;      // this lane's spill column
-	uint32_t stride;      // spill row stride (lanes in the grid)
+	uint32_t* lds;    // this wave's [entry][lane] block
+	uint32_t* spill;  // this lane's spill column
+	uint32_t stride;  // spill row stride (lanes in the grid)
 	uint32_t lane;
 	int sp;
 
 	__device__ __forceinline__ void push(uint32_t v)
 	{
 		if (sp < kLdsDepth)
-			lds[sp][lane] = v;
+			lds[sp * 64 + lane] = v;
 		else if (sp < kPersistentStack)
 			spill[size_t(sp - kLdsDepth) * stride] = v;
 		sp++;
@@ -273,34 +276,107 @@ struct LaneStack {
 	__device__ __forceinline__ uint32_t pop()
 	{
 		sp--;
-		if (sp < kLdsDepth) return lds[sp][lane];
+		if (sp < kLdsDepth) return lds[sp * 64 + lane];
 		if (sp < kPersistentStack) return spill[size_t(sp - kLdsDepth) * stride];
 		return 0u;
 	}
 };
 
-template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_closest_persistent(
-	const float4* __restrict__ nodes, const float4* __restrict__ tris,
-	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
-	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	unsigned long long* __restrict__ counters)
+// Visit decision of both children of a node (BVHTraversal.hpp:59-64; any-hit: tCurrMax = tMax).
+__device__ __forceinline__ void decide(const dev::Ray& r, bool fastSlab, const NodeWords& nd,
+                                       float tCurrMax, bool& visitL, bool& visitR)
 {
-	__shared__ uint32_t ldsStack[kWavesPerBlock][kLdsDepth][64];
+	float inL, outL, inR, outR;
+	if (fastSlab) {
+		dev::slab_fast(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
+		dev::slab_fast(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
+	} else {
+		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
+		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
+	}
+	visitL = (outL >= inL) && outL > r.tMin && inL < tCurrMax;
+	visitR = (outR >= inR) && outR > r.tMin && inR < tCurrMax;
+}
+
+// Per-lane traversal state of the persistent kernels. Every iteration a lane does one unit of
+// work in castRay's per-ray order: expand a node (decide both children, queue the triangles of
+// visited leaf children -- left, then right -- and pick the next node), or test one queued
+// triangle. Nodes are expanded only once the queued triangles are done, so each box decision
+// sees the closest.t that castRay would see.
+struct LaneTrace {
+	uint32_t triCur, triEnd;    // triangles being tested
+	uint32_t pendCur, pendEnd;  // right-leaf triangles queued behind them
+	int32_t next;               // node castRay pops next, or -1
+};
+
+__device__ __forceinline__ void expand(const NodeWords& nd, bool visitL, bool visitR,
+                                       LaneTrace& s, LaneStack& st)
+{
+	const bool innerL = visitL && nd.w3.z == 0;
+	const bool innerR = visitR && nd.w3.w == 0;
+	const bool leafL = visitL && !innerL;
+	const bool leafR = visitR && !innerR;
+	const uint32_t lFirst = child_index(nd.w3.x), rFirst = child_index(nd.w3.y);
+	if (leafL) {
+		s.triCur = lFirst;
+		s.triEnd = lFirst + uint32_t(nd.w3.z);
+		if (leafR) {
+			s.pendCur = rFirst;
+			s.pendEnd = rFirst + uint32_t(nd.w3.w);
+		}
+	} else if (leafR) {
+		s.triCur = rFirst;
+		s.triEnd = rFirst + uint32_t(nd.w3.w);
+	}
+	if (innerL && innerR) {
+		st.push(lFirst);
+		s.next = int32_t(rFirst);
+	} else if (innerR) {
+		s.next = int32_t(rFirst);
+	} else if (innerL) {
+		s.next = int32_t(lFirst);
+	} else {
+		s.next = st.sp > 0 ? int32_t(st.pop()) : -1;
+	}
+}
+
+// LDS bytes of a persistent launch: the stacks of every wave plus the treelet.
+__host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t blockThreads,
+                                                                  uint32_t topCount)
+{
+	return (blockThreads / 64u) * uint32_t(kLdsDepth) * 64u * 4u + topCount * 64u;
+}
+
+// Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Each
+// iteration every busy lane fetches one record -- its next node (64 B, from the LDS treelet or
+// global memory), its next triangle (48 B) or, if it was just refilled, its ray (32 B) -- through
+// the same 128-bit load instructions, so the fetches of a wave go out together.
+template <bool kAnyHit, bool kCount>
+__device__ __forceinline__ void persistent_trace(
+	const float4* __restrict__ nodes, const float4* __restrict__ tris,
+	const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ inLight,
+	uint32_t n, uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
+	bool finiteBounds, uint32_t topCount, unsigned long long* __restrict__ counters)
+{
+	extern __shared__ float4 ldsRaw[];
+	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
+	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsDepth) * 64u / 4u;
+	for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
+	__syncthreads();
+
 	const uint32_t lane = __lane_id();
-	const uint32_t wave = threadIdx.x >> 6;
 	LaneStack st;
-	st.lds = ldsStack[wave];
+	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsDepth) * 64u;
 	st.stride = gridDim.x * blockDim.x;
 	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
 	st.lane = lane;
 	st.sp = 0;
 
-	const NodeWords root = load_node(nodes, 0);
 	WaveQueue q;
 	uint32_t rayIdx = kInvalid;
-	int32_t next = -1;
+	LaneTrace s = {0u, 0u, 0u, 0u, -1};
 	dev::Ray r;
+	bool fastSlab = false;
 	Closest c;
 	Counts k;
 
@@ -318,46 +394,83 @@ __global__ __launch_bounds__(kBlock) void k_closest_persistent(
 		if (q.exhausted && __ballot(rayIdx != kInvalid) == 0) break;
 		if (rayIdx == kInvalid) continue;
 
-		NodeWords nd;
-		float4 ra, rb;
-		if (fresh) {
-			ra = rays[2 * size_t(rayIdx)];
-			rb = rays[2 * size_t(rayIdx) + 1];
+		// A refilled lane only loads and prepares its ray this iteration; its root node then comes
+		// through the same path as every other node. Words 2-3 of a refilled lane come from the root
+		// node instead of past its 32-B ray, so the global loads need no mask and issue back to
+		// back; a triangle's word 3 is the next triangle's first word (the triangle array carries
+		// 64 B of padding). Treelet nodes are read from LDS instead.
+		const bool doTri = !fresh && s.triCur < s.triEnd;
+		const bool inLds = !fresh && !doTri && uint32_t(s.next) < topCount;
+		const float4* src = fresh  ? rays + 2 * size_t(rayIdx)
+		                    : doTri ? tris + 3u * s.triCur
+		                            : nodes + 4u * uint32_t(s.next);
+		const float4* src23 = fresh ? nodes : src;
+		float4 a0, a1, a2, a3;
+		if (inLds) {
+			const float4* t = treelet + 4u * uint32_t(s.next);
+			a0 = t[0];
+			a1 = t[1];
+			a2 = t[2];
+			a3 = t[3];
 		} else {
-			nd = load_node(nodes, uint32_t(next));
-		}
-		if (fresh) {
-			r = dev::make_ray(ra, rb);
-			c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
-			st.sp = 0;
-			nd = root;
+			a0 = src[0];
+			a1 = src[1];
+			a2 = src23[2];
+			a3 = src23[3];
 		}
 
-		if (kCount) k.nodes++;
-		float inL, outL, inR, outR;
-		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
-		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
-		const float tCurrMax = dev::min_first(r.tMax, c.t);
-		const bool visitL = (outL >= inL) && outL > r.tMin && inL < tCurrMax;
-		const bool visitR = (outR >= inR) && outR > r.tMin && inR < tCurrMax;
-		const bool innerL = visitL && nd.w3.z == 0;
-		const bool innerR = visitR && nd.w3.w == 0;
-		if (kCount) k.boxHits += uint32_t(visitL) + uint32_t(visitR);
-		if (visitL && !innerL) closest_leaf<kCount>(r, tris, child_index(nd.w3.x), nd.w3.z, c, k);
-		if (visitR && !innerR) closest_leaf<kCount>(r, tris, child_index(nd.w3.y), nd.w3.w, c, k);
-		if (innerL && innerR) {
-			st.push(child_index(nd.w3.x));
-			next = int32_t(child_index(nd.w3.y));
-		} else if (innerR) {
-			next = int32_t(child_index(nd.w3.y));
-		} else if (innerL) {
-			next = int32_t(child_index(nd.w3.x));
+		bool done = false;
+		if (fresh) {
+			r = dev::make_ray(a0, a1);
+			fastSlab = finiteBounds && r.nanFree;
+			c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+			st.sp = 0;
+			s = {0u, 0u, 0u, 0u, 0};
+		} else if (doTri) {
+			float t, u, v;
+			const bool hit = dev::triangle_pre(r, a0, a1, a2, t, u, v);
+			if (kAnyHit) {
+				done = hit && r.tMin <= t && t < r.tMax;
+				if (done) c.tri = 0u;
+			} else {
+				if (hit && t < c.t && r.tMin <= t && t <= r.tMax) {
+					c.tri = s.triCur;
+					c.t = t;
+					c.u = u;
+					c.v = v;
+					if (kCount) k.accepted++;
+				}
+				if (kCount) k.triTests++;
+			}
+			s.triCur++;
+			if (s.triCur == s.triEnd) {
+				s.triCur = s.pendCur;
+				s.triEnd = s.pendEnd;
+				s.pendCur = s.pendEnd = 0u;
+			}
 		} else {
-			next = st.sp > 0 ? int32_t(st.pop()) : -1;
+			NodeWords nd;
+			nd.w0 = a0;
+			nd.w1 = a1;
+			nd.w2 = a2;
+			nd.w3 = make_int4(__float_as_int(a3.x), __float_as_int(a3.y), __float_as_int(a3.z),
+			                  __float_as_int(a3.w));
+			const float tCurrMax = kAnyHit ? r.tMax : dev::min_first(r.tMax, c.t);
+			bool visitL, visitR;
+			decide(r, fastSlab, nd, tCurrMax, visitL, visitR);
+			if (kCount) {
+				k.nodes++;
+				k.boxHits += uint32_t(visitL) + uint32_t(visitR);
+			}
+			expand(nd, visitL, visitR, s, st);
 		}
-		if (next < 0) {
-			hits[rayIdx] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u),
-			                          __float_as_uint(c.v));
+		if (!done) done = s.triCur >= s.triEnd && s.next < 0;
+		if (done) {
+			if (kAnyHit)
+				inLight[rayIdx] = (c.tri == 0u) ? 0 : 1;
+			else
+				hits[rayIdx] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u),
+				                          __float_as_uint(c.v));
 			rayIdx = kInvalid;
 		}
 	}
@@ -365,104 +478,61 @@ __global__ __launch_bounds__(kBlock) void k_closest_persistent(
 	retire_wave(work);
 }
 
-__global__ __launch_bounds__(kBlock) void k_anyhit_persistent(
+template <bool kCount>
+__global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
-	const float4* __restrict__ rays, uint8_t* __restrict__ inLight, uint32_t n,
-	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin)
+	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
+	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
+	uint32_t finiteBounds, uint32_t topCount, unsigned long long* __restrict__ counters)
 {
-	__shared__ uint32_t ldsStack[kWavesPerBlock][kLdsDepth][64];
-	const uint32_t lane = __lane_id();
-	const uint32_t wave = threadIdx.x >> 6;
-	LaneStack st;
-	st.lds = ldsStack[wave];
-	st.stride = gridDim.x * blockDim.x;
-	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
-	st.lane = lane;
-	st.sp = 0;
-
-	const NodeWords root = load_node(nodes, 0);
-	WaveQueue q;
-	uint32_t rayIdx = kInvalid;
-	int32_t next = -1;
-	dev::Ray r;
-
-	while (true) {
-		const uint64_t idle = __ballot(rayIdx == kInvalid);
-		bool fresh = false;
-		if (!q.exhausted && idle != 0 &&
-		    (uint32_t(__popcll(idle)) >= refillMin || idle == ~0ull)) {
-			const uint32_t got = refill(q, idle, work, n, rayIdx == kInvalid);
-			if (got != kInvalid) {
-				rayIdx = got;
-				fresh = true;
-			}
-		}
-		if (q.exhausted && __ballot(rayIdx != kInvalid) == 0) break;
-		if (rayIdx == kInvalid) continue;
-
-		NodeWords nd;
-		float4 ra, rb;
-		if (fresh) {
-			ra = rays[2 * size_t(rayIdx)];
-			rb = rays[2 * size_t(rayIdx) + 1];
-		} else {
-			nd = load_node(nodes, uint32_t(next));
-		}
-		if (fresh) {
-			r = dev::make_ray(ra, rb);
-			st.sp = 0;
-			nd = root;
-		}
-		float inL, outL, inR, outR;
-		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
-		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
-		const bool visitL = (outL >= inL) && outL > r.tMin && inL < r.tMax;
-		const bool visitR = (outR >= inR) && outR > r.tMin && inR < r.tMax;
-		const bool innerL = visitL && nd.w3.z == 0;
-		const bool innerR = visitR && nd.w3.w == 0;
-		bool occluded = false;
-		if (visitL && !innerL) occluded = anyhit_leaf(r, tris, child_index(nd.w3.x), nd.w3.z);
-		if (!occluded && visitR && !innerR)
-			occluded = anyhit_leaf(r, tris, child_index(nd.w3.y), nd.w3.w);
-		if (occluded) {
-			next = -1;
-		} else if (innerL && innerR) {
-			st.push(child_index(nd.w3.x));
-			next = int32_t(child_index(nd.w3.y));
-		} else if (innerR) {
-			next = int32_t(child_index(nd.w3.y));
-		} else if (innerL) {
-			next = int32_t(child_index(nd.w3.x));
-		} else {
-			next = st.sp > 0 ? int32_t(st.pop()) : -1;
-		}
-		if (next < 0) {
-			inLight[rayIdx] = occluded ? 0 : 1;
-			rayIdx = kInvalid;
-		}
-	}
-	retire_wave(work);
+	persistent_trace<false, kCount>(nodes, tris, rays, hits, nullptr, n, work, spill, refillMin,
+	                                finiteBounds != 0, topCount, counters);
 }
 
-uint32_t gridFor(uint32_t n) { return n / kBlock + (n % kBlock != 0u); }
+__global__ __launch_bounds__(kMaxBlock) void k_anyhit_persistent(
+	const float4* __restrict__ nodes, const float4* __restrict__ tris,
+	const float4* __restrict__ rays, uint8_t* __restrict__ inLight, uint32_t n,
+	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
+	uint32_t finiteBounds, uint32_t topCount)
+{
+	persistent_trace<true, false>(nodes, tris, rays, nullptr, inLight, n, work, spill, refillMin,
+	                              finiteBounds != 0, topCount, nullptr);
+}
+
+uint32_t gridFor(uint32_t n, uint32_t block) { return n / block + (n % block != 0u); }
+
+// Persistent launch shape: grid capped by the work and by the spill area's lane count.
+void persistentShape(const TraceLaunch& L, const SceneView& s, uint32_t n, uint32_t& grid,
+                     uint32_t& block, uint32_t& top, uint32_t& ldsBytes)
+{
+	block = L.blockThreads;
+	top = L.topNodes < s.topCount ? L.topNodes : s.topCount;
+	grid = L.gridBlocks < gridFor(n, block) ? L.gridBlocks : gridFor(n, block);
+	const uint32_t maxGrid = L.spillLanes / block;
+	if (grid > maxGrid) grid = maxGrid;
+	if (grid == 0) grid = 1;
+	ldsBytes = persistent_lds_bytes(block, top);
+}
 
 }  // namespace
 
-uint32_t persistentSpillWords(uint32_t gridBlocks)
+uint32_t persistentSpillWords(uint32_t lanes)
 {
-	return uint32_t(kPersistentStack - kLdsDepth) * gridBlocks * uint32_t(kBlock);
+	return uint32_t(kPersistentStack - kLdsDepth) * lanes;
 }
 
-hipError_t persistentBlocksPerCU(uint32_t* closestBlocks, uint32_t* anyhitBlocks)
+uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes)
 {
-	int a = 0, b = 0;
+	return persistent_lds_bytes(blockThreads, topNodes);
+}
+
+hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t* blocks)
+{
+	int a = 0;
 	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		&a, reinterpret_cast<const void*>(k_closest_persistent<false>), kBlock, 0);
-	if (e != hipSuccess) return e;
-	e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		&b, reinterpret_cast<const void*>(k_anyhit_persistent), kBlock, 0);
-	*closestBlocks = uint32_t(a > 0 ? a : 1);
-	*anyhitBlocks = uint32_t(b > 0 ? b : 1);
+		&a, reinterpret_cast<const void*>(k_closest_persistent<false>), int(blockThreads),
+		persistent_lds_bytes(blockThreads, topNodes));
+	*blocks = uint32_t(a > 0 ? a : 0);
 	return e;
 }
 
@@ -476,20 +546,23 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	uint4* h4 = reinterpret_cast<uint4*>(hits);
 	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
 		if (counters)
-			hipLaunchKernelGGL(k_closest_simple<true>, dim3(gridFor(n)), dim3(kBlock), 0, stream,
-			                   nodes, tris, r4, h4, n, counters);
+			hipLaunchKernelGGL(k_closest_simple<true>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0,
+			                   stream, nodes, tris, r4, h4, n, counters);
 		else
-			hipLaunchKernelGGL(k_closest_simple<false>, dim3(gridFor(n)), dim3(kBlock), 0, stream,
-			                   nodes, tris, r4, h4, n, counters);
+			hipLaunchKernelGGL(k_closest_simple<false>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0,
+			                   stream, nodes, tris, r4, h4, n, counters);
 		return hipGetLastError();
 	}
-	uint32_t grid = L.gridBlocks < gridFor(n) ? L.gridBlocks : gridFor(n);
+	uint32_t grid, block, top, lds;
+	persistentShape(L, s, n, grid, block, top, lds);
 	if (counters)
-		hipLaunchKernelGGL(k_closest_persistent<true>, dim3(grid), dim3(kBlock), 0, stream, nodes,
-		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, counters);
+		hipLaunchKernelGGL(k_closest_persistent<true>, dim3(grid), dim3(block), lds, stream, nodes,
+		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, s.finiteBounds, top,
+		                   counters);
 	else
-		hipLaunchKernelGGL(k_closest_persistent<false>, dim3(grid), dim3(kBlock), 0, stream, nodes,
-		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, counters);
+		hipLaunchKernelGGL(k_closest_persistent<false>, dim3(grid), dim3(block), lds, stream, nodes,
+		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, s.finiteBounds, top,
+		                   counters);
 	return hipGetLastError();
 }
 
@@ -501,13 +574,14 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	const float4* tris = reinterpret_cast<const float4*>(s.trisPre);
 	const float4* r4 = reinterpret_cast<const float4*>(rays);
 	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
-		hipLaunchKernelGGL(k_anyhit_simple, dim3(gridFor(n)), dim3(kBlock), 0, stream, nodes, tris,
-		                   r4, inLight, n);
+		hipLaunchKernelGGL(k_anyhit_simple, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, stream,
+		                   nodes, tris, r4, inLight, n);
 		return hipGetLastError();
 	}
-	uint32_t grid = L.gridBlocks < gridFor(n) ? L.gridBlocks : gridFor(n);
-	hipLaunchKernelGGL(k_anyhit_persistent, dim3(grid), dim3(kBlock), 0, stream, nodes, tris, r4,
-	                   inLight, n, L.work, L.spill, L.refillMin);
+	uint32_t grid, block, top, lds;
+	persistentShape(L, s, n, grid, block, top, lds);
+	hipLaunchKernelGGL(k_anyhit_persistent, dim3(grid), dim3(block), lds, stream, nodes, tris, r4,
+	                   inLight, n, L.work, L.spill, L.refillMin, s.finiteBounds, top);
 	return hipGetLastError();
 }
 

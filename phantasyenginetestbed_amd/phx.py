@@ -52,6 +52,15 @@ class CameraDef(C.Structure):
                         dtype=np.float32)
 
 
+class LaunchParams(C.Structure):
+    _fields_ = [("gridBlocksClosest", C.c_uint32), ("gridBlocksAnyhit", C.c_uint32),
+                ("refillMin", C.c_uint32), ("blockThreads", C.c_uint32),
+                ("topNodes", C.c_uint32), ("residentGridBlocks", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class PhxError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"phx status {status}: {message}")
@@ -76,6 +85,8 @@ _SIGNATURES = {
     "phx_destroy": (None, [_P]),
     "phx_context_stream": (_P, [_P]),
     "phx_context_num_cus": (_U32, [_P]),
+    "phx_get_launch_params": (C.c_int, [_P, C.POINTER(LaunchParams)]),
+    "phx_set_launch_params": (C.c_int, [_P, C.POINTER(LaunchParams)]),
     "phx_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "phx_device_free": (C.c_int, [_P, _P]),
     "phx_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -272,6 +283,18 @@ class Context:
     @property
     def num_cus(self) -> int:
         return lib().phx_context_num_cus(self.handle)
+
+    def launch_params(self) -> dict:
+        p = LaunchParams()
+        check(lib().phx_get_launch_params(self.handle, C.byref(p)))
+        return p.as_dict()
+
+    def set_launch_params(self, grid_closest: int = 0, grid_anyhit: int = 0,
+                          refill_min: int = 0, block_threads: int = 0,
+                          top_nodes: int | None = None) -> None:
+        p = LaunchParams(grid_closest, grid_anyhit, refill_min, block_threads,
+                         0xFFFFFFFF if top_nodes is None else top_nodes, 0)
+        check(lib().phx_set_launch_params(self.handle, C.byref(p)))
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
