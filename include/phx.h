@@ -269,6 +269,48 @@ phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const Phx
                                uint32_t numRays, uint32_t flags, uint64_t counters[5]);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Batched form: many ray batches in one persistent fused launch                               */
+/* ------------------------------------------------------------------------------------------ */
+
+#define PHX_BATCH_CLOSEST 0u /* out is PhxRayHit[numRays] (phx_cast_closest semantics)          */
+#define PHX_BATCH_ANYHIT 1u  /* out is uint8_t[numRays] inLight (phx_cast_anyhit semantics)     */
+
+/* One batch of a plan; rays and out are device pointers. */
+typedef struct PhxBatch {
+	const PhxScene* scene;
+	const PhxRayIn* rays;
+	void* out;
+	uint32_t numRays;
+	uint32_t kind;
+} PhxBatch;
+
+typedef struct PhxPlan PhxPlan;
+
+#define PHX_PLAN_TRACK_COMPLETION 0x10u /* phx_plan_create: record each batch's completion time  */
+#define PHX_PLAN_USE_GRAPH 0x20u        /* phx_plan_launch: replay the launch as a hipGraph      */
+
+/* Replaces the per-batch launches of launchRayCastKernel / launchShadowRayCastKernel
+ * (RayCastKernel.cu:806-886, one launch + device sync each, CudaTracerRenderer.cpp:610-721) with
+ * one persistent launch over all batches (up to 1024 per plan, any mix of closest-hit and any-hit,
+ * on one or more scenes of the context): the batches' rays are pulled from one shared counter, so
+ * the tail of one batch overlaps the work of the others. Per ray the results equal
+ * phx_cast_closest / phx_cast_anyhit. The plan keeps device copies of the batch table; the
+ * scenes and buffers must outlive it. */
+phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t numBatches,
+                           uint32_t flags, PhxPlan** outPlan);
+
+/* Launches the plan on the context stream. PHX_PLAN_USE_GRAPH captures the launch into a hipGraph
+ * the first time and replays it afterwards (the launch leaves its counters zeroed, so replays need
+ * no reset). PHX_ASYNC returns after enqueueing. */
+phx_status phx_plan_launch(PhxContext* ctx, PhxPlan* plan, uint32_t flags);
+
+/* After a tracked launch completed: milliseconds from the launch's first block start to each
+ * batch's completion (numBatches floats). */
+phx_status phx_plan_batch_times(PhxContext* ctx, PhxPlan* plan, float* msSinceStart);
+
+void phx_plan_release(PhxPlan* plan);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Ray generation                                                                             */
 /* ------------------------------------------------------------------------------------------ */
 

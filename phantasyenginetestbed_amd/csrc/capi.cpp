@@ -105,6 +105,20 @@ struct PhxContext {
 	size_t stageOutBytes = 0;
 };
 
+struct PhxPlan {
+	PhxContext* ctx = nullptr;
+	uint32_t numBatches = 0;
+	uint32_t total = 0;
+	phx::BatchDesc* descs = nullptr;
+	uint32_t* offsets = nullptr;
+	phx::BatchClock* clock = nullptr;
+	const float4* treeletNodes = nullptr;
+	uint32_t treeletCount = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graphExec = nullptr;
+	std::vector<phx::BatchDesc> hostDescs;
+};
+
 struct PhxScene {
 	PhxContext* ctx = nullptr;
 	PhxBVHNode* nodes = nullptr;
@@ -721,6 +735,144 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
 	if (e != hipSuccess) return hipFail(e, "phx_gen_bounce_rays");
 	if (e2 != hipSuccess) return hipFail(e2, "phx_gen_bounce_rays");
 	return ok();
+}
+
+phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t numBatches,
+                           uint32_t flags, PhxPlan** outPlan)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!outPlan) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_create: outPlan is NULL");
+	*outPlan = nullptr;
+	if (!batches || numBatches == 0 || numBatches > kMaxFusedBatches)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_create: 1..%u batches", kMaxFusedBatches);
+	std::vector<BatchDesc> descs(numBatches);
+	std::vector<uint32_t> offsets(numBatches + 1, 0);
+	uint64_t total = 0;
+	const PhxScene* firstScene = batches[0].scene;
+	bool oneScene = true;
+	for (uint32_t b = 0; b < numBatches; b++) {
+		const PhxBatch& in = batches[b];
+		st = checkScene(ctx, in.scene);
+		if (st != PHX_OK) return st;
+		if (in.kind != PHX_BATCH_CLOSEST && in.kind != PHX_BATCH_ANYHIT)
+			return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_create: batch %u kind %u", b, in.kind);
+		if (in.numRays && (!in.rays || !in.out))
+			return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_create: batch %u null pointer", b);
+		descs[b] = BatchDesc{reinterpret_cast<const float4*>(in.scene->nodes),
+		                     reinterpret_cast<const float4*>(in.scene->tris),
+		                     reinterpret_cast<const float4*>(in.rays), in.out, in.numRays, in.kind,
+		                     in.scene->finiteBounds, 0u};
+		offsets[b] = uint32_t(total);
+		total += in.numRays;
+		if (total > 0x7FFFFFFFull)
+			return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_create: more than 2^31 rays in a plan");
+		oneScene = oneScene && in.scene == firstScene;
+	}
+	offsets[numBatches] = uint32_t(total);
+	PhxPlan* p = new (std::nothrow) PhxPlan();
+	if (!p) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_plan_create: out of host memory");
+	p->ctx = ctx;
+	p->numBatches = numBatches;
+	p->total = uint32_t(total);
+	p->hostDescs = descs;
+	if (oneScene) {
+		p->treeletNodes = reinterpret_cast<const float4*>(firstScene->nodes);
+		p->treeletCount = firstScene->topCount;
+	}
+	size_t dummy = 0;
+	st = uploadArray(ctx, p->descs, descs.data(), numBatches, dummy);
+	if (st == PHX_OK) st = uploadArray(ctx, p->offsets, offsets.data(), numBatches + 1, dummy);
+	if (st == PHX_OK && (flags & PHX_PLAN_TRACK_COMPLETION)) {
+		BatchClock init;
+		std::memset(&init, 0, sizeof(init));
+		init.start = ~0ull;
+		st = uploadArray(ctx, p->clock, &init, 1, dummy);
+	}
+	if (st == PHX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+		st = fail(PHX_ERR_HIP, "phx_plan_create: upload failed");
+	if (st != PHX_OK) {
+		hipFree(p->descs);
+		hipFree(p->offsets);
+		hipFree(p->clock);
+		delete p;
+		return st;
+	}
+	registry().add(p);
+	*outPlan = p;
+	return ok();
+}
+
+phx_status phx_plan_launch(PhxContext* ctx, PhxPlan* plan, uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!plan || !registry().has(plan) || plan->ctx != ctx)
+		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
+	const TraceLaunch L = traceLaunch(ctx, 0, false);
+	auto launch = [&]() {
+		return launchBatches(plan->descs, plan->offsets, plan->numBatches, plan->total, L,
+		                     plan->treeletNodes, plan->treeletCount, plan->clock, ctx->stream);
+	};
+	if (flags & PHX_PLAN_USE_GRAPH) {
+		if (!plan->graphExec) {
+			PHX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+			        "phx_plan_launch: begin capture");
+			hipError_t e = launch();
+			hipGraph_t g = nullptr;
+			hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+			if (e != hipSuccess) return hipFail(e, "phx_plan_launch: captured launch");
+			if (e2 != hipSuccess) return hipFail(e2, "phx_plan_launch: end capture");
+			plan->graph = g;
+			PHX_HIP(hipGraphInstantiate(&plan->graphExec, g, nullptr, nullptr, 0),
+			        "phx_plan_launch: instantiate");
+		}
+		PHX_HIP(hipGraphLaunch(plan->graphExec, ctx->stream), "phx_plan_launch: graph launch");
+	} else {
+		PHX_HIP(launch(), "phx_plan_launch");
+	}
+	if (!(flags & PHX_ASYNC)) PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_plan_launch");
+	return ok();
+}
+
+phx_status phx_plan_batch_times(PhxContext* ctx, PhxPlan* plan, float* msSinceStart)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!plan || !registry().has(plan) || plan->ctx != ctx)
+		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
+	if (!plan->clock)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_batch_times: plan not tracked");
+	if (!msSinceStart) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_batch_times: NULL output");
+	BatchClock host;
+	PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_plan_batch_times");
+	PHX_HIP(hipMemcpy(&host, plan->clock, sizeof(host), hipMemcpyDeviceToHost),
+	        "phx_plan_batch_times");
+	int rateKHz = 0;
+	PHX_HIP(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, ctx->device),
+	        "phx_plan_batch_times");
+	const double ticksPerMs = double(rateKHz);
+	for (uint32_t b = 0; b < plan->numBatches; b++) {
+		if (plan->hostDescs[b].count == 0) {
+			msSinceStart[b] = 0.0f;
+			continue;
+		}
+		msSinceStart[b] = float(double(host.finish[b] - host.lastStart) / ticksPerMs);
+	}
+	return ok();
+}
+
+void phx_plan_release(PhxPlan* plan)
+{
+	if (!plan || !registry().remove(plan)) return;
+	hipSetDevice(plan->ctx->device);
+	hipStreamSynchronize(plan->ctx->stream);
+	if (plan->graphExec) hipGraphExecDestroy(plan->graphExec);
+	if (plan->graph) hipGraphDestroy(plan->graph);
+	hipFree(plan->descs);
+	hipFree(plan->offsets);
+	hipFree(plan->clock);
+	delete plan;
 }
 
 phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32_t width,

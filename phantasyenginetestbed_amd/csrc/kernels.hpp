@@ -53,6 +53,37 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, uint8_t* inLight,
                         const TraceLaunch& launch, hipStream_t stream);
 
+// Device descriptor of one batch of a fused launch (48 B).
+struct BatchDesc {
+	const float4* nodes;
+	const float4* tris;       // pre-transformed triangles
+	const float4* rays;
+	void* out;                // PhxRayHit* (closest) or uint8_t* (any hit)
+	uint32_t count;
+	uint32_t kind;            // 0 closest hit, 1 any hit
+	uint32_t finiteBounds;
+	uint32_t pad;
+};
+static_assert(sizeof(BatchDesc) == 48, "BatchDesc layout");
+
+// Per-batch completion tracking of a fused launch (device memory).
+struct BatchClock {
+	unsigned long long start;      // earliest block start of the running launch (~0 between launches)
+	unsigned long long lastStart;  // start of the last completed launch
+	uint32_t done[1024];           // rays completed per batch (reset by the launch's last wave)
+	unsigned long long finish[1024];  // device time each batch completed
+};
+
+constexpr uint32_t kMaxFusedBatches = 1024;
+
+uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches);
+
+// One persistent launch over `numBatches` batches; offsets[b] = first ray of batch b in the fused
+// index space, offsets[numBatches] = total. `clock` may be null (no completion tracking).
+hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32_t numBatches,
+                         uint32_t total, const TraceLaunch& launch, const float4* treeletNodes,
+                         uint32_t treeletCount, BatchClock* clock, hipStream_t stream);
+
 // Pinhole primary rays.
 hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float tMin,
                                 float tMax, uint32_t order, PhxRayIn* rays, hipStream_t stream);

@@ -499,6 +499,197 @@ __global__ __launch_bounds__(kMaxBlock) void k_anyhit_persistent(
 	                              finiteBounds != 0, topCount, nullptr);
 }
 
+// ---- fused multi-batch launch ---------------------------------------------------------------------
+//
+// One persistent launch over many ray batches (closest-hit or any-hit, on one or several scenes):
+// the batches' rays form one index space [0, total) pulled from the shared counter as above; a
+// lane maps its ray index to (batch, local index) by binary search over the batch offsets held in
+// LDS. The treelet, if any, belongs to one scene (`treeletNodes`); lanes on other scenes read
+// every node from global memory. With completion tracking on, the lanes completing rays in the
+// same iteration add their count to their batch's counter with one atomic per (wave, batch); the
+// lane that completes a batch stamps its device time, so each batch's latency inside the launch
+// is known. The last wave to retire resets the counters for the next launch.
+
+constexpr uint32_t kMaxBatches = 1024;
+
+__device__ __forceinline__ uint32_t find_batch(const uint32_t* offsets, uint32_t numBatches,
+                                               uint32_t g)
+{
+	// largest b with offsets[b] <= g (offsets[0] = 0, offsets[numBatches] = total > g)
+	uint32_t lo = 0, hi = numBatches;
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (offsets[mid] <= g)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+__global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
+	const BatchDesc* __restrict__ descs, const uint32_t* __restrict__ offsetsGlobal,
+	uint32_t numBatches, uint32_t total, uint32_t* __restrict__ work,
+	uint32_t* __restrict__ spill, uint32_t refillMin, const float4* __restrict__ treeletNodes,
+	uint32_t topCount, BatchClock* __restrict__ clock)
+{
+	extern __shared__ float4 ldsRaw[];
+	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
+	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsDepth) * 64u / 4u;
+	uint32_t* offsets = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
+	if (clock && threadIdx.x == 0) atomicMin(&clock->start, wall_clock64());
+	for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = treeletNodes[i];
+	for (uint32_t i = threadIdx.x; i <= numBatches; i += blockDim.x) offsets[i] = offsetsGlobal[i];
+	__syncthreads();
+
+	LaneStack st;
+	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsDepth) * 64u;
+	st.stride = gridDim.x * blockDim.x;
+	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
+	st.lane = __lane_id();
+	st.sp = 0;
+
+	WaveQueue q;
+	uint32_t rayIdx = kInvalid;  // index within the batch
+	uint32_t batch = 0;
+	LaneTrace s = {0u, 0u, 0u, 0u, -1};
+	dev::Ray r;
+	bool fastSlab = false, anyHit = false, useTreelet = false;
+	const float4* nodes = nullptr;
+	const float4* tris = nullptr;
+	void* out = nullptr;
+	Closest c;
+
+	while (true) {
+		const uint64_t idle = __ballot(rayIdx == kInvalid);
+		bool fresh = false;
+		uint32_t g = kInvalid;
+		if (!q.exhausted && idle != 0 &&
+		    (uint32_t(__popcll(idle)) >= refillMin || idle == ~0ull)) {
+			g = refill(q, idle, work, total, rayIdx == kInvalid);
+			fresh = g != kInvalid;
+		}
+		if (q.exhausted && __ballot(rayIdx != kInvalid || fresh) == 0) break;
+
+		const float4* src;
+		const float4* src23;
+		if (fresh) {
+			batch = find_batch(offsets, numBatches, g);
+			rayIdx = g - offsets[batch];
+			const BatchDesc d = descs[batch];
+			nodes = d.nodes;
+			tris = d.tris;
+			out = d.out;
+			anyHit = d.kind != 0;
+			useTreelet = d.nodes == treeletNodes;
+			fastSlab = d.finiteBounds != 0;
+			src = d.rays + 2 * size_t(rayIdx);
+			src23 = nodes;
+		} else {
+			src = nullptr;
+			src23 = nullptr;
+		}
+		if (rayIdx == kInvalid) continue;
+
+		const bool doTri = !fresh && s.triCur < s.triEnd;
+		const bool inLds = !fresh && !doTri && useTreelet && uint32_t(s.next) < topCount;
+		if (!fresh) {
+			src = doTri ? tris + 3u * s.triCur : nodes + 4u * uint32_t(s.next);
+			src23 = src;
+		}
+		float4 a0, a1, a2, a3;
+		if (inLds) {
+			const float4* t = treelet + 4u * uint32_t(s.next);
+			a0 = t[0];
+			a1 = t[1];
+			a2 = t[2];
+			a3 = t[3];
+		} else {
+			a0 = src[0];
+			a1 = src[1];
+			a2 = src23[2];
+			a3 = src23[3];
+		}
+
+		bool done = false;
+		if (fresh) {
+			r = dev::make_ray(a0, a1);
+			fastSlab = fastSlab && r.nanFree;
+			c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+			st.sp = 0;
+			s = {0u, 0u, 0u, 0u, 0};
+		} else if (doTri) {
+			float t, u, v;
+			const bool hit = dev::triangle_pre(r, a0, a1, a2, t, u, v);
+			if (anyHit) {
+				done = hit && r.tMin <= t && t < r.tMax;
+				if (done) c.tri = 0u;
+			} else if (hit && t < c.t && r.tMin <= t && t <= r.tMax) {
+				c.tri = s.triCur;
+				c.t = t;
+				c.u = u;
+				c.v = v;
+			}
+			s.triCur++;
+			if (s.triCur == s.triEnd) {
+				s.triCur = s.pendCur;
+				s.triEnd = s.pendEnd;
+				s.pendCur = s.pendEnd = 0u;
+			}
+		} else {
+			NodeWords nd;
+			nd.w0 = a0;
+			nd.w1 = a1;
+			nd.w2 = a2;
+			nd.w3 = make_int4(__float_as_int(a3.x), __float_as_int(a3.y), __float_as_int(a3.z),
+			                  __float_as_int(a3.w));
+			const float tCurrMax = anyHit ? r.tMax : dev::min_first(r.tMax, c.t);
+			bool visitL, visitR;
+			decide(r, fastSlab, nd, tCurrMax, visitL, visitR);
+			expand(nd, visitL, visitR, s, st);
+		}
+		if (!done) done = s.triCur >= s.triEnd && s.next < 0;
+		if (done) {
+			if (anyHit)
+				static_cast<uint8_t*>(out)[rayIdx] = (c.tri == 0u) ? 0 : 1;
+			else
+				static_cast<uint4*>(out)[rayIdx] = make_uint4(
+					c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
+			rayIdx = kInvalid;
+		}
+		if (clock) {
+			uint64_t pending = __ballot(done);
+			while (pending) {
+				const int leader = __ffsll((unsigned long long)pending) - 1;
+				const uint32_t lb = __shfl(batch, leader);
+				const uint64_t same = __ballot(done && batch == lb);
+				if (int(__lane_id()) == leader) {
+					const uint32_t cnt = uint32_t(__popcll(same));
+					const uint32_t before = atomicAdd(&clock->done[lb], cnt);
+					if (before + cnt == offsets[lb + 1] - offsets[lb])
+						atomicExch(&clock->finish[lb], wall_clock64());
+				}
+				pending &= ~same;
+			}
+		}
+	}
+	// The last wave resets the shared counters (and, with tracking, the batch counters).
+	if (__lane_id() == 0) {
+		__threadfence();
+		const uint32_t totalWaves = gridDim.x * (blockDim.x / 64u);
+		const bool last = atomicAdd(&work[1], 1u) == totalWaves - 1u;
+		if (last) {
+			atomicExch(&work[0], 0u);
+			atomicExch(&work[1], 0u);
+			if (clock) {
+				for (uint32_t b = 0; b < numBatches; b++) atomicExch(&clock->done[b], 0u);
+				clock->lastStart = clock->start;
+				atomicExch(&clock->start, ~0ull);
+			}
+		}
+	}
+}
+
 uint32_t gridFor(uint32_t n, uint32_t block) { return n / block + (n % block != 0u); }
 
 // Persistent launch shape: grid capped by the work and by the spill area's lane count.
@@ -582,6 +773,27 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	persistentShape(L, s, n, grid, block, top, lds);
 	hipLaunchKernelGGL(k_anyhit_persistent, dim3(grid), dim3(block), lds, stream, nodes, tris, r4,
 	                   inLight, n, L.work, L.spill, L.refillMin, s.finiteBounds, top);
+	return hipGetLastError();
+}
+
+uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches)
+{
+	return persistent_lds_bytes(blockThreads, topNodes) + (numBatches + 1u) * 4u;
+}
+
+hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32_t numBatches,
+                         uint32_t total, const TraceLaunch& L, const float4* treeletNodes,
+                         uint32_t treeletCount, BatchClock* clock, hipStream_t stream)
+{
+	if (total == 0) return hipSuccess;
+	const uint32_t block = L.blockThreads;
+	const uint32_t top = L.topNodes < treeletCount ? L.topNodes : treeletCount;
+	uint32_t grid = L.gridBlocks < gridFor(total, block) ? L.gridBlocks : gridFor(total, block);
+	if (grid > L.spillLanes / block) grid = L.spillLanes / block;
+	if (grid == 0) grid = 1;
+	hipLaunchKernelGGL(k_trace_batches, dim3(grid), dim3(block),
+	                   batchLdsBytes(block, top, numBatches), stream, descs, offsets, numBatches,
+	                   total, L.work, L.spill, L.refillMin, treeletNodes, top, clock);
 	return hipGetLastError();
 }
 

@@ -39,6 +39,10 @@ ASYNC = 0x2
 TRAVERSAL_REFERENCE_ORDER = 0x00
 TRAVERSAL_NEAR_FIRST = 0x10
 KERNEL_ONE_RAY_PER_LANE = 0x100
+BATCH_CLOSEST = 0
+BATCH_ANYHIT = 1
+PLAN_TRACK_COMPLETION = 0x10
+PLAN_USE_GRAPH = 0x20
 RAYS_ROW_MAJOR = 0
 RAYS_TILED_8X8 = 1
 
@@ -59,6 +63,11 @@ class LaunchParams(C.Structure):
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Batch(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("rays", C.c_void_p), ("out", C.c_void_p),
+                ("numRays", C.c_uint32), ("kind", C.c_uint32)]
 
 
 class PhxError(RuntimeError):
@@ -101,6 +110,10 @@ _SIGNATURES = {
     "phx_cast_anyhit": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_traversal_stats": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(C.c_uint64)]),
     "phx_gen_bounce_rays": (C.c_int, [_P, _P, _P, _P, _U32, C.c_uint64, _U32, _P, _U32]),
+    "phx_plan_create": (C.c_int, [_P, C.POINTER(Batch), _U32, _U32, C.POINTER(_P)]),
+    "phx_plan_launch": (C.c_int, [_P, _P, _U32]),
+    "phx_plan_batch_times": (C.c_int, [_P, _P, _PF]),
+    "phx_plan_release": (None, [_P]),
     "phx_camera_def": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
     "phx_camera_look_at": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
     "phx_gen_primary_rays": (C.c_int, [_P, C.POINTER(CameraDef), _U32, _U32, _F32, _F32, _U32,
@@ -267,6 +280,36 @@ class Scene:
     def release(self) -> None:
         if self.handle:
             lib().phx_scene_release(self.handle)
+            self.handle = None
+
+
+class Plan:
+    """A fused multi-batch launch (phx_plan_*). `batches` = [(scene, rays_ptr, out_ptr, n, kind)]."""
+
+    def __init__(self, ctx: "Context", batches, track: bool = False):
+        self.ctx = ctx
+        arr = (Batch * len(batches))()
+        for i, (scene, rays_ptr, out_ptr, n, kind) in enumerate(batches):
+            arr[i] = Batch(scene.handle, rays_ptr, out_ptr, n, kind)
+        h = C.c_void_p()
+        check(lib().phx_plan_create(ctx.handle, arr, len(batches),
+                                    PLAN_TRACK_COMPLETION if track else 0, C.byref(h)))
+        self.handle = h.value
+        self.num_batches = len(batches)
+
+    def launch(self, graph: bool = False, asynchronous: bool = False) -> None:
+        flags = (PLAN_USE_GRAPH if graph else 0) | (ASYNC if asynchronous else 0)
+        check(lib().phx_plan_launch(self.ctx.handle, self.handle, flags))
+
+    def batch_times_ms(self) -> np.ndarray:
+        out = np.zeros(self.num_batches, np.float32)
+        check(lib().phx_plan_batch_times(self.ctx.handle, self.handle,
+                                         out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            lib().phx_plan_release(self.handle)
             self.handle = None
 
 

@@ -229,3 +229,57 @@ def test_upload_rejects_malformed_bvh(ctx):
     with pytest.raises(phx.PhxError) as e:
         ctx.upload_scene(nodes, fx["tris"])
     assert e.value.status == phx.ERR_MALFORMED_BVH
+
+
+def test_fused_plan_matches_per_batch_results(ctx, c1, c2):
+    """Batches of both kinds on two scenes in one persistent launch give exactly the per-batch
+    results; graph replays (with host work between them) repeat them; empty batches are fine."""
+    say("[gpu] fused plan")
+    b1, s1 = c1
+    b2, s2 = c2
+    rng = np.random.default_rng(11)
+    specs = []  # (scene, bvh, n, kind)
+    for i in range(40):
+        scene, bvh = (s1, b1) if i % 3 else (s2, b2)
+        n = int(rng.integers(0, 20000)) if i % 7 else 0
+        specs.append((scene, bvh, n, phx.BATCH_ANYHIT if i % 4 == 0 else phx.BATCH_CLOSEST))
+    bufs, batches, inputs = [], [], []
+    for scene, bvh, n, kind in specs:
+        rays = np.zeros(n, phx.RAY)
+        rays["o"] = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+        d = rng.normal(size=(n, 3))
+        rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        rays["tmin"] = np.float32(1e-4)
+        rays["tmax"] = np.float32(0.3) if kind == phx.BATCH_ANYHIT else np.float32(phx.FLT_MAX)
+        rb = ctx.alloc(max(n, 1) * 32)
+        if n:
+            rb.upload(rays)
+        ob = ctx.alloc(max(n, 1) * 16)
+        bufs += [rb, ob]
+        batches.append((scene, rb.ptr, ob.ptr, n, kind))
+        inputs.append((scene, bvh, rays, kind, ob))
+    plan = phx.Plan(ctx, batches, track=True)
+    try:
+        for graph in (False, True, True, True):
+            plan.launch(graph=graph)
+            for scene, bvh, rays, kind, ob in inputs:
+                n = len(rays)
+                if not n:
+                    continue
+                if kind == phx.BATCH_ANYHIT:
+                    got = ob.download(np.uint8, n)
+                    assert (got == ctx.cast_anyhit(scene, rays)).all()
+                else:
+                    got = ob.download(phx.HIT, n)
+                    assert_same_hits(got, ctx.cast_closest(scene, rays), "fused")
+            times = plan.batch_times_ms()
+            ns = np.array([len(x[2]) for x in inputs])
+            assert (times[ns > 0] > 0).all() and (times[ns == 0] == 0).all()
+        # one batch checked against the oracle as well
+        scene, bvh, rays, kind, ob = next(x for x in inputs if len(x[2]) and x[3] == 0)
+        assert_same_hits(ob.download(phx.HIT, len(rays)), O.cast_closest(bvh.nodes, bvh.tris, rays),
+                         "fused vs oracle")
+    finally:
+        plan.release()
+        for b in bufs:
+            b.free()
