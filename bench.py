@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
 """Incoherent closest-hit throughput on one MI355X.
 
-Workload: the 1,004,000-triangle heightfield scene (700x700 fBm grid + 2,000 boxes, seed
-0xC0FFEE02), SBVH built with the reference's builder semantics. 1920x1080 primary rays from
-(0.5, 0.6, -0.3) towards (0.5, 0.05, 0.6), 60 deg; every primary hit spawns one cosine-weighted
-diffuse bounce ray (tMin 1e-3). A step is one closest-hit pass over all bounce rays (device
-resident). Prints one JSON line.
+Workload (the 1M-triangle heightfield configuration): 700x700 fBm heightfield + 2,000 boxes =
+1,004,000 triangles (seed 0xC0FFEE02), SBVH built with the reference builder's semantics;
+1920x1080 primary rays from (0.5, 0.6, -0.3) towards (0.5, 0.05, 0.6), 60 deg; one cosine-
+weighted diffuse bounce per primary hit (tMin 1e-3). A step is one closest-hit launch over all
+bounce rays (device resident). The launch shape is picked by the library's autotuner first.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Also reported (same GPU, same scene): the batched form -- 256 ray batches (sizes lognormal,
+median 32k, 1k..2M; 40% primary / 40% diffuse / 20% shadow) as 256 plain launches, as one fused
+persistent launch, and as its hipGraph replay, with identical outputs -- and the CPU baseline
+(the reference's castRay compiled from its sources, on all host cores, bounded sample).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -27,9 +31,6 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s incoherent closest-hit @1M tris; p50/p99 ray-batch latency"
 HBM_PEAK_GBS = 8000.0
-CAMERA = ((0.5, 0.6, -0.3), (0.5, 0.05, 0.6), (0.0, 1.0, 0.0))
-SCENE_SEED = 0xC0FFEE02
-BOUNCE_SEED = 0xC0FFEE02
 
 
 def log(msg: str) -> None:
@@ -107,6 +108,77 @@ def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
                       f" on {threads} threads ({cpu_model}); hits identical to GPU: {same}"}
 
 
+def batched_form(ctx, scene, pools, ev, reps: int = 3) -> dict:
+    from phantasyenginetestbed_amd import workloads as W
+    bs = W.batch_set(pools)
+    W.upload_batches(ctx, bs)
+    stream = ctx.stream
+    sizes = np.array([len(r) for r in bs.rays])
+    log(f"batched form: {len(sizes)} batches, {bs.total} rays")
+
+    # plain: one launch per batch, each timed with its own events
+    evs = ev.create(len(sizes) + 1)
+    W.launch_plain(ctx, scene, bs)
+    ctx.synchronize()
+    ref = W.download_outputs(bs)
+    plain_best, plain_lat = None, None
+    for _ in range(reps):
+        ev.record(evs[0], stream)
+        for i, (k, r, (rb, ob)) in enumerate(zip(bs.kinds, bs.rays, bs.device)):
+            if k == "shadow":
+                ctx.cast_anyhit_device(scene, rb.ptr, len(r), ob.ptr, asynchronous=True)
+            else:
+                ctx.cast_closest_device(scene, rb.ptr, len(r), ob.ptr, asynchronous=True)
+            ev.record(evs[i + 1], stream)
+        ctx.synchronize()
+        lat = np.array([ev.elapsed_ms(evs[i], evs[i + 1]) for i in range(len(sizes))])
+        tot = ev.elapsed_ms(evs[0], evs[-1])
+        if plain_best is None or tot < plain_best:
+            plain_best, plain_lat = tot, lat
+    ev.destroy(evs)
+
+    plan = W.plan_for(ctx, scene, bs, track=True)
+    tuned = plan.autotune()
+    e2 = ev.create(2)
+
+    def timed(graph: bool) -> float:
+        best = None
+        plan.launch(graph=graph)
+        for _ in range(reps):
+            ev.record(e2[0], stream)
+            plan.launch(graph=graph, asynchronous=True)
+            ev.record(e2[1], stream)
+            ms = ev.elapsed_ms(e2[0], e2[1])
+            best = ms if best is None else min(best, ms)
+        return best
+
+    fused_ms = timed(False)
+    fused_lat = plan.batch_times_ms()
+    same_fused = all(a.tobytes() == b.tobytes() for a, b in zip(ref, W.download_outputs(bs)))
+    graph_ms = timed(True)
+    same_graph = all(a.tobytes() == b.tobytes() for a, b in zip(ref, W.download_outputs(bs)))
+    ev.destroy(e2)
+    plan.release()
+    W.free_batches(bs)
+    return {
+        "batches": int(len(sizes)), "rays": int(bs.total),
+        "mix": {k: int(sum(len(r) for kk, r in zip(bs.kinds, bs.rays) if kk == k))
+                for k in ("primary", "diffuse", "shadow")},
+        "plain_ms": round(plain_best, 3), "plain_mrays": round(bs.total / plain_best / 1e3, 1),
+        "plain_batch_p50_ms": round(float(np.percentile(plain_lat, 50)), 4),
+        "plain_batch_p99_ms": round(float(np.percentile(plain_lat, 99)), 4),
+        "fused_ms": round(fused_ms, 3), "fused_mrays": round(bs.total / fused_ms / 1e3, 1),
+        "fused_graph_ms": round(graph_ms, 3),
+        "fused_graph_mrays": round(bs.total / graph_ms / 1e3, 1),
+        "fused_batch_completion_p50_ms": round(float(np.percentile(fused_lat, 50)), 4),
+        "fused_batch_completion_p99_ms": round(float(np.percentile(fused_lat, 99)), 4),
+        "fused_vs_plain": round(plain_best / fused_ms, 2),
+        "identical_outputs": bool(same_fused and same_graph),
+        "fused_launch": {k: tuned[k] for k in ("gridBlocksClosest", "blockThreads", "topNodes",
+                                               "refillMin")},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,45 +187,36 @@ def main() -> int:
     ap.add_argument("--grid", type=int, default=700)
     ap.add_argument("--boxes", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane"], default="persistent")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane"], default=None,
+                    help="force a launch model (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (a plain copy kernel "
                          "of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard with "
-              "no exchange, see DESIGN.md; this build runs on a single MI355X)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the rays of a batch "
+              "shard across GPUs with no exchange step (each GPU holds the scene), and this build "
+              "runs on a single MI355X", file=sys.stderr)
         return 2
 
-    from phantasyenginetestbed_amd import phx
+    from phantasyenginetestbed_amd import phx, workloads as W
 
     log(f"scene {args.grid}x{args.grid} heightfield + {args.boxes} boxes")
-    tris, datas = phx.heightfield_boxes(args.grid, args.boxes, SCENE_SEED)
     t0 = time.perf_counter()
-    bvh = phx.BVH(tris, datas)
+    tris, datas, bvh = W.heightfield_scene(args.grid, args.boxes)
     log(f"SBVH: {len(tris)} tris -> {len(bvh.nodes)} nodes, {len(bvh.tris)} refs, "
         f"depth {bvh.depth()}, {time.perf_counter() - t0:.2f}s")
 
     ctx = phx.Context(0)
     scene = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas)
-    W, H = 1920, 1080
-    n_px = W * H
-    cam = phx.camera_look_at(*CAMERA, math.radians(60.0), W, H)
-    prim = ctx.alloc(n_px * 32)
-    phit = ctx.alloc(n_px * 16)
-    bnc = ctx.alloc(n_px * 32)
-    ctx.gen_primary_rays_device(cam, W, H, prim.ptr)
-    ctx.cast_closest_device(scene, prim.ptr, n_px, phit.ptr)
-    ctx.gen_bounce_rays_device(scene, prim.ptr, phit.ptr, n_px, BOUNCE_SEED, 0, bnc.ptr)
-    hits = phit.download(phx.HIT, n_px)
-    bounce_all = bnc.download(phx.RAY, n_px)
-    bounce = np.ascontiguousarray(bounce_all[hits["tri"] != phx.MISS])
+    pools = W.ray_pools(ctx, scene)
+    bounce = pools.bounce
     n = len(bounce)
     rays = ctx.alloc(n * 32).upload(bounce)
     out = ctx.alloc(n * 16)
-    for b in (prim, phit, bnc):
-        b.free()
-    log(f"{n} bounce rays from {n_px} primary rays")
+    log(f"{n} bounce rays from {len(pools.primary)} primary rays")
 
     counters = ctx.traversal_stats_device(scene, rays.ptr, n)
     nodes_per_ray = counters["nodesVisited"] / n
@@ -170,7 +233,13 @@ def main() -> int:
         dst.free()
         log(f"calibration copy of {args.calibrate_copy} MiB done")
 
-    mode = phx.KERNEL_ONE_RAY_PER_LANE if args.kernel == "one-ray-per-lane" else 0
+    launch = {"kernel": "persistent", **ctx.launch_params()}
+    if not args.no_autotune:
+        launch = ctx.autotune(scene, rays.ptr, n)
+        log(f"autotuned launch: {launch}")
+    kernel = args.kernel or launch["kernel"]
+    mode = phx.KERNEL_ONE_RAY_PER_LANE if kernel == "one-ray-per-lane" else 0
+
     ev = HipEvents()
     stream = ctx.stream
     for _ in range(args.warmup):
@@ -200,11 +269,13 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "diffuse-bounce closest-hit, 1,004,000-tri heightfield+boxes "
-                               "SBVH, rays from 1920x1080 primary hits",
+                               "SBVH, rays from 1920x1080 primary hits, one launch per step",
                    "scene_tris": int(len(tris)), "sbvh_nodes": int(len(bvh.nodes)),
                    "sbvh_refs": int(len(bvh.tris)), "rays_per_step": int(n),
                    "traversal": "reference-order (bit-identical to castRay)",
-                   "kernel": args.kernel},
+                   "kernel": kernel,
+                   "launch": {k: launch[k] for k in ("gridBlocksClosest", "blockThreads",
+                                                     "topNodes", "refillMin") if k in launch}},
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -212,6 +283,9 @@ def main() -> int:
                      "traffic": None,
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
+    if not args.no_batched:
+        result["batched"] = batched_form(ctx, scene, pools, ev)
+        log(f"batched: {result['batched']}")
     if not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(bvh.nodes, bvh.tris, bounce, gpu_hits)

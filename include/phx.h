@@ -183,6 +183,7 @@ phx_status phx_bvh_validate(const PhxBVHNode* nodes, uint32_t numNodes,
 /* ------------------------------------------------------------------------------------------ */
 
 typedef struct PhxContext PhxContext;
+typedef struct PhxScene PhxScene;
 
 /* Replaces the CudaTracerRenderer constructor's device setup (CudaTracerRenderer.cpp:123-167):
  * binds `device`, reads its properties, creates the context stream. No device reset on destroy. */
@@ -215,6 +216,17 @@ typedef struct PhxLaunchParams {
 phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out);
 phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* params);
 
+/* Autotuner for single-batch casts: times phx_cast_closest (or, with PHX_AUTOTUNE_ANYHIT,
+ * phx_cast_anyhit) on the given device rays under candidate launch shapes -- block 256/512,
+ * treelet 0/512/1024 nodes, 1/4, 1/2 or all of the resident grid, refill 8/32, and the
+ * one-ray-per-lane kernel -- with HIP events, and applies the fastest persistent shape to the
+ * context. `chosen` (may be NULL) receives it; residentGridBlocks = 0xFFFFFFFF there means the
+ * one-ray-per-lane kernel won (pass PHX_KERNEL_ONE_RAY_PER_LANE to the casts). `rays` must be
+ * device memory (PHX_MEM_DEVICE). */
+#define PHX_AUTOTUNE_ANYHIT 0x1000u
+phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                        uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen);
+
 /* Device buffers (CudaBuffer.hpp:93-108 equivalents). Allocations are 256-B aligned. */
 phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr);
 phx_status phx_device_free(PhxContext* ctx, void* ptr);
@@ -227,8 +239,6 @@ phx_status phx_synchronize(PhxContext* ctx);
 /* Static scene upload (replaces CudaTracerRenderer::setStaticScene, CudaTracerRenderer.cpp:251-   */
 /* 270, and setMaterialsAndTextures' material upload, :192-239)                                */
 /* ------------------------------------------------------------------------------------------ */
-
-typedef struct PhxScene PhxScene;
 
 /* Copies the three SBVH arrays (+ optional materials and lights) into device memory owned by the
  * returned scene. The BVH is validated first (PHX_ERR_MALFORMED_BVH). `datas`, `materials` and
@@ -307,6 +317,16 @@ phx_status phx_plan_launch(PhxContext* ctx, PhxPlan* plan, uint32_t flags);
 /* After a tracked launch completed: milliseconds from the launch's first block start to each
  * batch's completion (numBatches floats). */
 phx_status phx_plan_batch_times(PhxContext* ctx, PhxPlan* plan, float* msSinceStart);
+
+/* Launch shape of one plan (a fused launch is throughput bound, so plans default to the resident
+ * grid without a treelet). gridBlocksClosest is the grid; gridBlocksAnyhit is ignored. */
+phx_status phx_plan_set_launch_params(PhxContext* ctx, PhxPlan* plan,
+                                      const PhxLaunchParams* params);
+
+/* Times the plan under a set of candidate shapes (block 256/512, treelet 0/512/1024 nodes, 1/4,
+ * 1/2 or all of the resident grid, refill 8/32) and keeps the fastest. Runs the plan's batches
+ * several times (outputs are overwritten with the same values). */
+phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* chosen);
 
 void phx_plan_release(PhxPlan* plan);
 

@@ -107,6 +107,7 @@ struct PhxContext {
 
 struct PhxPlan {
 	PhxContext* ctx = nullptr;
+	phx::TraceLaunch launch;  // launch shape of this plan (set at creation, tunable)
 	uint32_t numBatches = 0;
 	uint32_t total = 0;
 	phx::BatchDesc* descs = nullptr;
@@ -775,6 +776,13 @@ phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t nu
 	p->ctx = ctx;
 	p->numBatches = numBatches;
 	p->total = uint32_t(total);
+	// A fused launch is throughput bound: default to the resident grid without a treelet.
+	p->launch = traceLaunch(ctx, 0, false);
+	p->launch.topNodes = 0;
+	uint32_t bpc = 0;
+	persistentBlocksPerCU(p->launch.blockThreads, 0, &bpc);
+	p->launch.gridBlocks = std::max(1u, std::min(ctx->numCUs * bpc,
+	                                             ctx->spillLanes / p->launch.blockThreads));
 	p->hostDescs = descs;
 	if (oneScene) {
 		p->treeletNodes = reinterpret_cast<const float4*>(firstScene->nodes);
@@ -809,7 +817,7 @@ phx_status phx_plan_launch(PhxContext* ctx, PhxPlan* plan, uint32_t flags)
 	if (st != PHX_OK) return st;
 	if (!plan || !registry().has(plan) || plan->ctx != ctx)
 		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
-	const TraceLaunch L = traceLaunch(ctx, 0, false);
+	const TraceLaunch L = plan->launch;
 	auto launch = [&]() {
 		return launchBatches(plan->descs, plan->offsets, plan->numBatches, plan->total, L,
 		                     plan->treeletNodes, plan->treeletCount, plan->clock, ctx->stream);
@@ -859,6 +867,189 @@ phx_status phx_plan_batch_times(PhxContext* ctx, PhxPlan* plan, float* msSinceSt
 		}
 		msSinceStart[b] = float(double(host.finish[b] - host.lastStart) / ticksPerMs);
 	}
+	return ok();
+}
+
+}  // extern "C"
+
+namespace {
+
+// Checks a launch shape against the device (block size, LDS, spill area); fills `L`.
+phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, bool plan)
+{
+	const uint32_t block = p.blockThreads;
+	if (block == 0 || block % 64 != 0 || block > 1024)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: blockThreads %u", block);
+	if (p.topNodes > kTopCap)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: topNodes above %u", kTopCap);
+	uint32_t bpc = 0;
+	persistentBlocksPerCU(block, p.topNodes, &bpc);
+	if (bpc == 0) return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: shape does not fit a CU");
+	const uint32_t maxGrid = ctx->spillLanes / block;
+	const uint32_t grid = plan ? p.gridBlocksClosest : p.gridBlocksClosest;
+	if (grid == 0 || grid > maxGrid)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: grid %u outside 1..%u", grid, maxGrid);
+	if (p.refillMin == 0 || p.refillMin > 64)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: refillMin %u", p.refillMin);
+	L.blockThreads = block;
+	L.topNodes = p.topNodes;
+	L.gridBlocks = grid;
+	L.refillMin = p.refillMin;
+	L.spillLanes = ctx->spillLanes;
+	L.work = ctx->work;
+	L.spill = ctx->spill;
+	return PHX_OK;
+}
+
+struct Candidate {
+	TraceLaunch launch;
+	PhxLaunchParams params;
+	float ms = 0.0f;
+};
+
+// Candidate shapes: block 256/512, treelet 0/512/1024 nodes, a quarter, half or all of the
+// resident grid, refill threshold 8 or 32 -- plus the one-ray-per-lane kernel for single batches.
+std::vector<Candidate> candidates(PhxContext* ctx, bool plan)
+{
+	std::vector<Candidate> out;
+	for (uint32_t block : {256u, 512u})
+		for (uint32_t top : {0u, 512u, 1024u}) {
+			uint32_t bpc = 0;
+			persistentBlocksPerCU(block, top, &bpc);
+			if (bpc == 0) continue;
+			const uint32_t resident = std::min(ctx->numCUs * bpc, ctx->spillLanes / block);
+			for (uint32_t div : {4u, 2u, 1u})
+				for (uint32_t refill : {8u, 32u}) {
+					Candidate c;
+					c.params = PhxLaunchParams{std::max(1u, resident / div),
+					                           std::max(1u, resident / div), refill, block, top,
+					                           resident};
+					if (shapeFor(ctx, c.params, c.launch, plan) == PHX_OK) out.push_back(c);
+				}
+		}
+	if (!plan) {
+		Candidate c;
+		c.launch.kernel = TraceKernel::Simple;
+		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0};
+		out.push_back(c);
+	}
+	return out;
+}
+
+// Times `launch(L)` for every candidate (best of `reps` after one warm-up) with HIP events on the
+// context stream; returns the fastest.
+template <class Launch>
+phx_status tune(PhxContext* ctx, std::vector<Candidate>& cands, int reps, Launch launch,
+                Candidate& best)
+{
+	hipEvent_t a = nullptr, b = nullptr;
+	PHX_HIP(hipEventCreate(&a), "autotune");
+	hipError_t e = hipEventCreate(&b);
+	float bestMs = 3.0e38f;
+	for (Candidate& c : cands) {
+		if (e == hipSuccess) e = launch(c.launch);
+		float m = 3.0e38f;
+		for (int r = 0; r < reps && e == hipSuccess; r++) {
+			e = hipEventRecord(a, ctx->stream);
+			if (e == hipSuccess) e = launch(c.launch);
+			if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+			if (e == hipSuccess) e = hipEventSynchronize(b);
+			float ms = 0.0f;
+			if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+			m = std::min(m, ms);
+		}
+		c.ms = m;
+		if (e == hipSuccess && m < bestMs) {
+			bestMs = m;
+			best = c;
+		}
+	}
+	hipEventDestroy(a);
+	if (b) hipEventDestroy(b);
+	if (e != hipSuccess) return hipFail(e, "autotune");
+	return ok();
+}
+
+}  // namespace
+
+extern "C" {
+
+phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                        uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (!rays || numRays == 0 || !(flags & PHX_MEM_DEVICE))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_autotune: needs device rays");
+	const bool anyhit = (flags & PHX_AUTOTUNE_ANYHIT) != 0;
+	void* out = nullptr;
+	PHX_HIP(hipMalloc(&out, size_t(numRays) * (anyhit ? 1 : sizeof(PhxRayHit))), "phx_autotune");
+	std::vector<Candidate> cands = candidates(ctx, false);
+	SceneView sv = view(scene);
+	Candidate best;
+	st = tune(ctx, cands, 3, [&](const TraceLaunch& L) {
+		return anyhit ? launchAnyHit(sv, rays, numRays, static_cast<uint8_t*>(out), L, ctx->stream)
+		              : launchClosest(sv, rays, numRays, static_cast<PhxRayHit*>(out), L, nullptr,
+		                              ctx->stream);
+	}, best);
+	hipStreamSynchronize(ctx->stream);
+	hipFree(out);
+	if (st != PHX_OK) return st;
+	// Apply: the one-ray-per-lane kernel is selected per call by flag; record the persistent shape.
+	if (best.launch.kernel == TraceKernel::Persistent) {
+		ctx->blockThreads = best.launch.blockThreads;
+		ctx->topNodes = best.launch.topNodes;
+		if (anyhit)
+			ctx->gridAnyhit = best.launch.gridBlocks;
+		else
+			ctx->gridClosest = best.launch.gridBlocks;
+		ctx->refillMin = best.launch.refillMin;
+	}
+	if (chosen) {
+		*chosen = best.params;
+		chosen->residentGridBlocks = best.launch.kernel == TraceKernel::Simple ? 0xFFFFFFFFu : 0u;
+	}
+	return ok();
+}
+
+phx_status phx_plan_set_launch_params(PhxContext* ctx, PhxPlan* plan, const PhxLaunchParams* p)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!plan || !registry().has(plan) || plan->ctx != ctx)
+		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
+	if (!p) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_set_launch_params: NULL params");
+	TraceLaunch L;
+	st = shapeFor(ctx, *p, L, true);
+	if (st != PHX_OK) return st;
+	plan->launch = L;
+	if (plan->graphExec) {
+		hipGraphExecDestroy(plan->graphExec);
+		hipGraphDestroy(plan->graph);
+		plan->graphExec = nullptr;
+		plan->graph = nullptr;
+	}
+	return ok();
+}
+
+phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* chosen)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!plan || !registry().has(plan) || plan->ctx != ctx)
+		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
+	std::vector<Candidate> cands = candidates(ctx, true);
+	Candidate best;
+	st = tune(ctx, cands, 2, [&](const TraceLaunch& L) {
+		return launchBatches(plan->descs, plan->offsets, plan->numBatches, plan->total, L,
+		                     plan->treeletNodes, plan->treeletCount, nullptr, ctx->stream);
+	}, best);
+	if (st != PHX_OK) return st;
+	st = phx_plan_set_launch_params(ctx, plan, &best.params);
+	if (st != PHX_OK) return st;
+	if (chosen) *chosen = best.params;
 	return ok();
 }
 

@@ -71,12 +71,13 @@ struct BatchClock {
 	unsigned long long start;      // earliest block start of the running launch (~0 between launches)
 	unsigned long long lastStart;  // start of the last completed launch
 	uint32_t done[1024];           // rays completed per batch (reset by the launch's last wave)
-	unsigned long long finish[1024];  // device time each batch completed
+	unsigned long long finish[1024];  // device time of each batch's last completed ray (max)
 };
 
 constexpr uint32_t kMaxFusedBatches = 1024;
 
-uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches);
+uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches,
+                       bool tracked);
 
 // One persistent launch over `numBatches` batches; offsets[b] = first ray of batch b in the fused
 // index space, offsets[numBatches] = total. `clock` may be null (no completion tracking).

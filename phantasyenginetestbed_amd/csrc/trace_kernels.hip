@@ -506,9 +506,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_anyhit_persistent(
 // lane maps its ray index to (batch, local index) by binary search over the batch offsets held in
 // LDS. The treelet, if any, belongs to one scene (`treeletNodes`); lanes on other scenes read
 // every node from global memory. With completion tracking on, the lanes completing rays in the
-// same iteration add their count to their batch's counter with one atomic per (wave, batch); the
-// lane that completes a batch stamps its device time, so each batch's latency inside the launch
-// is known. The last wave to retire resets the counters for the next launch.
+// block count completions and the latest completion time per batch in LDS, and every block adds
+// them to the batch table with one global atomic per batch when it retires, so each batch's
+// latency inside the launch is known (the clock only grows, so completion times need no reset
+// between launches). The last wave to retire resets the counters for the next launch.
 
 constexpr uint32_t kMaxBatches = 1024;
 
@@ -536,10 +537,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
 	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsDepth) * 64u / 4u;
-	uint32_t* offsets = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
+	unsigned long long* blockFinish = reinterpret_cast<unsigned long long*>(treelet + 4u * topCount);
+	uint32_t* blockDone = reinterpret_cast<uint32_t*>(blockFinish + (clock ? numBatches : 0u));
+	uint32_t* offsets = blockDone + (clock ? numBatches : 0u);
 	if (clock && threadIdx.x == 0) atomicMin(&clock->start, wall_clock64());
 	for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = treeletNodes[i];
 	for (uint32_t i = threadIdx.x; i <= numBatches; i += blockDim.x) offsets[i] = offsetsGlobal[i];
+	if (clock)
+		for (uint32_t i = threadIdx.x; i < numBatches; i += blockDim.x) {
+			blockFinish[i] = 0ull;
+			blockDone[i] = 0u;
+		}
 	__syncthreads();
 
 	LaneStack st;
@@ -657,21 +665,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 					c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
 			rayIdx = kInvalid;
 		}
-		if (clock) {
-			uint64_t pending = __ballot(done);
-			while (pending) {
-				const int leader = __ffsll((unsigned long long)pending) - 1;
-				const uint32_t lb = __shfl(batch, leader);
-				const uint64_t same = __ballot(done && batch == lb);
-				if (int(__lane_id()) == leader) {
-					const uint32_t cnt = uint32_t(__popcll(same));
-					const uint32_t before = atomicAdd(&clock->done[lb], cnt);
-					if (before + cnt == offsets[lb + 1] - offsets[lb])
-						atomicExch(&clock->finish[lb], wall_clock64());
-				}
-				pending &= ~same;
-			}
+		if (clock && done) {
+			// Block-local LDS atomics; flushed to the batch table when the block retires.
+			atomicAdd(&blockDone[batch], 1u);
+			atomicMax(&blockFinish[batch], wall_clock64());
 		}
+	}
+	if (clock) {
+		__syncthreads();
+		for (uint32_t b = threadIdx.x; b < numBatches; b += blockDim.x)
+			if (blockDone[b]) {
+				atomicAdd(&clock->done[b], blockDone[b]);
+				atomicMax(&clock->finish[b], blockFinish[b]);
+			}
 	}
 	// The last wave resets the shared counters (and, with tracking, the batch counters).
 	if (__lane_id() == 0) {
@@ -776,9 +782,11 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	return hipGetLastError();
 }
 
-uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches)
+uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches,
+                       bool tracked)
 {
-	return persistent_lds_bytes(blockThreads, topNodes) + (numBatches + 1u) * 4u;
+	return persistent_lds_bytes(blockThreads, topNodes) + (numBatches + 1u) * 4u +
+	       (tracked ? numBatches * 12u : 0u);
 }
 
 hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32_t numBatches,
@@ -792,7 +800,8 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
 	if (grid > L.spillLanes / block) grid = L.spillLanes / block;
 	if (grid == 0) grid = 1;
 	hipLaunchKernelGGL(k_trace_batches, dim3(grid), dim3(block),
-	                   batchLdsBytes(block, top, numBatches), stream, descs, offsets, numBatches,
+	                   batchLdsBytes(block, top, numBatches, clock != nullptr), stream, descs,
+	                   offsets, numBatches,
 	                   total, L.work, L.spill, L.refillMin, treeletNodes, top, clock);
 	return hipGetLastError();
 }

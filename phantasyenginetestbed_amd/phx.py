@@ -43,6 +43,7 @@ BATCH_CLOSEST = 0
 BATCH_ANYHIT = 1
 PLAN_TRACK_COMPLETION = 0x10
 PLAN_USE_GRAPH = 0x20
+AUTOTUNE_ANYHIT = 0x1000
 RAYS_ROW_MAJOR = 0
 RAYS_TILED_8X8 = 1
 
@@ -114,6 +115,9 @@ _SIGNATURES = {
     "phx_plan_launch": (C.c_int, [_P, _P, _U32]),
     "phx_plan_batch_times": (C.c_int, [_P, _P, _PF]),
     "phx_plan_release": (None, [_P]),
+    "phx_plan_set_launch_params": (C.c_int, [_P, _P, C.POINTER(LaunchParams)]),
+    "phx_plan_autotune": (C.c_int, [_P, _P, C.POINTER(LaunchParams)]),
+    "phx_autotune": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(LaunchParams)]),
     "phx_camera_def": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
     "phx_camera_look_at": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
     "phx_gen_primary_rays": (C.c_int, [_P, C.POINTER(CameraDef), _U32, _U32, _F32, _F32, _U32,
@@ -301,6 +305,16 @@ class Plan:
         flags = (PLAN_USE_GRAPH if graph else 0) | (ASYNC if asynchronous else 0)
         check(lib().phx_plan_launch(self.ctx.handle, self.handle, flags))
 
+    def autotune(self) -> dict:
+        p = LaunchParams()
+        check(lib().phx_plan_autotune(self.ctx.handle, self.handle, C.byref(p)))
+        return p.as_dict()
+
+    def set_launch_params(self, grid: int, block_threads: int = 256, top_nodes: int = 0,
+                          refill_min: int = 8) -> None:
+        p = LaunchParams(grid, 0, refill_min, block_threads, top_nodes, 0)
+        check(lib().phx_plan_set_launch_params(self.ctx.handle, self.handle, C.byref(p)))
+
     def batch_times_ms(self) -> np.ndarray:
         out = np.zeros(self.num_batches, np.float32)
         check(lib().phx_plan_batch_times(self.ctx.handle, self.handle,
@@ -338,6 +352,17 @@ class Context:
         p = LaunchParams(grid_closest, grid_anyhit, refill_min, block_threads,
                          0xFFFFFFFF if top_nodes is None else top_nodes, 0)
         check(lib().phx_set_launch_params(self.handle, C.byref(p)))
+
+    def autotune(self, scene: Scene, rays_ptr: int, n: int, anyhit: bool = False) -> dict:
+        """Times candidate launch shapes on device rays and applies the fastest. Returns it;
+        'kernel' is 'one-ray-per-lane' when that kernel won (pass KERNEL_ONE_RAY_PER_LANE)."""
+        p = LaunchParams()
+        flags = MEM_DEVICE | (AUTOTUNE_ANYHIT if anyhit else 0)
+        check(lib().phx_autotune(self.handle, scene.handle, rays_ptr, n, flags, C.byref(p)))
+        d = p.as_dict()
+        d["kernel"] = "one-ray-per-lane" if d.pop("residentGridBlocks") == 0xFFFFFFFF \
+            else "persistent"
+        return d
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
