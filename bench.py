@@ -71,6 +71,21 @@ class HipEvents:
             self.L.hipEventDestroy(e)
 
 
+PROFILE = ROOT / "profiles" / "closest_persistent_bounce1M.md"
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the closest-hit kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (scaled by a plain copy's readings); None without it."""
+    try:
+        for line in PROFILE.read_text().splitlines():
+            if "traffic_bytes_per_launch:" in line:
+                return int(line.split("traffic_bytes_per_launch:")[1].split(")")[0].strip())
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
     """The reference's own castRay (compiled from its sources into oracle/_ref) if present,
     else the oracle's C restatement, on all host cores over a bounded sample of the rays."""
@@ -189,6 +204,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
+                    help="fixed persistent launch shape instead of the autotuner's (profiling)")
     ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane"], default=None,
                     help="force a launch model (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -234,7 +251,13 @@ def main() -> int:
         log(f"calibration copy of {args.calibrate_copy} MiB done")
 
     launch = {"kernel": "persistent", **ctx.launch_params()}
-    if not args.no_autotune:
+    if args.launch:
+        grid, block, top, refill = (int(x) for x in args.launch.split(","))
+        ctx.set_launch_params(grid_closest=grid, grid_anyhit=grid, refill_min=refill,
+                              block_threads=block, top_nodes=top)
+        launch = {"kernel": "persistent", **ctx.launch_params()}
+        log(f"fixed launch: {launch}")
+    elif not args.no_autotune:
         launch = ctx.autotune(scene, rays.ptr, n)
         log(f"autotuned launch: {launch}")
     kernel = args.kernel or launch["kernel"]
@@ -280,7 +303,9 @@ def main() -> int:
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic() if kernel == "persistent" else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 768x256 "
+                                       "top 512 refill 32 (profiles/closest_persistent_bounce1M.md)",
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_batched:
