@@ -194,6 +194,38 @@ def batched_form(ctx, scene, pools, ev, reps: int = 3) -> dict:
     }
 
 
+def frame_form(ctx, scene, ev, lights: int, frames: int = 16) -> dict:
+    """The wavefront frame at 1920x1080 with TAA over 16 Halton-jittered frames."""
+    from phantasyenginetestbed_amd import phx, workloads as W
+    w, h = 1920, 1080
+    cam = W.camera(w, h)
+    r = ctx.renderer(w, h)
+    out = ctx.alloc(w * h * 16)
+    flags = phx.FRAME_TAA
+    for f in range(2):  # warm-up (allocates the per-light buffers)
+        r.render(scene, cam, f, 3, flags | phx.FRAME_TAA_RESET, out_ptr=out.ptr)
+    evs = ev.create(frames + 1)
+    ev.record(evs[0], ctx.stream)
+    for f in range(frames):
+        r.render(scene, cam, f, 3, flags | (phx.FRAME_TAA_RESET if f == 0 else 0),
+                 out_ptr=out.ptr, asynchronous=True)
+        ev.record(evs[f + 1], ctx.stream)
+    ctx.synchronize()
+    per = np.array([ev.elapsed_ms(evs[i], evs[i + 1]) for i in range(frames)])
+    ev.destroy(evs)
+    hist = r.download(phx.BUFFER_HISTORY, np.float32).reshape(h, w, 4)
+    r.release()
+    out.free()
+    n, m = w * h, (w // 2) * (h // 2)
+    rays = n + n * lights + m + m * lights
+    ms = float(per.mean())
+    return {"resolution": [w, h], "lights": lights, "materials": 64, "frames": frames,
+            "ms_per_frame": round(ms, 3), "p50_ms": round(float(np.percentile(per, 50)), 3),
+            "p99_ms": round(float(np.percentile(per, 99)), 3),
+            "rays_per_frame": int(rays), "mrays": round(rays / ms / 1e3, 1),
+            "taa_samples_after": float(hist[..., 3].min())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +235,7 @@ def main() -> int:
     ap.add_argument("--boxes", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
                     help="fixed persistent launch shape instead of the autotuner's (profiling)")
@@ -222,12 +255,14 @@ def main() -> int:
 
     log(f"scene {args.grid}x{args.grid} heightfield + {args.boxes} boxes")
     t0 = time.perf_counter()
-    tris, datas, bvh = W.heightfield_scene(args.grid, args.boxes)
+    tris, datas, bvh = W.heightfield_scene(args.grid, args.boxes, materials=64)
     log(f"SBVH: {len(tris)} tris -> {len(bvh.nodes)} nodes, {len(bvh.tris)} refs, "
         f"depth {bvh.depth()}, {time.perf_counter() - t0:.2f}s")
 
     ctx = phx.Context(0)
-    scene = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas)
+    lights = phx.lights_random(8, 0xC0FFEE03)
+    scene = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas, phx.materials_random(64, 0xC0FFEE03),
+                             lights)
     pools = W.ray_pools(ctx, scene)
     bounce = pools.bounce
     n = len(bounce)
@@ -311,6 +346,9 @@ def main() -> int:
     if not args.no_batched:
         result["batched"] = batched_form(ctx, scene, pools, ev)
         log(f"batched: {result['batched']}")
+    if not args.no_frame:
+        result["frame"] = frame_form(ctx, scene, ev, len(lights))
+        log(f"frame: {result['frame']}")
     if not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(bvh.nodes, bvh.tris, bounce, gpu_hits)

@@ -361,6 +361,51 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
                                uint32_t frame, PhxRayIn* outRays, uint32_t flags);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Render frame: the wavefront path of CudaTracerRenderer::render (CudaTracerRenderer.cpp:637-   */
+/* 720) and the TAA pass of GameScreen::render (GameScreen.cpp:187-251)                        */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct PhxRenderer PhxRenderer;
+
+#define PHX_FRAME_TAA 0x10000u       /* jitter by Halton(2,3)[frame % 16], resolve into history  */
+#define PHX_FRAME_TAA_RESET 0x20000u /* start the TAA history at this frame                      */
+#define PHX_FRAME_NO_CLAMP 0x40000u  /* TAA without the 3x3 neighbourhood clamp of the history   */
+
+/* Frame buffers of one resolution (targetResolutionUpdated, CudaTracerRenderer.cpp:735-793).
+ * Per-light buffers grow on the first frame that needs them. */
+phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
+                               PhxRenderer** outRenderer);
+void phx_renderer_release(PhxRenderer* renderer);
+
+/* Renders one frame of `scene` (its materials and lights) seen through `cam`, camera position
+ * cam->origin: primary rays (Halton jitter with PHX_FRAME_TAA), closest hit, hit records
+ * (interpretRayHitKernel), shading of every light with a shadow ray each (gBufferMaterialKernel),
+ * any hit, shadow logic; one bounce ray per 2x2 pixel block (createSecondaryRaysKernel), closest
+ * hit, shading x path throughput with shadow rays (materialKernel), any hit, shadow logic added
+ * at half resolution; then, with PHX_FRAME_TAA, the TAA resolve (taa.frag:25-71 for a static
+ * camera, FP32 history, 3x3 clamp). The primary shadow rays and the bounce rays are traced in one
+ * fused persistent launch. `rgba` (width*height*4 floats, row-major; host or device with
+ * PHX_MEM_DEVICE; may be NULL) receives the output. Random numbers: PCG32 streams keyed by
+ * (seed, frameIndex, pass, record). */
+phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* renderer, const PhxScene* scene,
+                            const PhxCameraDef* cam, uint32_t frameIndex, uint64_t seed,
+                            uint32_t flags, float* rgba);
+
+#define PHX_BUFFER_COLOR 0u       /* float4 per pixel, row-major: the frame before TAA        */
+#define PHX_BUFFER_OUTPUT 1u      /* float4 per pixel: the last output (TAA or frame colour)  */
+#define PHX_BUFFER_GBUFFER 2u     /* 3 float4 per primary record (pixel_id order)             */
+#define PHX_BUFFER_BOUNCE_RAYS 3u /* PhxRayIn per half-resolution record                      */
+#define PHX_BUFFER_HISTORY 4u     /* float4 per pixel: TAA history (rgb, sample count)        */
+
+/* Device address and size of one of the renderer's buffers (valid until the next frame). */
+phx_status phx_renderer_buffer(PhxRenderer* renderer, uint32_t which, const void** devPtr,
+                               size_t* bytes);
+
+/* Pixel jitter of frame f in [0, 1)^2: the radical inverses of (f % 16) + 1 in bases 2 and 3
+ * (HALTON_SEQ, GameScreen.cpp:21-38). The sample of pixel (x, y) lies at (x + jx, y + jy). */
+void phx_halton23(uint32_t frameIndex, float* jx, float* jy);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Synthetic scenes of the benchmark workloads (host)                                         */
 /* ------------------------------------------------------------------------------------------ */
 
@@ -370,6 +415,14 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
  * (geometric normal in n0..n2, zero uvs, material 0). */
 phx_status phx_scene_random_triangles(uint32_t n, uint64_t seed, float eMin, float eMax,
                                       PhxTriangleVertices* triangles, PhxTriangleData* datas);
+
+/* `count` constant materials: albedo ~ U[0,1]^3 (alpha 1), roughness ~ U[0.05, 1], metallic 0,
+ * no textures (iData = -1). */
+phx_status phx_scene_materials_random(uint32_t count, uint64_t seed, PhxMaterial* out);
+
+/* `count` point lights (radius 0) at U[0,1] x U[0.3,1.5] x U[0,1], range 3, strength
+ * ~ U[0.5, 2]^3, static and dynamic shadows on. */
+phx_status phx_scene_lights_random(uint32_t count, uint64_t seed, PhxSphereLight* out);
 
 /* Triangle count of phx_scene_heightfield_boxes. */
 uint32_t phx_scene_heightfield_boxes_count(uint32_t grid, uint32_t numBoxes);

@@ -16,6 +16,7 @@
  *   oracle_brute_closest  - closed-form rays against hand-built triangles.
  *   oracle_primary_rays   - the reference's CPU renderer formula compiled from its sources
  *                           (golden fixture) and closed form at the image centre/corners.
+ *   frame.c functions     - closed forms and distribution checks listed in frame.c's header.
  * The structs are the byte layouts of include/phx.h, restated so the oracle stays standalone.
  */
 #ifndef PHX_ORACLE_H
@@ -89,6 +90,56 @@ int oracle_triangle(const float p0[3], const float p1[3], const float p2[3], con
 /* Intersection.hpp:77-90 on one box with invDir = 1/d; returns hit flag. */
 int oracle_aabb(const float o[3], const float d[3], const float mn[3], const float mx[3],
                 float* tIn, float* tOut);
+
+/* ---- wavefront frame (frame.c) ---------------------------------------------------------- */
+
+/* TriangleData (Triangle.hpp:25-37), Material (Material.hpp:17-55), SphereLight
+ * (SphereLight.hpp:13-26), RayHitInfo (InterpretRayHitKernel.hpp:21-58). */
+typedef struct OrTriData {
+	float n0[3], n1[3], n2[3];
+	float uv0[2], uv1[2], uv2[2];
+	uint32_t materialIndex;
+} OrTriData;
+
+typedef struct OrMaterial {
+	int32_t i[4];
+	float albedo[4];
+	float f2[4];
+} OrMaterial;
+
+typedef struct OrLight {
+	float pos[3];
+	float radius;
+	float range;
+	float strength[3];
+	uint8_t staticShadows, dynamicShadows, pad[2];
+} OrLight;
+
+typedef struct OrHitInfo {
+	float d1[4]; /* position, roughness */
+	float d2[4]; /* normal, metallic */
+	float d3[4]; /* albedo, alpha (negative: no hit) */
+} OrHitInfo;
+
+void oracle_turn_sincos(float r, float* s, float* c);
+float oracle_ggx(float nDotH, float a);
+float oracle_falloff(float d, float range);
+void oracle_shade(const float n[3], const float v[3], const float albedo[3], float rough,
+                  float metal, const float l[3], float dist, const float strength[3], float range,
+                  float out[3]);
+void oracle_sample_lambert(float r1, float r2, const float n[3], float out[3]);
+void oracle_primary_rays_jittered(const float cam[12], uint32_t w, uint32_t h, float jx, float jy,
+                                  float tMin, float tMax, OrRay* rays);
+void oracle_bounce_rays(const OrTriData* datas, const OrRay* rays, const OrHit* hits, uint32_t n,
+                        uint64_t seed, uint32_t frame, OrRay* out);
+void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas,
+                  const OrMaterial* mats, uint32_t numMats, const OrLight* lights,
+                  uint32_t numLights, const float cam[12], uint32_t w, uint32_t h, float jx,
+                  float jy, uint64_t seed, uint32_t frame, float* color, OrHitInfo* gbufOut,
+                  OrRay* bounceOut, uint32_t threads);
+void oracle_taa(const float* cur, const float* histIn, const float* roughness, uint32_t w,
+                uint32_t h, int reset, int clampHistory, float* histOut, float* out);
+void oracle_halton23(uint32_t f, float* jx, float* jy);
 
 #ifdef __cplusplus
 }

@@ -8,8 +8,7 @@
 #include <new>
 #include <unordered_set>
 
-#include "kernels.hpp"
-#include "phx_internal.hpp"
+#include "runtime.hpp"
 
 namespace phx {
 
@@ -34,122 +33,11 @@ phx_status ok()
 	return PHX_OK;
 }
 
-namespace {
-
-// Live handles, so a released or foreign pointer is reported instead of dereferenced.
-struct Registry {
-	std::mutex mu;
-	std::unordered_set<const void*> live;
-	void add(const void* p)
-	{
-		std::lock_guard<std::mutex> g(mu);
-		live.insert(p);
-	}
-	bool remove(const void* p)
-	{
-		std::lock_guard<std::mutex> g(mu);
-		return live.erase(p) != 0;
-	}
-	bool has(const void* p)
-	{
-		std::lock_guard<std::mutex> g(mu);
-		return live.count(p) != 0;
-	}
-};
-
-Registry& registry()
-{
-	static Registry r;
-	return r;
-}
-
-phx_status hipFail(hipError_t e, const char* what)
-{
-	return fail(PHX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
-}
-
-#define PHX_HIP(call, what)                        \
-	do {                                           \
-		hipError_t e_ = (call);                    \
-		if (e_ != hipSuccess) return hipFail(e_, what); \
-	} while (0)
-
-}  // namespace
-
 }  // namespace phx
 
 using namespace phx;
 
-struct PhxBVH {
-	BuiltBVH bvh;
-};
-
-struct PhxContext {
-	int device = 0;
-	hipStream_t stream = nullptr;
-	uint32_t numCUs = 0;
-	// Persistent-kernel state of this context's stream: the ray counter pair (left at zero by
-	// every launch), the spill area of the LDS stacks, and the launch shape.
-	uint32_t* work = nullptr;
-	uint32_t* spill = nullptr;
-	uint32_t gridClosest = 0;
-	uint32_t gridAnyhit = 0;
-	uint32_t blockThreads = 256;
-	uint32_t topNodes = 1024;
-	uint32_t refillMin = 8;
-	uint32_t spillLanes = 0;
-	// Staging buffers for calls that pass host pointers, grown on demand.
-	void* stageIn = nullptr;
-	size_t stageInBytes = 0;
-	void* stageOut = nullptr;
-	size_t stageOutBytes = 0;
-};
-
-struct PhxPlan {
-	PhxContext* ctx = nullptr;
-	phx::TraceLaunch launch;  // launch shape of this plan (set at creation, tunable)
-	uint32_t numBatches = 0;
-	uint32_t total = 0;
-	phx::BatchDesc* descs = nullptr;
-	uint32_t* offsets = nullptr;
-	phx::BatchClock* clock = nullptr;
-	const float4* treeletNodes = nullptr;
-	uint32_t treeletCount = 0;
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t graphExec = nullptr;
-	std::vector<phx::BatchDesc> hostDescs;
-};
-
-struct PhxScene {
-	PhxContext* ctx = nullptr;
-	PhxBVHNode* nodes = nullptr;
-	PhxTriangleVertices* tris = nullptr;
-	PhxTriangleData* datas = nullptr;
-	PhxMaterial* materials = nullptr;
-	PhxSphereLight* lights = nullptr;
-	uint32_t numNodes = 0, numTris = 0, numMaterials = 0, numLights = 0;
-	uint32_t depth = 0;
-	uint32_t finiteBounds = 0;
-	uint32_t topCount = 0;
-	size_t bytes = 0;
-};
-
 namespace {
-
-phx_status checkCtx(PhxContext* ctx)
-{
-	if (!ctx || !registry().has(ctx)) return fail(PHX_ERR_INVALID_HANDLE, "invalid context handle");
-	hipError_t e = hipSetDevice(ctx->device);
-	if (e != hipSuccess) return fail(PHX_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
-	return PHX_OK;
-}
-
-phx_status checkScene(PhxContext* ctx, const PhxScene* scene)
-{
-	if (!scene || !registry().has(scene)) return fail(PHX_ERR_INVALID_HANDLE, "invalid scene handle");
-	if (scene->ctx != ctx) return fail(PHX_ERR_INVALID_HANDLE, "scene belongs to another context");
-	return PHX_OK;
-}
 
 phx_status growStage(void*& buf, size_t& have, size_t need)
 {
@@ -228,25 +116,6 @@ phx_status castCommon(PhxContext* ctx, const PhxRayIn* rays, uint32_t n, void* o
 	PHX_HIP(hipMemcpyAsync(out, ctx->stageOut, outBytes, hipMemcpyDeviceToHost, ctx->stream), what);
 	PHX_HIP(hipStreamSynchronize(ctx->stream), what);
 	return ok();
-}
-
-SceneView view(const PhxScene* s)
-{
-	return SceneView{s->nodes, s->tris, s->numNodes, s->numTris, s->finiteBounds, s->topCount};
-}
-
-TraceLaunch traceLaunch(const PhxContext* ctx, uint32_t flags, bool anyhit)
-{
-	TraceLaunch L;
-	L.kernel = (flags & PHX_KERNEL_ONE_RAY_PER_LANE) ? TraceKernel::Simple : TraceKernel::Persistent;
-	L.gridBlocks = anyhit ? ctx->gridAnyhit : ctx->gridClosest;
-	L.blockThreads = ctx->blockThreads;
-	L.topNodes = ctx->topNodes;
-	L.refillMin = ctx->refillMin;
-	L.spillLanes = ctx->spillLanes;
-	L.work = ctx->work;
-	L.spill = ctx->spill;
-	return L;
 }
 
 // Most node visits of incoherent rays happen in the top levels of the tree, which the persistent
@@ -1090,7 +959,8 @@ phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32
 		if (st != PHX_OK) return st;
 		dst = static_cast<PhxRayIn*>(ctx->stageOut);
 	}
-	PHX_HIP(launchGenPrimaryRays(*cam, width, height, tMin, tMax, order, dst, ctx->stream),
+	PHX_HIP(launchGenPrimaryRays(*cam, width, height, 0.0f, 0.0f, tMin, tMax, order, dst,
+	                             ctx->stream),
 	        "phx_gen_primary_rays");
 	if (!device)
 		PHX_HIP(hipMemcpyAsync(rays, dst, n * sizeof(PhxRayIn), hipMemcpyDeviceToHost, ctx->stream),

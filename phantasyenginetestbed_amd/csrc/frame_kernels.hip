@@ -1,0 +1,373 @@
+// Wavefront frame kernels: interpret hits, shade + shadow rays per light, bounce-ray generation,
+// shadow logic and the TAA resolve. Traversal itself is trace_kernels.hip.
+//
+// Records of full-resolution passes are indexed by pixel_id (8x8 tiles when both sizes are
+// multiples of 8, as getRayIdx in CudaDeviceHelpers.cuh:49-62; row-major otherwise); images are
+// row-major RGBA32F. Per-light arrays are light-major: record id + light * count.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "kernels.hpp"
+#include "rng.hpp"
+#include "shading.hpp"
+
+namespace phx {
+namespace {
+
+using namespace shading;
+
+constexpr uint32_t kBlock1D = 256;
+
+__device__ __forceinline__ uint32_t pixel_id(uint32_t x, uint32_t y, uint32_t w, uint32_t h)
+{
+	if ((w % 8u) == 0u && (h % 8u) == 0u) {
+		const uint32_t tile = (y / 8u) * (w / 8u) + (x / 8u);
+		return tile * 64u + (y % 8u) * 8u + (x % 8u);
+	}
+	return y * w + x;
+}
+
+__device__ __forceinline__ Pcg32 frame_rng(uint64_t seed, uint32_t frame, uint32_t purpose,
+                                           uint32_t id)
+{
+	return Pcg32::seeded(seed ^ (uint64_t(frame) << 32), (uint64_t(purpose) << 32) | id);
+}
+
+__device__ __forceinline__ void store_ray(float4* rays, size_t i, V3 o, float tMin, V3 d,
+                                          float tMax)
+{
+	rays[2 * i] = make_float4(o.x, o.y, o.z, tMin);
+	rays[2 * i + 1] = make_float4(d.x, d.y, d.z, tMax);
+}
+
+// setToDummyRay, MaterialKernel.cu:41-47.
+__device__ __forceinline__ void store_dummy(float4* rays, size_t i)
+{
+	rays[2 * i] = make_float4(1000000.0f, 1000000.0f, 1000000.0f, 0.001f);
+	rays[2 * i + 1] = make_float4(0.0f, 1.0f, 0.0f, 0.001f);
+}
+
+__device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
+
+// interpretRayHitKernel, InterpretRayHitKernel.cu:58-121: position o + d*t, normal
+// normalize(n0 + (n1-n0)u + (n2-n0)v), constant material (albedo rgba, roughness, metallic; an
+// index outside the table reads albedo 1, roughness 1, metallic 0). Record layout (48 B):
+// (pos, roughness) (normal, metallic) (albedo, alpha), alpha negative for a miss.
+__global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict__ rays,
+                                                        const uint4* __restrict__ hits, uint32_t n,
+                                                        const PhxTriangleData* __restrict__ datas,
+                                                        const PhxMaterial* __restrict__ mats,
+                                                        uint32_t numMats,
+                                                        float4* __restrict__ info)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint4 h = hits[i];
+	if (h.x == 0xFFFFFFFFu) {
+		info[3 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		info[3 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		info[3 * size_t(i) + 2] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+		return;
+	}
+	const float4 ro = rays[2 * size_t(i)];
+	const float4 rd = rays[2 * size_t(i) + 1];
+	const float t = __uint_as_float(h.y), bu = __uint_as_float(h.z), bv = __uint_as_float(h.w);
+	const PhxTriangleData& d = datas[h.x];
+	float nn[3];
+	for (int k = 0; k < 3; k++) {
+		const float a = (d.n1[k] - d.n0[k]) * bu;
+		const float b = (d.n2[k] - d.n0[k]) * bv;
+		nn[k] = (d.n0[k] + a) + b;
+	}
+	const V3 nrm = normalize(v3(nn[0], nn[1], nn[2]));
+	float4 alb = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+	float rough = 1.0f, metal = 0.0f;
+	if (d.materialIndex < numMats) {
+		const PhxMaterial& m = mats[d.materialIndex];
+		alb = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], m.albedo[3]);
+		rough = m.fData2[0];
+		metal = m.fData2[1];
+	}
+	info[3 * size_t(i)] = make_float4(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t, rough);
+	info[3 * size_t(i) + 1] = make_float4(nrm.x, nrm.y, nrm.z, metal);
+	info[3 * size_t(i) + 2] = make_float4(alb.x, alb.y, alb.z, fabsf(alb.w));
+}
+
+// shadeHit (MaterialKernel.cu:49-114) of every record: per light the contribution
+// mask * falloff * shade and a shadow ray towards a point on the light's disc (soft shadows,
+// radius 0 = point light), or the dummy ray when the light has no static shadows or contributes
+// nothing (every channel <= 1e-4). Primary records (rays == nullptr) view towards camPos with
+// mask 1 (gBufferMaterialKernel, :119-140); bounce records view along -ray.dir with their path
+// throughput (materialKernel, :145-184). Records without a hit get dummy rays and zero.
+__global__ __launch_bounds__(kBlock1D) void k_shade(const float4* __restrict__ info,
+                                                    const float4* __restrict__ rays,
+                                                    const float4* __restrict__ throughput,
+                                                    float cx, float cy, float cz,
+                                                    const PhxSphereLight* __restrict__ lights,
+                                                    uint32_t numLights, uint32_t count,
+                                                    uint64_t seed, uint32_t frame,
+                                                    uint32_t purpose,
+                                                    float4* __restrict__ contrib,
+                                                    float4* __restrict__ shadow)
+{
+	const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+	if (id >= count) return;
+	const float4 d1 = info[3 * size_t(id)];
+	const float4 d2 = info[3 * size_t(id) + 1];
+	const float4 d3 = info[3 * size_t(id) + 2];
+	if (!(d3.w >= 0.0f)) {
+		for (uint32_t i = 0; i < numLights; i++) {
+			const size_t s = size_t(id) + size_t(i) * count;
+			contrib[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			store_dummy(shadow, s);
+		}
+		return;
+	}
+	const V3 p = xyz(d1), n = xyz(d2), albedo = xyz(d3);
+	const float rough = d1.w, metal = d2.w;
+	V3 view, mask;
+	if (rays) {
+		const float4 rd = rays[2 * size_t(id) + 1];
+		view = v3(-rd.x, -rd.y, -rd.z);
+		mask = xyz(throughput[id]);
+	} else {
+		view = normalize(sub(v3(cx, cy, cz), p));
+		mask = v3(1.0f, 1.0f, 1.0f);
+	}
+	Pcg32 rng = frame_rng(seed, frame, purpose, id);
+	for (uint32_t i = 0; i < numLights; i++) {
+		const PhxSphereLight& L = lights[i];
+		const V3 lp = v3(L.pos[0], L.pos[1], L.pos[2]);
+		const V3 toL = sub(lp, p);
+		const float dist = length(toL);
+		const V3 l = divs(toL, dist);
+		const float fo = falloff(dist, L.range);
+		const V3 sh = shade(n, view, albedo, rough, metal, l, dist,
+		                    v3(L.strength[0], L.strength[1], L.strength[2]), L.range);
+		const V3 c = mul(scale(mask, fo), sh);
+		const size_t s = size_t(id) + size_t(i) * count;
+		contrib[s] = make_float4(c.x, c.y, c.z, 0.0f);
+		const float eps = 0.0001f;
+		const bool lit = c.x > eps || c.y > eps || c.z > eps;
+		if (L.staticShadows && lit) {
+			const V3 cu = fabsf(n.z) > 0.01f ? normalize(v3(0.0f, -n.z, n.y))
+			                                 : normalize(v3(-n.y, n.x, 0.0f));
+			const V3 cv = cross(cu, l);
+			const float r1 = rng.uniform();
+			const float r2 = rng.uniform();
+			const float cd = L.radius * (2.0f * r2 - 1.0f);
+			float sn, cs;
+			turn_sincos(r1, sn, cs);
+			const V3 off = add(scale(scale(cu, cs), cd), scale(scale(cv, sn), cd));
+			const V3 diff = sub(add(lp, off), p);
+			store_ray(shadow, s, p, 0.001f, normalize(diff), length(diff));
+		} else {
+			store_dummy(shadow, s);
+		}
+	}
+}
+
+// createSecondaryRaysKernel (MaterialKernel.cu:251-312) on the half-resolution grid: a random
+// pixel of the 2x2 block, then diffuse (Lambert) or GGX-around-the-reflection by Russian roulette
+// on roughness, throughput albedo * 0.6, tMin 1e-3; alpha <= 0.5 takes the reference's glass
+// path. A block pixel without a hit gets the dummy ray and zero throughput.
+__global__ __launch_bounds__(kBlock1D) void k_gen_secondary(const float4* __restrict__ gbuf,
+                                                            uint32_t w, uint32_t h, float cx,
+                                                            float cy, float cz, uint64_t seed,
+                                                            uint32_t frame,
+                                                            float4* __restrict__ rays,
+                                                            float4* __restrict__ throughput)
+{
+	const uint32_t hw = w / 2u, hh = h / 2u;
+	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
+	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
+	if (x >= hw || y >= hh) return;
+	const uint32_t hid = pixel_id(x, y, hw, hh);
+	Pcg32 rng = frame_rng(seed, frame, 2u, hid);
+	const uint32_t pix = rng.next() % 4u;
+	const uint32_t fid = pixel_id(2u * x + pix % 2u, 2u * y + pix / 2u, w, h);
+	const float4 d1 = gbuf[3 * size_t(fid)];
+	const float4 d2 = gbuf[3 * size_t(fid) + 1];
+	const float4 d3 = gbuf[3 * size_t(fid) + 2];
+	if (!(d3.w >= 0.0f)) {
+		store_dummy(rays, hid);
+		throughput[hid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		return;
+	}
+	const V3 gp = xyz(d1), gn = xyz(d2), alb = xyz(d3);
+	const float rough = d1.w;
+	const V3 fromCam = normalize(sub(gp, v3(cx, cy, cz)));
+	const float r1 = rng.uniform();
+	const float r2 = rng.uniform();
+	float tMin = 0.001f;
+	V3 dir, tp;
+	if (fabsf(d3.w) > 0.5f) {
+		const float pick = rng.uniform();
+		tp = scale(alb, 0.6f);
+		if (pick < rough)
+			dir = sample_lambert(r1, r2, gn);
+		else
+			dir = sample_ggx(r1, r2, rough * rough, reflect(fromCam, gn), gn);
+	} else {
+		dir = normalize(add(scale(fromCam, 0.8f), scale(v3(-gn.x, -gn.y, -gn.z), 0.2f)));
+		tp = v3(1.0f, 1.0f, 1.0f);
+		tMin = 0.2f;
+	}
+	store_ray(rays, hid, gp, tMin, dir, FLT_MAX);
+	throughput[hid] = make_float4(tp.x, tp.y, tp.z, 0.0f);
+}
+
+// shadowLogicKernel, MaterialKernel.cu:361-388: colour (+)= the contributions of the lights whose
+// shadow ray reached them, in light order; a 1/scale pass is read at loc / scale.
+__global__ __launch_bounds__(kBlock1D) void k_shadow_logic(float4* __restrict__ img, uint32_t w,
+                                                           uint32_t h, uint32_t sc, uint32_t addTo,
+                                                           const uint8_t* __restrict__ inLight,
+                                                           const float4* __restrict__ contrib,
+                                                           uint32_t numLights)
+{
+	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
+	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
+	if (x >= w || y >= h) return;
+	const uint32_t sw = w / sc, sh = h / sc;
+	const uint32_t sx = min(x / sc, sw - 1u), sy = min(y / sc, sh - 1u);
+	const uint32_t sid = pixel_id(sx, sy, sw, sh);
+	const size_t sN = size_t(sw) * sh;
+	float4* px = img + size_t(y) * w + x;
+	float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+	if (addTo) {
+		const float4 cur = *px;
+		c0 = cur.x;
+		c1 = cur.y;
+		c2 = cur.z;
+	}
+	for (uint32_t i = 0; i < numLights; i++) {
+		const size_t s = size_t(sid) + size_t(i) * sN;
+		if (inLight[s]) {
+			const float4 c = contrib[s];
+			c0 = c0 + c.x;
+			c1 = c1 + c.y;
+			c2 = c2 + c.z;
+		}
+	}
+	*px = make_float4(c0, c1, c2, 1.0f);
+}
+
+// TAA resolve (taa.frag:25-71) for a static camera (zero velocity: trust 1, history sample at the
+// same pixel), FP32 history with the effective sample count in alpha, and the history colour
+// clamped to the current frame's 3x3 neighbourhood min/max (clampHistory).
+__global__ __launch_bounds__(kBlock1D) void k_taa(const float4* __restrict__ cur,
+                                                  const float4* __restrict__ histIn,
+                                                  const float4* __restrict__ gbuf, uint32_t w,
+                                                  uint32_t h, uint32_t reset,
+                                                  uint32_t clampHistory,
+                                                  float4* __restrict__ histOut,
+                                                  float4* __restrict__ out)
+{
+	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
+	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
+	if (x >= w || y >= h) return;
+	const size_t p = size_t(y) * w + x;
+	const float4 c = cur[p];
+	float o0, o1, o2, n2;
+	if (reset) {
+		o0 = c.x;
+		o1 = c.y;
+		o2 = c.z;
+		n2 = 1.0f;
+	} else {
+		const float rough = gbuf[3 * size_t(pixel_id(x, y, w, h))].w;
+		const float vDistrust = min_num(1.0f, 1.2f * sqrtf(fabsf(0.0f - 0.0f)));
+		const float rDistrust = 3.0f * (1.0f - rough) + 1.0f * rough;
+		const float trust = clamp_num(1.0f - rDistrust * vDistrust, 0.0f, 1.0f);
+		const float4 hs = histIn[p];
+		float h0 = hs.x, h1 = hs.y, h2 = hs.z;
+		if (clampHistory) {
+			float lo0 = c.x, lo1 = c.y, lo2 = c.z, hi0 = c.x, hi1 = c.y, hi2 = c.z;
+			const uint32_t y0 = y ? y - 1u : 0u, y1 = y + 1u < h ? y + 1u : h - 1u;
+			const uint32_t x0 = x ? x - 1u : 0u, x1 = x + 1u < w ? x + 1u : w - 1u;
+			for (uint32_t yy = y0; yy <= y1; yy++)
+				for (uint32_t xx = x0; xx <= x1; xx++) {
+					const float4 q = cur[size_t(yy) * w + xx];
+					lo0 = min_num(lo0, q.x);
+					lo1 = min_num(lo1, q.y);
+					lo2 = min_num(lo2, q.z);
+					hi0 = max_num(hi0, q.x);
+					hi1 = max_num(hi1, q.y);
+					hi2 = max_num(hi2, q.z);
+				}
+			h0 = clamp_num(h0, lo0, hi0);
+			h1 = clamp_num(h1, lo1, hi1);
+			h2 = clamp_num(h2, lo2, hi2);
+		}
+		n2 = hs.w * trust + 1.0f;
+		const float blend = 1.0f - 1.0f / n2;
+		o0 = c.x * (1.0f - blend) + h0 * blend;
+		o1 = c.y * (1.0f - blend) + h1 * blend;
+		o2 = c.z * (1.0f - blend) + h2 * blend;
+	}
+	histOut[p] = make_float4(o0, o1, o2, n2);
+	out[p] = make_float4(o0, o1, o2, 1.0f);
+}
+
+uint32_t blocks1d(uint32_t n) { return n / kBlock1D + (n % kBlock1D != 0u); }
+dim3 grid2d(uint32_t w, uint32_t h) { return dim3((w + 15u) / 16u, (h + 15u) / 16u); }
+
+}  // namespace
+
+hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
+                           const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
+                           float4* info, hipStream_t stream)
+{
+	if (n == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_interpret, dim3(blocks1d(n)), dim3(kBlock1D), 0, stream,
+	                   reinterpret_cast<const float4*>(rays), reinterpret_cast<const uint4*>(hits),
+	                   n, datas, mats, numMats, info);
+	return hipGetLastError();
+}
+
+hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
+                       const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
+                       uint32_t count, uint64_t seed, uint32_t frame, uint32_t purpose,
+                       float4* contrib, PhxRayIn* shadow, hipStream_t stream)
+{
+	if (count == 0 || numLights == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_shade, dim3(blocks1d(count)), dim3(kBlock1D), 0, stream, info,
+	                   reinterpret_cast<const float4*>(rays), throughput, camPos[0], camPos[1],
+	                   camPos[2], lights, numLights, count, seed, frame, purpose, contrib,
+	                   reinterpret_cast<float4*>(shadow));
+	return hipGetLastError();
+}
+
+hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const float camPos[3],
+                              uint64_t seed, uint32_t frame, PhxRayIn* rays, float4* throughput,
+                              hipStream_t stream)
+{
+	if (w < 2u || h < 2u) return hipSuccess;
+	hipLaunchKernelGGL(k_gen_secondary, grid2d(w / 2u, h / 2u), dim3(kBlock1D), 0, stream, gbuf, w,
+	                   h, camPos[0], camPos[1], camPos[2], seed, frame,
+	                   reinterpret_cast<float4*>(rays), throughput);
+	return hipGetLastError();
+}
+
+hipError_t launchShadowLogic(float4* img, uint32_t w, uint32_t h, uint32_t scale, bool addTo,
+                             const uint8_t* inLight, const float4* contrib, uint32_t numLights,
+                             hipStream_t stream)
+{
+	if (w == 0 || h == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_shadow_logic, grid2d(w, h), dim3(kBlock1D), 0, stream, img, w, h, scale,
+	                   uint32_t(addTo), inLight, contrib, numLights);
+	return hipGetLastError();
+}
+
+hipError_t launchTaa(const float4* cur, const float4* histIn, const float4* gbuf, uint32_t w,
+                     uint32_t h, bool reset, bool clampHistory, float4* histOut, float4* out,
+                     hipStream_t stream)
+{
+	if (w == 0 || h == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_taa, grid2d(w, h), dim3(kBlock1D), 0, stream, cur, histIn, gbuf, w, h,
+	                   uint32_t(reset), uint32_t(clampHistory), histOut, out);
+	return hipGetLastError();
+}
+
+}  // namespace phx

@@ -86,8 +86,28 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
                          uint32_t treeletCount, BatchClock* clock, hipStream_t stream);
 
 // Pinhole primary rays.
-hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float tMin,
-                                float tMax, uint32_t order, PhxRayIn* rays, hipStream_t stream);
+hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float jx,
+                                float jy, float tMin, float tMax, uint32_t order, PhxRayIn* rays,
+                                hipStream_t stream);
+
+// Wavefront frame kernels (frame_kernels.hip). Hit records are 3 float4 each:
+// (position, roughness) (normal, metallic) (albedo, alpha; alpha < 0 = no hit).
+hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
+                           const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
+                           float4* info, hipStream_t stream);
+hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
+                       const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
+                       uint32_t count, uint64_t seed, uint32_t frame, uint32_t purpose,
+                       float4* contrib, PhxRayIn* shadow, hipStream_t stream);
+hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const float camPos[3],
+                              uint64_t seed, uint32_t frame, PhxRayIn* rays, float4* throughput,
+                              hipStream_t stream);
+hipError_t launchShadowLogic(float4* img, uint32_t w, uint32_t h, uint32_t scale, bool addTo,
+                             const uint8_t* inLight, const float4* contrib, uint32_t numLights,
+                             hipStream_t stream);
+hipError_t launchTaa(const float4* cur, const float4* histIn, const float4* gbuf, uint32_t w,
+                     uint32_t h, bool reset, bool clampHistory, float4* histOut, float4* out,
+                     hipStream_t stream);
 
 }  // namespace phx
 

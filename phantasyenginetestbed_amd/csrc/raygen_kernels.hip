@@ -5,6 +5,7 @@
 
 #include "kernels.hpp"
 #include "rng.hpp"
+#include "shading.hpp"
 
 namespace phx {
 namespace {
@@ -13,15 +14,16 @@ namespace {
 // the CPU renderer's formula (CPUTracerRenderer.cpp:93-97): c = (x, y) / res * 2 - 1,
 // dir = normalize((camDir + cx * dX) + cy * dY), normalize(v) = v / sqrt((x*x + y*y) + z*z).
 // Compiled without contraction and with correctly rounded sqrt/div, so it matches the CPU bits.
+// A jitter (jx, jy) in pixels moves the sample to (x + jx, y + jy); (0, 0) is the formula above.
 __global__ __launch_bounds__(256) void k_gen_primary(PhxCameraDef cam, uint32_t w, uint32_t h,
-                                                     float tMin, float tMax, uint32_t order,
-                                                     float4* __restrict__ rays)
+                                                     float jx, float jy, float tMin, float tMax,
+                                                     uint32_t order, float4* __restrict__ rays)
 {
 	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
 	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
 	if (x >= w || y >= h) return;
-	const float cx = (float(x) / float(w)) * 2.0f - 1.0f;
-	const float cy = (float(y) / float(h)) * 2.0f - 1.0f;
+	const float cx = ((float(x) + jx) / float(w)) * 2.0f - 1.0f;
+	const float cy = ((float(y) + jy) / float(h)) * 2.0f - 1.0f;
 	float d[3];
 	for (int k = 0; k < 3; k++) {
 		float a = cam.dX[k] * cx;
@@ -60,7 +62,8 @@ __device__ __forceinline__ void normalize3(float v[3])
 
 // Bounce ray of ray i: hit position o + t*d (InterpretRayHitKernel.cu:64-70), shading normal
 // normalize(n0 + (n1-n0)u + (n2-n0)v) (BVHTraversal.hpp:259-262) turned to face the incoming ray,
-// cosine-weighted direction (sampleLambertian + hemisphereToWorld, MaterialKernel.cu:206-246),
+// cosine-weighted direction (sampleLambertian + hemisphereToWorld, MaterialKernel.cu:206-246,
+// with cos/sin(2 pi r1) from shading::turn_sincos), PCG32 stream pixel_rng(seed, frame, i),
 // tMin 0.001, tMax FLT_MAX. A miss gets the dummy ray of MaterialKernel.cu:41-47.
 __global__ __launch_bounds__(256) void k_gen_bounce(const float4* __restrict__ rays,
                                                     const uint4* __restrict__ hits,
@@ -99,20 +102,8 @@ __global__ __launch_bounds__(256) void k_gen_bounce(const float4* __restrict__ r
 	Pcg32 rng = pixel_rng(seed, frame, i);
 	const float r1 = rng.uniform();
 	const float r2 = rng.uniform();
-	const float phi = 6.28318530717958647692f * r1;
-	const float sinT = sqrtf(r2);
-	const float cosT = sqrtf(1.0f - r2);
-	const float hx = sinT * cosf(phi), hy = sinT * sinf(phi), hz = cosT;
-
-	const bool zUp = fabsf(nrm[2]) < 0.999f;
-	const float up[3] = {zUp ? 0.0f : 1.0f, 0.0f, zUp ? 1.0f : 0.0f};
-	float u[3] = {up[1] * nrm[2] - up[2] * nrm[1], up[2] * nrm[0] - up[0] * nrm[2],
-	              up[0] * nrm[1] - up[1] * nrm[0]};
-	normalize3(u);
-	const float v[3] = {nrm[1] * u[2] - nrm[2] * u[1], nrm[2] * u[0] - nrm[0] * u[2],
-	                    nrm[0] * u[1] - nrm[1] * u[0]};
-	float dir[3];
-	for (int k = 0; k < 3; k++) dir[k] = (u[k] * hx + v[k] * hy) + nrm[k] * hz;
+	const shading::V3 dv = shading::sample_lambert(r1, r2, shading::v3(nrm[0], nrm[1], nrm[2]));
+	const float dir[3] = {dv.x, dv.y, dv.z};
 	out[2 * size_t(i)] = make_float4(pos[0], pos[1], pos[2], 0.001f);
 	out[2 * size_t(i) + 1] = make_float4(dir[0], dir[1], dir[2], FLT_MAX);
 }
@@ -131,12 +122,13 @@ hipError_t launchGenBounceRays(const PhxRayIn* rays, const PhxRayHit* hits,
 	return hipGetLastError();
 }
 
-hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float tMin,
-                                float tMax, uint32_t order, PhxRayIn* rays, hipStream_t stream)
+hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float jx,
+                                float jy, float tMin, float tMax, uint32_t order, PhxRayIn* rays,
+                                hipStream_t stream)
 {
 	dim3 grid((w + 15u) / 16u, (h + 15u) / 16u);
-	hipLaunchKernelGGL(k_gen_primary, grid, dim3(256), 0, stream, cam, w, h, tMin, tMax, order,
-	                   reinterpret_cast<float4*>(rays));
+	hipLaunchKernelGGL(k_gen_primary, grid, dim3(256), 0, stream, cam, w, h, jx, jy, tMin, tMax,
+	                   order, reinterpret_cast<float4*>(rays));
 	return hipGetLastError();
 }
 

@@ -1,0 +1,279 @@
+// The render-frame C ABI: frame buffers of one resolution and the wavefront frame sequence
+// (CudaTracerRenderer::render's normal path, CudaTracerRenderer.cpp:637-720, with the G-buffer
+// traced instead of rasterised) followed by the TAA resolve of GameScreen::render.
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <new>
+
+#include "runtime.hpp"
+
+using namespace phx;
+
+struct PhxRenderer {
+	PhxContext* ctx = nullptr;
+	uint32_t w = 0, h = 0, n = 0;    // full resolution
+	uint32_t hw = 0, hh = 0, m = 0;  // half resolution (bounce records)
+	uint32_t lightCap = 0;           // lights the per-light buffers hold
+	PhxRayIn* prim = nullptr;
+	PhxRayHit* primHits = nullptr;
+	float4* gbuf = nullptr;
+	PhxRayIn* bounce = nullptr;
+	float4* throughput = nullptr;
+	PhxRayHit* bounceHits = nullptr;
+	float4* bounceInfo = nullptr;
+	float4* contrib = nullptr;  // n * lightCap
+	PhxRayIn* shadow = nullptr;  // n * lightCap
+	uint8_t* inLight = nullptr;  // n * lightCap
+	float4* color = nullptr;
+	float4* hist[2] = {nullptr, nullptr};
+	float4* out = nullptr;
+	uint32_t histIndex = 0;
+	bool haveHistory = false;
+	BatchDesc* descs = nullptr;  // 2 descriptors of the fused shadow + bounce launch
+	uint32_t* offsets = nullptr;
+	BatchDesc hostDescs[2];      // what descs / offsets hold
+	uint32_t hostOff[3] = {0u, 0u, 0u};
+	bool descsValid = false;
+	float4* lastOut = nullptr;
+};
+
+namespace {
+
+void freeRenderer(PhxRenderer* r)
+{
+	void* bufs[] = {r->prim,    r->primHits, r->gbuf,   r->bounce,  r->throughput,
+	                r->bounceHits, r->bounceInfo, r->contrib, r->shadow, r->inLight,
+	                r->color,   r->hist[0],  r->hist[1], r->out,    r->descs,
+	                r->offsets};
+	for (void* b : bufs) hipFree(b);
+	delete r;
+}
+
+template <class T>
+phx_status devAlloc(T*& p, size_t count, const char* what)
+{
+	p = nullptr;
+	if (count == 0) return PHX_OK;
+	if (hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) != hipSuccess) {
+		p = nullptr;
+		return fail(PHX_ERR_OUT_OF_MEMORY, "%s: device allocation of %zu bytes failed", what,
+		            count * sizeof(T));
+	}
+	return PHX_OK;
+}
+
+phx_status growLights(PhxRenderer* r, uint32_t lights)
+{
+	if (lights <= r->lightCap) return PHX_OK;
+	hipStreamSynchronize(r->ctx->stream);  // earlier frames may still use the buffers
+	hipFree(r->contrib);
+	hipFree(r->shadow);
+	hipFree(r->inLight);
+	r->contrib = nullptr;
+	r->shadow = nullptr;
+	r->inLight = nullptr;
+	r->lightCap = 0;
+	const size_t k = size_t(r->n) * lights;
+	phx_status st = devAlloc(r->contrib, k, "phx_render_frame");
+	if (st == PHX_OK) st = devAlloc(r->shadow, k, "phx_render_frame");
+	if (st == PHX_OK) st = devAlloc(r->inLight, k, "phx_render_frame");
+	if (st == PHX_OK) r->lightCap = lights;
+	return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+void phx_halton23(uint32_t frameIndex, float* jx, float* jy)
+{
+	const uint32_t i = frameIndex % 16u + 1u;
+	double r = 0.0, inv = 0.5;
+	for (uint32_t k = i; k; k /= 2u, inv *= 0.5) r += double(k % 2u) * inv;
+	if (jx) *jx = float(r);
+	r = 0.0;
+	inv = 1.0 / 3.0;
+	for (uint32_t k = i; k; k /= 3u, inv /= 3.0) r += double(k % 3u) * inv;
+	if (jy) *jy = float(r);
+}
+
+phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
+                               PhxRenderer** outRenderer)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!outRenderer) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_renderer_create: NULL output");
+	*outRenderer = nullptr;
+	if (width == 0 || height == 0 || width > 16384 || height > 16384)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_renderer_create: resolution %ux%u", width,
+		            height);
+	PhxRenderer* r = new (std::nothrow) PhxRenderer();
+	if (!r) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_renderer_create: out of host memory");
+	r->ctx = ctx;
+	r->w = width;
+	r->h = height;
+	r->n = width * height;
+	r->hw = width / 2u;
+	r->hh = height / 2u;
+	r->m = r->hw * r->hh;
+	const char* what = "phx_renderer_create";
+	st = devAlloc(r->prim, r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->primHits, r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->gbuf, size_t(r->n) * 3u, what);
+	if (st == PHX_OK) st = devAlloc(r->bounce, r->m, what);
+	if (st == PHX_OK) st = devAlloc(r->throughput, r->m, what);
+	if (st == PHX_OK) st = devAlloc(r->bounceHits, r->m, what);
+	if (st == PHX_OK) st = devAlloc(r->bounceInfo, size_t(r->m) * 3u, what);
+	if (st == PHX_OK) st = devAlloc(r->color, r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->hist[0], r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->hist[1], r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->out, r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->descs, 2, what);
+	if (st == PHX_OK) st = devAlloc(r->offsets, 3, what);
+	if (st != PHX_OK) {
+		freeRenderer(r);
+		return st;
+	}
+	r->lastOut = r->color;
+	registry().add(r);
+	*outRenderer = r;
+	return ok();
+}
+
+void phx_renderer_release(PhxRenderer* r)
+{
+	if (!r || !registry().remove(r)) return;
+	hipSetDevice(r->ctx->device);
+	hipStreamSynchronize(r->ctx->stream);
+	freeRenderer(r);
+}
+
+phx_status phx_renderer_buffer(PhxRenderer* r, uint32_t which, const void** devPtr, size_t* bytes)
+{
+	if (!r || !registry().has(r)) return fail(PHX_ERR_INVALID_HANDLE, "invalid renderer handle");
+	if (!devPtr) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_renderer_buffer: NULL output");
+	const void* p = nullptr;
+	size_t b = 0;
+	switch (which) {
+	case PHX_BUFFER_COLOR: p = r->color; b = size_t(r->n) * 16u; break;
+	case PHX_BUFFER_OUTPUT: p = r->lastOut; b = size_t(r->n) * 16u; break;
+	case PHX_BUFFER_GBUFFER: p = r->gbuf; b = size_t(r->n) * 48u; break;
+	case PHX_BUFFER_BOUNCE_RAYS: p = r->bounce; b = size_t(r->m) * sizeof(PhxRayIn); break;
+	case PHX_BUFFER_HISTORY: p = r->hist[r->histIndex]; b = size_t(r->n) * 16u; break;
+	default: return fail(PHX_ERR_INVALID_ARGUMENT, "phx_renderer_buffer: unknown buffer %u", which);
+	}
+	*devPtr = p;
+	if (bytes) *bytes = b;
+	return ok();
+}
+
+phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
+                            const PhxCameraDef* cam, uint32_t frameIndex, uint64_t seed,
+                            uint32_t flags, float* rgba)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (!r || !registry().has(r) || r->ctx != ctx)
+		return fail(PHX_ERR_INVALID_HANDLE, "invalid renderer handle");
+	if (!cam) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: NULL camera");
+	if (!scene->datas) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: scene has no TriangleData");
+	if (uint64_t(r->n) * scene->numLights + r->m > 0x7FFFFFFFull)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: more than 2^31 shadow rays");
+	const bool device = (flags & PHX_MEM_DEVICE) != 0;
+	const bool async = (flags & PHX_ASYNC) != 0;
+	if (async && rgba && !device)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: PHX_ASYNC needs PHX_MEM_DEVICE");
+	st = growLights(r, scene->numLights);
+	if (st != PHX_OK) return st;
+
+	hipStream_t s = ctx->stream;
+	const uint32_t L = scene->numLights;
+	const bool taa = (flags & PHX_FRAME_TAA) != 0;
+	float jx = 0.0f, jy = 0.0f;
+	if (taa) phx_halton23(frameIndex, &jx, &jy);
+	const float camPos[3] = {cam->origin[0], cam->origin[1], cam->origin[2]};
+	const uint32_t order = (r->w % 8u == 0u && r->h % 8u == 0u) ? PHX_RAYS_TILED_8X8 : PHX_RAYS_ROW_MAJOR;
+	const SceneView sv = view(scene);
+	const TraceLaunch closest = traceLaunch(ctx, 0, false);
+	const TraceLaunch anyhit = traceLaunch(ctx, 0, true);
+	const char* what = "phx_render_frame";
+
+	// Primary hits -> hit records -> contributions and shadow rays of every light.
+	PHX_HIP(launchGenPrimaryRays(*cam, r->w, r->h, jx, jy, 1e-4f, FLT_MAX, order, r->prim, s), what);
+	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, closest, nullptr, s), what);
+	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->materials,
+	                        scene->numMaterials, r->gbuf, s),
+	        what);
+	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, seed,
+	                    frameIndex, 1u, r->contrib, r->shadow, s),
+	        what);
+	// Bounce rays (they need only the hit records), traced together with the primary shadow rays
+	// in one fused persistent launch.
+	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, camPos, seed, frameIndex, r->bounce,
+	                           r->throughput, s),
+	        what);
+	{
+		BatchDesc d[2];
+		const float4* nodes = reinterpret_cast<const float4*>(scene->nodes);
+		const float4* tris = reinterpret_cast<const float4*>(scene->tris);
+		d[0] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->shadow), r->inLight,
+		                 r->n * L, 1u, scene->finiteBounds, 0u};
+		d[1] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->bounce), r->bounceHits,
+		                 r->m, 0u, scene->finiteBounds, 0u};
+		const uint32_t off[3] = {0u, r->n * L, r->n * L + r->m};
+		if (!r->descsValid || std::memcmp(d, r->hostDescs, sizeof(d)) != 0 ||
+		    std::memcmp(off, r->hostOff, sizeof(off)) != 0) {
+			// The copies read the renderer's host copies, which change only after the stream
+			// has drained the frames that used them.
+			PHX_HIP(hipStreamSynchronize(s), what);
+			std::memcpy(r->hostDescs, d, sizeof(d));
+			std::memcpy(r->hostOff, off, sizeof(off));
+			PHX_HIP(hipMemcpyAsync(r->descs, r->hostDescs, sizeof(d), hipMemcpyHostToDevice, s),
+			        what);
+			PHX_HIP(hipMemcpyAsync(r->offsets, r->hostOff, sizeof(off), hipMemcpyHostToDevice, s),
+			        what);
+			r->descsValid = true;
+		}
+		PHX_HIP(launchBatches(r->descs, r->offsets, 2u, off[2], closest,
+		                      reinterpret_cast<const float4*>(scene->nodes), scene->topCount,
+		                      nullptr, s),
+		        what);
+	}
+	PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 1u, false, r->inLight, r->contrib, L, s), what);
+
+	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
+	if (r->m) {
+		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->materials,
+		                        scene->numMaterials, r->bounceInfo, s),
+		        what);
+		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
+		                    r->m, seed, frameIndex, 3u, r->contrib, r->shadow, s),
+		        what);
+		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, anyhit, s), what);
+		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
+		        what);
+	}
+
+	r->lastOut = r->color;
+	if (taa) {
+		const bool reset = (flags & PHX_FRAME_TAA_RESET) || !r->haveHistory;
+		const uint32_t next = r->histIndex ^ 1u;
+		PHX_HIP(launchTaa(r->color, r->hist[r->histIndex], r->gbuf, r->w, r->h, reset,
+		                  !(flags & PHX_FRAME_NO_CLAMP), r->hist[next], r->out, s),
+		        what);
+		r->histIndex = next;
+		r->haveHistory = true;
+		r->lastOut = r->out;
+	}
+	if (rgba)
+		PHX_HIP(hipMemcpyAsync(rgba, r->lastOut, size_t(r->n) * 16u,
+		                       device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s),
+		        what);
+	if (!async) PHX_HIP(hipStreamSynchronize(s), what);
+	return ok();
+}
+
+}  // extern "C"

@@ -298,3 +298,38 @@ extern "C" phx_status phx_camera_look_at(const float pos[3], const float target[
 	normalize3(u);
 	return phx_camera_def(pos, d, u, vertFovRad, resX, resY, outCam);
 }
+
+extern "C" phx_status phx_scene_materials_random(uint32_t count, uint64_t seed, PhxMaterial* out)
+{
+	if (count && !out) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_materials_random: NULL");
+	SplitMix64 rng(mix64(seed, 4));
+	for (uint32_t i = 0; i < count; i++) {
+		PhxMaterial& m = out[i];
+		std::memset(&m, 0, sizeof(m));
+		m.iData[0] = m.iData[1] = m.iData[2] = -1;
+		for (int k = 0; k < 3; k++) m.albedo[k] = float(rng.uniform());
+		m.albedo[3] = 1.0f;
+		m.fData2[0] = float(0.05 + 0.95 * rng.uniform());
+		m.fData2[1] = 0.0f;
+	}
+	return ok();
+}
+
+extern "C" phx_status phx_scene_lights_random(uint32_t count, uint64_t seed, PhxSphereLight* out)
+{
+	if (count && !out) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_lights_random: NULL");
+	SplitMix64 rng(mix64(seed, 5));
+	for (uint32_t i = 0; i < count; i++) {
+		PhxSphereLight& l = out[i];
+		std::memset(&l, 0, sizeof(l));
+		l.pos[0] = float(rng.uniform());
+		l.pos[1] = float(0.3 + 1.2 * rng.uniform());
+		l.pos[2] = float(rng.uniform());
+		l.radius = 0.0f;
+		l.range = 3.0f;
+		for (int k = 0; k < 3; k++) l.strength[k] = float(0.5 + 1.5 * rng.uniform());
+		l.staticShadows = 1;
+		l.dynamicShadows = 1;
+	}
+	return ok();
+}

@@ -25,6 +25,9 @@ assert RAY.itemsize == 32 and HIT.itemsize == 16 and MATERIAL.itemsize == 48
 assert LIGHT.itemsize == 36
 
 MISS = 0xFFFFFFFF
+HITINFO = np.dtype([("pos", "<f4", (3,)), ("roughness", "<f4"), ("normal", "<f4", (3,)),
+                    ("metallic", "<f4"), ("albedo", "<f4", (3,)), ("alpha", "<f4")])
+assert HITINFO.itemsize == 48
 COUNTER_NAMES = ("nodesVisited", "trianglesIntersected", "triangleIntersectionTests",
                  "aabbIntersectionTests", "aabbIntersections")
 FLT_MAX = float(np.finfo(np.float32).max)
@@ -44,6 +47,10 @@ BATCH_ANYHIT = 1
 PLAN_TRACK_COMPLETION = 0x10
 PLAN_USE_GRAPH = 0x20
 AUTOTUNE_ANYHIT = 0x1000
+FRAME_TAA = 0x10000
+FRAME_TAA_RESET = 0x20000
+FRAME_NO_CLAMP = 0x40000
+BUFFER_COLOR, BUFFER_OUTPUT, BUFFER_GBUFFER, BUFFER_BOUNCE_RAYS, BUFFER_HISTORY = range(5)
 RAYS_ROW_MAJOR = 0
 RAYS_TILED_8X8 = 1
 
@@ -124,6 +131,14 @@ _SIGNATURES = {
                                        _P, _U32]),
     "phx_scene_random_triangles": (C.c_int, [_U32, C.c_uint64, _F32, _F32, _P, _P]),
     "phx_scene_heightfield_boxes_count": (_U32, [_U32, _U32]),
+    "phx_scene_materials_random": (C.c_int, [_U32, C.c_uint64, _P]),
+    "phx_scene_lights_random": (C.c_int, [_U32, C.c_uint64, _P]),
+    "phx_renderer_create": (C.c_int, [_P, _U32, _U32, C.POINTER(_P)]),
+    "phx_renderer_release": (None, [_P]),
+    "phx_render_frame": (C.c_int, [_P, _P, _P, C.POINTER(CameraDef), _U32, C.c_uint64, _U32,
+                                   _P]),
+    "phx_renderer_buffer": (C.c_int, [_P, _U32, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "phx_halton23": (None, [_U32, _PF, _PF]),
     "phx_scene_heightfield_boxes": (C.c_int, [_U32, _U32, C.c_uint64, _F32, _F32, _F32, _U32,
                                               _P, _P]),
 }
@@ -187,6 +202,24 @@ def heightfield_boxes(grid: int, num_boxes: int, seed: int, amp: float = 0.1,
     check(lib().phx_scene_heightfield_boxes(grid, num_boxes, seed, amp, box_mu, box_sigma,
                                             num_materials, _ptr(tris), _ptr(datas)))
     return tris, datas
+
+
+def materials_random(count: int, seed: int) -> np.ndarray:
+    out = np.zeros(count, MATERIAL)
+    check(lib().phx_scene_materials_random(count, seed, _ptr(out)))
+    return out
+
+
+def lights_random(count: int, seed: int) -> np.ndarray:
+    out = np.zeros(count, LIGHT)
+    check(lib().phx_scene_lights_random(count, seed, _ptr(out)))
+    return out
+
+
+def halton23(frame: int) -> tuple[float, float]:
+    jx, jy = C.c_float(), C.c_float()
+    lib().phx_halton23(frame, C.byref(jx), C.byref(jy))
+    return jx.value, jy.value
 
 
 class BVH:
@@ -327,6 +360,49 @@ class Plan:
             self.handle = None
 
 
+class Renderer:
+    """Frame buffers of one resolution; render() runs the wavefront frame (+ TAA)."""
+
+    def __init__(self, ctx: "Context", width: int, height: int):
+        self.ctx, self.width, self.height = ctx, width, height
+        h = C.c_void_p()
+        check(lib().phx_renderer_create(ctx.handle, width, height, C.byref(h)))
+        self.handle = h.value
+
+    def render(self, scene: "Scene", cam: CameraDef, frame: int, seed: int, flags: int = 0,
+               out: np.ndarray | None = None, out_ptr: int | None = None,
+               asynchronous: bool = False) -> np.ndarray | None:
+        """Host output (a (h, w, 4) float32 array) unless out_ptr (device) is given."""
+        if out_ptr is not None:
+            f = flags | MEM_DEVICE | (ASYNC if asynchronous else 0)
+            check(lib().phx_render_frame(self.ctx.handle, self.handle, scene.handle,
+                                         C.byref(cam), frame, seed, f, out_ptr))
+            return None
+        if out is None:
+            out = np.empty((self.height, self.width, 4), np.float32)
+        check(lib().phx_render_frame(self.ctx.handle, self.handle, scene.handle, C.byref(cam),
+                                     frame, seed, flags, _ptr(out)))
+        return out
+
+    def buffer(self, which: int) -> tuple[int, int]:
+        p, n = C.c_void_p(), C.c_size_t()
+        check(lib().phx_renderer_buffer(self.handle, which, C.byref(p), C.byref(n)))
+        return p.value, n.value
+
+    def download(self, which: int, dtype, count: int | None = None) -> np.ndarray:
+        ptr, nbytes = self.buffer(which)
+        dtype = np.dtype(dtype)
+        count = nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        check(lib().phx_copy_to_host(self.ctx.handle, _ptr(out), ptr, count * dtype.itemsize))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            lib().phx_renderer_release(self.handle)
+            self.handle = None
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = C.c_void_p()
@@ -433,6 +509,9 @@ class Context:
                                 order: int = RAYS_ROW_MAJOR):
         check(lib().phx_gen_primary_rays(self.handle, C.byref(cam), w, h, t_min, t_max, order,
                                          out_ptr, MEM_DEVICE))
+
+    def renderer(self, width: int, height: int) -> Renderer:
+        return Renderer(self, width, height)
 
     def copy_on_device(self, dst: int, src: int, nbytes: int) -> None:
         check(lib().phx_copy_on_device(self.handle, dst, src, nbytes))
