@@ -226,6 +226,61 @@ def frame_form(ctx, scene, ev, lights: int, frames: int = 16) -> dict:
             "taa_samples_after": float(hist[..., 3].min())}
 
 
+def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int = 3) -> dict:
+    """Poisson-arriving requests against 4 resident scenes through the request server, at
+    saturation and at 25/50/90% of the saturated request rate, beside the plain per-batch server
+    on the same requests (identical outputs)."""
+    from phantasyenginetestbed_amd import phx, workloads as W
+    t0 = time.perf_counter()
+    scenes = [W.served_scene(ctx, name) for name in ("100k", "1M", "4M", "8M")]
+    log(f"served: 4 scenes ({', '.join(f'{s.name}={s.triangles}' for s in scenes)}) "
+        f"in {time.perf_counter() - t0:.1f}s")
+    rs = W.request_set(len(scenes), n_requests)
+    W.allocate_outputs(rs)
+    reqs = W.server_requests(scenes, rs)
+    total = rs.total
+
+    def run(server, arrivals):
+        lat, mk = server.replay(reqs, arrivals)
+        return {"mrays": round(total / mk / 1e3, 1), "makespan_ms": round(mk, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+    srv = phx.Server(ctx, int(budget_gb * (1 << 30)), slots=slots)
+    for s in scenes:
+        srv.add_scene(s.scene)
+    zeros = np.zeros(n_requests)
+    run(srv, zeros)  # warm-up
+    sat = run(srv, zeros)
+    served_out = rs.out.array.copy()
+    rate = n_requests / sat["makespan_ms"]  # requests per ms at saturation
+    rates = {}
+    for frac in (0.25, 0.5, 0.9):
+        arr = W.poisson_arrivals(n_requests, frac * rate, seed=int(frac * 100))
+        rates[f"{int(frac * 100)}%"] = {"offered_req_per_s": round(frac * rate * 1e3, 1),
+                                        **run(srv, arr)}
+    st = srv.stats()
+    srv.destroy()
+
+    plain = phx.Server(ctx, int(budget_gb * (1 << 30)), slots=1, plain=True)
+    for s in scenes:
+        plain.add_scene(s.scene)
+    run(plain, zeros)
+    psat = run(plain, zeros)
+    same = bool(np.array_equal(served_out, rs.out.array))
+    plain.destroy()
+    rs.out.free()
+    for s in scenes:
+        W.free_served_scene(s)
+    return {"scenes": {s.name: s.triangles for s in scenes}, "requests": n_requests,
+            "rays": total, "budget_gb": budget_gb, "slots": slots,
+            "served_saturated": sat, "served_at_rates": rates, "plain_saturated": psat,
+            "served_vs_plain": round(sat["mrays"] / psat["mrays"], 2),
+            "identical_outputs": same, "launches": int(st["launches"]),
+            "peak_request_bytes": int(st["peakBytesInUse"]),
+            "note": "host-resident (pinned) request buffers: times include PCIe copies"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +291,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-frame", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
                     help="fixed persistent launch shape instead of the autotuner's (profiling)")
@@ -349,6 +405,9 @@ def main() -> int:
     if not args.no_frame:
         result["frame"] = frame_form(ctx, scene, ev, len(lights))
         log(f"frame: {result['frame']}")
+    if not args.no_served:
+        result["served"] = served_form(ctx)
+        log(f"served: {result['served']}")
     if not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(bvh.nodes, bvh.tris, bounce, gpu_hits)

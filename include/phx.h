@@ -361,6 +361,72 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
                                uint32_t frame, PhxRayIn* outRays, uint32_t flags);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Served form: a request loop over resident scenes with continuous batching. No counterpart in  */
+/* the reference, whose renderer issues one synchronous launch per ray batch                  */
+/* (RayCastKernel.cu:806-886); these calls replace that pattern for many concurrent clients.   */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct PhxServer PhxServer;
+
+/* One ray-query request. With PHX_MEM_DEVICE in flags, rays/out are device pointers used in place;
+ * otherwise they are host pointers (pinned memory from phx_host_alloc copies fastest) and the
+ * server copies them through its request arena. out: PhxRayHit per ray (closest) or one byte per
+ * ray, 1 = unoccluded (any hit). */
+typedef struct PhxRequest {
+	const PhxScene* scene;
+	const PhxRayIn* rays;
+	void* out;
+	uint32_t numRays;
+	uint32_t kind; /* PHX_BATCH_CLOSEST or PHX_BATCH_ANYHIT */
+	uint32_t flags;
+	uint32_t pad;
+} PhxRequest;
+
+typedef struct PhxServerStats {
+	uint64_t requestsSubmitted;
+	uint64_t requestsDone;
+	uint64_t raysDone;
+	uint64_t launches;
+	uint64_t rejected;      /* requests larger than the whole request budget */
+	uint64_t bytesInUse;    /* request arena bytes held by admitted requests */
+	uint64_t peakBytesInUse;
+	uint64_t requestBudget; /* budget minus resident scene bytes */
+	uint64_t sceneBytes;
+	uint32_t queued;        /* submitted requests not yet launched */
+	uint32_t pad;
+} PhxServerStats;
+
+#define PHX_SERVE_PLAIN 0x1u /* one request per launch, one launch in flight (per-batch baseline) */
+
+/* A server on ctx's device with an HBM budget (resident scenes + request buffers) and `slots`
+ * (1..4) launch streams. Requests are admitted in arrival order while their buffers fit, and all
+ * admitted requests are coalesced into the next fused persistent launch on an idle slot; a launch
+ * formed while another runs fills that launch's tail. A dispatcher thread owns the device work. */
+phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t slots,
+                             uint32_t flags, PhxServer** outServer);
+/* Waits for every submitted request, then frees the server. */
+void phx_server_destroy(PhxServer* server);
+/* Makes a scene of the server's context resident: its bytes count against the budget
+ * (PHX_ERR_BUDGET if they do not fit); call before submitting requests. */
+phx_status phx_server_add_scene(PhxServer* server, const PhxScene* scene);
+/* Queues a request; never blocks. Fails with PHX_ERR_BUDGET if the request alone exceeds the
+ * request budget, PHX_ERR_INVALID_HANDLE if its scene is not resident. */
+phx_status phx_server_submit(PhxServer* server, const PhxRequest* request, uint64_t* ticket);
+/* done = 1 once the request's outputs are in place; latency = submit to completion (ms). */
+phx_status phx_server_poll(PhxServer* server, uint64_t ticket, int* done, double* latencyMs);
+/* Blocks until the request completes, then forgets the ticket. */
+phx_status phx_server_wait(PhxServer* server, uint64_t ticket, double* latencyMs);
+phx_status phx_server_stats(PhxServer* server, PhxServerStats* stats);
+/* Load driver: submits reqs[i] at arrivalMs[i] after the call starts (open loop), waits for all,
+ * and returns each request's latency and the time from the call's start to the last completion. */
+phx_status phx_server_replay(PhxServer* server, const PhxRequest* reqs, const double* arrivalMs,
+                             uint32_t n, double* latencyMs, double* makespanMs);
+
+/* Pinned host memory (hipHostMalloc), for request buffers that copy at full PCIe rate. */
+phx_status phx_host_alloc(size_t bytes, void** out);
+phx_status phx_host_free(void* p);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Render frame: the wavefront path of CudaTracerRenderer::render (CudaTracerRenderer.cpp:637-   */
 /* 720) and the TAA pass of GameScreen::render (GameScreen.cpp:187-251)                        */
 /* ------------------------------------------------------------------------------------------ */

@@ -47,6 +47,7 @@ BATCH_ANYHIT = 1
 PLAN_TRACK_COMPLETION = 0x10
 PLAN_USE_GRAPH = 0x20
 AUTOTUNE_ANYHIT = 0x1000
+SERVE_PLAIN = 0x1
 FRAME_TAA = 0x10000
 FRAME_TAA_RESET = 0x20000
 FRAME_NO_CLAMP = 0x40000
@@ -62,6 +63,23 @@ class CameraDef(C.Structure):
     def as_array(self) -> np.ndarray:
         return np.array(list(self.origin) + list(self.dir) + list(self.dX) + list(self.dY),
                         dtype=np.float32)
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("rays", C.c_void_p), ("out", C.c_void_p),
+                ("numRays", C.c_uint32), ("kind", C.c_uint32), ("flags", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requestsSubmitted", C.c_uint64), ("requestsDone", C.c_uint64),
+                ("raysDone", C.c_uint64), ("launches", C.c_uint64), ("rejected", C.c_uint64),
+                ("bytesInUse", C.c_uint64), ("peakBytesInUse", C.c_uint64),
+                ("requestBudget", C.c_uint64), ("sceneBytes", C.c_uint64),
+                ("queued", C.c_uint32), ("pad", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
 class LaunchParams(C.Structure):
@@ -139,6 +157,18 @@ _SIGNATURES = {
                                    _P]),
     "phx_renderer_buffer": (C.c_int, [_P, _U32, C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "phx_halton23": (None, [_U32, _PF, _PF]),
+    "phx_server_create": (C.c_int, [_P, C.c_uint64, _U32, _U32, C.POINTER(_P)]),
+    "phx_server_destroy": (None, [_P]),
+    "phx_server_add_scene": (C.c_int, [_P, _P]),
+    "phx_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "phx_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]),
+    "phx_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
+    "phx_server_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "phx_server_replay": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_double), _U32,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "phx_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),
+    "phx_host_free": (C.c_int, [_P]),
     "phx_scene_heightfield_boxes": (C.c_int, [_U32, _U32, C.c_uint64, _F32, _F32, _F32, _U32,
                                               _P, _P]),
 }
@@ -357,6 +387,80 @@ class Plan:
     def release(self) -> None:
         if self.handle:
             lib().phx_plan_release(self.handle)
+            self.handle = None
+
+
+class HostBuffer:
+    """Pinned host memory (phx_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, dtype, count: int):
+        self.dtype = np.dtype(dtype)
+        self.count = count
+        p = C.c_void_p()
+        check(lib().phx_host_alloc(max(1, count * self.dtype.itemsize), C.byref(p)))
+        self.ptr = p.value
+        raw = (C.c_uint8 * max(1, count * self.dtype.itemsize)).from_address(self.ptr)
+        self.array = np.frombuffer(raw, np.uint8)[:count * self.dtype.itemsize].view(self.dtype)
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            lib().phx_host_free(self.ptr)
+            self.ptr = None
+
+
+class Server:
+    """Request loop over resident scenes (phx_server_*)."""
+
+    def __init__(self, ctx: "Context", budget_bytes: int, slots: int = 3, plain: bool = False):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().phx_server_create(ctx.handle, budget_bytes, slots,
+                                      SERVE_PLAIN if plain else 0, C.byref(h)))
+        self.handle = h.value
+
+    def add_scene(self, scene: "Scene") -> None:
+        check(lib().phx_server_add_scene(self.handle, scene.handle))
+
+    @staticmethod
+    def request(scene: "Scene", rays_ptr: int, out_ptr: int, n: int, kind: int,
+                device: bool = False) -> Request:
+        return Request(scene.handle, rays_ptr, out_ptr, n, kind, MEM_DEVICE if device else 0, 0)
+
+    def submit(self, req: Request) -> int:
+        t = C.c_uint64()
+        check(lib().phx_server_submit(self.handle, C.byref(req), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int) -> tuple[bool, float]:
+        d, lat = C.c_int(), C.c_double()
+        check(lib().phx_server_poll(self.handle, ticket, C.byref(d), C.byref(lat)))
+        return bool(d.value), lat.value
+
+    def wait(self, ticket: int) -> float:
+        lat = C.c_double()
+        check(lib().phx_server_wait(self.handle, ticket, C.byref(lat)))
+        return lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().phx_server_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def replay(self, reqs: list, arrivals_ms) -> tuple[np.ndarray, float]:
+        n = len(reqs)
+        arr = (Request * max(1, n))(*reqs)
+        at = np.ascontiguousarray(arrivals_ms, np.float64)
+        lat = np.zeros(max(1, n), np.float64)
+        mk = C.c_double()
+        check(lib().phx_server_replay(self.handle, arr, at.ctypes.data_as(C.POINTER(C.c_double)),
+                                      n, lat.ctypes.data_as(C.POINTER(C.c_double)),
+                                      C.byref(mk)))
+        return lat[:n], mk.value
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().phx_server_destroy(self.handle)
             self.handle = None
 
 

@@ -130,3 +130,129 @@ def download_outputs(bs: BatchSet) -> list:
     for k, r, (rb, ob) in zip(bs.kinds, bs.rays, bs.device):
         out.append(ob.download(np.uint8 if k == "shadow" else phx.HIT, len(r)))
     return out
+
+
+# ---- served form --------------------------------------------------------------------------------
+#
+#   4 resident scenes: 100k random triangles (seed 0xC0FFEE01), the 1,004,000-triangle heightfield,
+#   a 1414x1414 heightfield + 2,000 boxes (4,022,792 triangles) and a 2000x2000 heightfield + 2,000
+#   boxes (8,024,000 triangles); per scene ray pools of primary, diffuse-bounce and shadow rays, repeated to
+#   4,194,304 rays and held in pinned host memory. Requests: sizes lognormal(ln 32768, 1.5) clipped
+#   to [4096, 4194304], kinds 40% primary / 40% diffuse / 20% shadow, scene uniform (seed
+#   0xC0FFEE05), each a contiguous slice of its pool.
+
+SERVE_SEED = 0xC0FFEE05
+RANDOM_CAM = ((0.5, 0.5, -1.5), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0))
+RANDOM_LIGHT = (0.5, 2.0, 0.5)
+POOL_RAYS = 4194304
+
+
+@dataclass
+class ServedScene:
+    name: str
+    triangles: int
+    scene: object            # phx.Scene
+    pools: dict              # kind -> phx.HostBuffer of RAY (POOL_RAYS records)
+
+
+def _scene_geometry(name: str):
+    if name == "100k":
+        t, d = phx.random_triangles(100000, 0xC0FFEE01)
+        return t, d, RANDOM_CAM, RANDOM_LIGHT, (1024, 1024)
+    if name == "1M":
+        t, d = phx.heightfield_boxes(700, 2000, SCENE_SEED)
+        return t, d, CAMERA, LIGHT, (1920, 1080)
+    if name == "4M":
+        t, d = phx.heightfield_boxes(1414, 2000, SCENE_SEED)
+        return t, d, CAMERA, LIGHT, (1920, 1080)
+    if name == "8M":
+        t, d = phx.heightfield_boxes(2000, 2000, SCENE_SEED)
+        return t, d, CAMERA, LIGHT, (1920, 1080)
+    raise ValueError(name)
+
+
+def _tile(rays: np.ndarray, n: int, dst: np.ndarray) -> None:
+    reps = -(-n // len(rays))
+    dst[:] = np.tile(rays, reps)[:n]
+
+
+def served_scene(ctx: phx.Context, name: str, pool_rays: int = POOL_RAYS) -> ServedScene:
+    t, d, cam_spec, light, (w, h) = _scene_geometry(name)
+    bvh = phx.BVH(t, d)
+    scene = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas)
+    cam = phx.camera_look_at(*cam_spec, math.radians(FOV_DEG), w, h)
+    prim = ctx.gen_primary_rays(cam, w, h)
+    hits = ctx.cast_closest(scene, prim)
+    valid = hits["tri"] != phx.MISS
+    bounce = ctx.gen_bounce_rays(scene, prim, hits, BOUNCE_SEED, 0)[valid]
+    shadow = shadow_rays(prim, hits, light)
+    pools = {}
+    for kind, rays in (("primary", prim), ("diffuse", bounce), ("shadow", shadow)):
+        hb = phx.HostBuffer(phx.RAY, pool_rays)
+        _tile(rays, pool_rays, hb.array)
+        pools[kind] = hb
+    return ServedScene(name, len(t), scene, pools)
+
+
+def free_served_scene(s: ServedScene) -> None:
+    for hb in s.pools.values():
+        hb.free()
+    s.scene.release()
+
+
+@dataclass
+class RequestSet:
+    scene_index: np.ndarray
+    kinds: list
+    offsets: np.ndarray
+    sizes: np.ndarray
+    out: object = None          # phx.HostBuffer (uint8) holding every request's output
+    out_offsets: np.ndarray = None
+
+    @property
+    def total(self) -> int:
+        return int(self.sizes.sum())
+
+
+def request_set(num_scenes: int, n: int, seed: int = SERVE_SEED, median: int = 32768,
+                sigma: float = 1.5, lo: int = 4096, hi: int = 4194304, mix=(0.4, 0.4, 0.2),
+                pool_rays: int = POOL_RAYS) -> RequestSet:
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(math.log(median) + sigma * rng.standard_normal(n)), lo, hi)
+    sizes = np.minimum(sizes.astype(np.int64), pool_rays)
+    kinds = list(rng.choice(["primary", "diffuse", "shadow"], size=n, p=list(mix)))
+    scene_index = rng.integers(0, num_scenes, n)
+    offsets = np.array([int(rng.integers(0, pool_rays - s + 1)) for s in sizes], np.int64)
+    return RequestSet(scene_index, kinds, offsets, sizes)
+
+
+def allocate_outputs(rs: RequestSet) -> None:
+    per = np.array([1 if k == "shadow" else 16 for k in rs.kinds], np.int64)
+    nbytes = (rs.sizes * per + 255) // 256 * 256
+    rs.out_offsets = np.concatenate([[0], np.cumsum(nbytes)[:-1]])
+    rs.out = phx.HostBuffer(np.uint8, int(nbytes.sum()))
+
+
+def server_requests(scenes: list, rs: RequestSet) -> list:
+    reqs = []
+    for i in range(len(rs.sizes)):
+        s = scenes[int(rs.scene_index[i])]
+        pool = s.pools[rs.kinds[i]]
+        kind = phx.BATCH_ANYHIT if rs.kinds[i] == "shadow" else phx.BATCH_CLOSEST
+        reqs.append(phx.Server.request(s.scene, pool.ptr + int(rs.offsets[i]) * 32,
+                                       rs.out.ptr + int(rs.out_offsets[i]), int(rs.sizes[i]),
+                                       kind))
+    return reqs
+
+
+def request_output(rs: RequestSet, i: int) -> np.ndarray:
+    n = int(rs.sizes[i])
+    o = int(rs.out_offsets[i])
+    if rs.kinds[i] == "shadow":
+        return rs.out.array[o:o + n].copy()
+    return rs.out.array[o:o + 16 * n].view(phx.HIT).copy()
+
+
+def poisson_arrivals(n: int, rate_per_ms: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_ms, n))

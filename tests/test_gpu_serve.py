@@ -1,0 +1,204 @@
+"""The request server through the C ABI: every request's result is the same alone and in the mix
+(bit-identical, both ray kinds, host and device buffers), admission respects the HBM budget, and
+the plain per-batch mode gives the same results."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from phantasyenginetestbed_amd import phx
+from phantasyenginetestbed_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = phx.Context(0)
+    yield c
+    c.destroy()
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    say("[gpu-serve] scenes")
+    out = []
+    for grid, boxes in ((48, 30), (80, 100)):
+        t, d = phx.heightfield_boxes(grid, boxes, 0xC0FFEE02)
+        b = phx.BVH(t, d)
+        s = ctx.upload_scene(b.nodes, b.tris, b.datas)
+        cam = W.camera(320, 200)
+        prim = ctx.gen_primary_rays(cam, 320, 200)
+        hits = ctx.cast_closest(s, prim)
+        bounce = ctx.gen_bounce_rays(s, prim, hits, 3, 0)[hits["tri"] != phx.MISS]
+        pools = {"primary": prim, "diffuse": bounce, "shadow": W.shadow_rays(prim, hits)}
+        out.append((b, s, pools))
+    yield out
+    for _, s, _ in out:
+        s.release()
+
+
+def make_requests(scenes, n, seed, max_rays=20000):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        si = int(rng.integers(0, len(scenes)))
+        kind = ["primary", "diffuse", "shadow"][int(rng.integers(0, 3))]
+        pool = scenes[si][2][kind]
+        m = int(rng.integers(0, min(max_rays, len(pool)) + 1)) if i % 7 else int(rng.integers(0, 3))
+        off = int(rng.integers(0, len(pool) - m + 1))
+        specs.append((si, kind, np.ascontiguousarray(pool[off:off + m])))
+    return specs
+
+
+def host_buffers(specs):
+    bufs = []
+    for _, kind, rays in specs:
+        rb = phx.HostBuffer(phx.RAY, max(1, len(rays)))
+        rb.array[:len(rays)] = rays
+        ob = phx.HostBuffer(np.uint8 if kind == "shadow" else phx.HIT, max(1, len(rays)))
+        bufs.append((rb, ob))
+    return bufs
+
+
+def alone(ctx, scenes, specs):
+    res = []
+    for si, kind, rays in specs:
+        s = scenes[si][1]
+        res.append(ctx.cast_anyhit(s, rays) if kind == "shadow" else ctx.cast_closest(s, rays))
+    return res
+
+
+@pytest.mark.parametrize("slots", [1, 3])
+def test_results_identical_alone_and_in_mix(ctx, scenes, slots):
+    specs = make_requests(scenes, 80, seed=slots)
+    bufs = host_buffers(specs)
+    want = alone(ctx, scenes, specs)
+    srv = phx.Server(ctx, 1 << 30, slots=slots)
+    try:
+        for _, s, _ in scenes:
+            srv.add_scene(s)
+        reqs = [phx.Server.request(scenes[si][1], rb.ptr, ob.ptr, len(rays),
+                                   phx.BATCH_ANYHIT if kind == "shadow" else phx.BATCH_CLOSEST)
+                for (si, kind, rays), (rb, ob) in zip(specs, bufs)]
+        for label, arrivals in (("burst", np.zeros(len(reqs))),
+                                ("poisson", W.poisson_arrivals(len(reqs), 2.0, 1))):
+            say(f"[gpu-serve] slots={slots} {label}")
+            for _, ob in bufs:
+                ob.array.view(np.uint8)[:] = 0xAB
+            lat, mk = srv.replay(reqs, arrivals)
+            assert (lat > 0).all() and mk > 0
+            for i, ((si, kind, rays), (rb, ob)) in enumerate(zip(specs, bufs)):
+                got = ob.array[:len(rays)]
+                assert got.tobytes() == want[i].tobytes(), f"request {i} ({kind}, {len(rays)})"
+        st = srv.stats()
+        assert st["requestsDone"] == 2 * len(reqs) and st["queued"] == 0
+        assert st["launches"] <= 2 * len(reqs)
+    finally:
+        srv.destroy()
+        for rb, ob in bufs:
+            rb.free()
+            ob.free()
+    # the served results are the oracle's too
+    si, kind, rays = specs[1]
+    b = scenes[si][0]
+    ref = O.cast_anyhit(b.nodes, b.tris, rays) if kind == "shadow" else \
+        O.cast_closest(b.nodes, b.tris, rays)
+    assert ref.tobytes() == want[1].tobytes()
+
+
+def test_plain_mode_and_device_requests(ctx, scenes):
+    specs = make_requests(scenes, 30, seed=9)
+    want = alone(ctx, scenes, specs)
+    plain = phx.Server(ctx, 1 << 30, slots=1, plain=True)
+    srv = phx.Server(ctx, 1 << 30, slots=2)
+    dev = []
+    try:
+        for _, s, _ in scenes:
+            plain.add_scene(s)
+            srv.add_scene(s)
+        bufs = host_buffers(specs)
+        reqs = [phx.Server.request(scenes[si][1], rb.ptr, ob.ptr, len(rays),
+                                   phx.BATCH_ANYHIT if kind == "shadow" else phx.BATCH_CLOSEST)
+                for (si, kind, rays), (rb, ob) in zip(specs, bufs)]
+        say("[gpu-serve] plain mode")
+        plain.replay(reqs, np.zeros(len(reqs)))
+        for i, (rb, ob) in enumerate(bufs):
+            assert ob.array[:len(specs[i][2])].tobytes() == want[i].tobytes()
+        assert plain.stats()["launches"] == len(reqs)
+        say("[gpu-serve] device-resident requests")
+        dreqs = []
+        for si, kind, rays in specs:
+            rb = ctx.alloc(max(1, len(rays)) * 32).upload(rays) if len(rays) else ctx.alloc(32)
+            ob = ctx.alloc(max(1, len(rays)) * (1 if kind == "shadow" else 16))
+            dev.append((rb, ob))
+            dreqs.append(phx.Server.request(scenes[si][1], rb.ptr, ob.ptr, len(rays),
+                                            phx.BATCH_ANYHIT if kind == "shadow"
+                                            else phx.BATCH_CLOSEST, device=True))
+        tickets = [srv.submit(r) for r in dreqs]
+        for t in tickets:
+            assert srv.wait(t) > 0
+        for i, ((si, kind, rays), (rb, ob)) in enumerate(zip(specs, dev)):
+            if len(rays):
+                got = ob.download(np.uint8 if kind == "shadow" else phx.HIT, len(rays))
+                assert got.tobytes() == want[i].tobytes()
+        assert srv.stats()["peakBytesInUse"] == 0  # device requests use no arena memory
+        for rb, ob in bufs:
+            rb.free()
+            ob.free()
+    finally:
+        plain.destroy()
+        srv.destroy()
+        for rb, ob in dev:
+            rb.free()
+            ob.free()
+
+
+def test_admission_budget(ctx, scenes):
+    _, s0, pools = scenes[0]
+    scene_bytes = s0.device_bytes()
+    budget = scene_bytes + (1 << 20)  # 1 MiB of request memory
+    srv = phx.Server(ctx, budget, slots=2)
+    bufs = []
+    try:
+        srv.add_scene(s0)
+        st = srv.stats()
+        assert st["requestBudget"] == (1 << 20) and st["sceneBytes"] == scene_bytes
+        with pytest.raises(phx.PhxError) as ei:  # 1 MiB cannot hold 30000 rays (1.44 MB)
+            rb = phx.HostBuffer(phx.RAY, 30000)
+            ob = phx.HostBuffer(phx.HIT, 30000)
+            bufs.append((rb, ob))
+            srv.submit(phx.Server.request(s0, rb.ptr, ob.ptr, 30000, phx.BATCH_CLOSEST))
+        assert ei.value.status == phx.ERR_BUDGET
+        assert srv.stats()["rejected"] == 1
+        # 40 requests of 8000 rays (384 KB each): at most 2 fit at once, all complete in order
+        rays = np.ascontiguousarray(pools["diffuse"][:8000])
+        want = ctx.cast_closest(s0, rays)
+        reqs = []
+        for _ in range(40):
+            rb = phx.HostBuffer(phx.RAY, 8000)
+            rb.array[:] = rays
+            ob = phx.HostBuffer(phx.HIT, 8000)
+            bufs.append((rb, ob))
+            reqs.append(phx.Server.request(s0, rb.ptr, ob.ptr, 8000, phx.BATCH_CLOSEST))
+        say("[gpu-serve] admission under a 1 MiB request budget")
+        lat, _ = srv.replay(reqs, np.zeros(len(reqs)))
+        for rb, ob in bufs[1:]:
+            assert ob.array.tobytes() == want.tobytes()
+        st = srv.stats()
+        assert 0 < st["peakBytesInUse"] <= (1 << 20) and st["bytesInUse"] == 0
+        assert st["launches"] >= 20
+        # a scene that is not resident is refused; so is a scene over the budget
+        with pytest.raises(phx.PhxError):
+            srv.submit(phx.Server.request(scenes[1][1], bufs[1][0].ptr, bufs[1][1].ptr, 10,
+                                          phx.BATCH_CLOSEST))
+        with pytest.raises(phx.PhxError):
+            srv.add_scene(scenes[1][1])
+    finally:
+        srv.destroy()
+        for rb, ob in bufs:
+            rb.free()
+            ob.free()
