@@ -23,6 +23,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libphx.so"
+CHECK = PKG / "libhiptracer_check.so"   # engine-side check of the C++ renderer interface
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
@@ -68,14 +69,23 @@ def build(verbose: bool = True) -> Path:
     _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC",
           *map(str, objs), "-o", str(tmp), "-lpthread"])
     os.replace(tmp, LIB)
+    # The C++ renderer interface over the C ABI and its engine-side check program.
+    check_tmp = CHECK.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-I", str(ROOT / "include"),
+          str(ROOT / "tests" / "cpp" / "hiptracer_check.cpp"),
+          str(PKG / "cpp" / "HipTracerRenderer.cpp"), "-L", str(PKG), "-lphx",
+          "-Wl,-rpath,$ORIGIN", "-o", str(check_tmp)])
+    os.replace(check_tmp, CHECK)
     if verbose:
-        print(f"built {LIB.relative_to(ROOT)} from {len(sources)} sources", flush=True)
+        print(f"built {LIB.relative_to(ROOT)} from {len(sources)} sources and "
+              f"{CHECK.relative_to(ROOT)}", flush=True)
     return LIB
 
 
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
     LIB.unlink(missing_ok=True)
+    CHECK.unlink(missing_ok=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,179 @@
+// Engine-side check of the C++ renderer interface (HipTracerRenderer) and the kernel-level launch
+// functions, written the way engine code uses the reference's CudaTracerRenderer: set materials
+// and the static scene, set the target resolution, update the camera, render frames. Every frame
+// must equal the C ABI's phx_render_frame on the same inputs, byte for byte, and the launch
+// functions must equal phx_cast_closest / phx_cast_anyhit. Built as a shared library and called in
+// process (hiptracer_check_run returns the number of failed checks).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../phantasyenginetestbed_amd/cpp/HipTracerRenderer.hpp"
+
+using namespace phe_hip;
+
+static int failures = 0;
+
+#define CHECK(cond, ...)                              \
+	do {                                              \
+		if (!(cond)) {                                \
+			std::printf("FAILED %s:%d: ", __FILE__, __LINE__); \
+			std::printf(__VA_ARGS__);                 \
+			std::printf("\n");                        \
+			failures++;                               \
+		}                                             \
+	} while (0)
+
+static vec3 normalized(double x, double y, double z)
+{
+	const double l = std::sqrt(x * x + y * y + z * z);
+	return vec3{float(x / l), float(y / l), float(z / l)};
+}
+
+extern "C" int hiptracer_check_run()
+{
+	failures = 0;
+	std::setvbuf(stdout, nullptr, _IONBF, 0);
+	std::printf("hiptracer_check: scene\n");
+	const uint32_t grid = 64, boxes = 40, numMats = 16, numLights = 4;
+	const uint32_t n = phx_scene_heightfield_boxes_count(grid, boxes);
+	std::vector<PhxTriangleVertices> tris(n);
+	std::vector<PhxTriangleData> datas(n);
+	CHECK(phx_scene_heightfield_boxes(grid, boxes, 0xC0FFEE02ull, 0.1f, -3.0f, 0.5f, numMats,
+	                                  tris.data(), datas.data()) == PHX_OK, "scene");
+	PhxBVH* bvh = nullptr;
+	CHECK(phx_bvh_build(tris.data(), datas.data(), n, 0, &bvh) == PHX_OK, "bvh build");
+	const PhxBVHNode* nodes = nullptr;
+	const PhxTriangleVertices* verts = nullptr;
+	const PhxTriangleData* bdatas = nullptr;
+	uint32_t numNodes = 0, numRefs = 0;
+	CHECK(phx_bvh_arrays(bvh, &nodes, &numNodes, &verts, &bdatas, &numRefs) == PHX_OK, "arrays");
+
+	StaticScene scene;
+	scene.bvh.nodes.assign(nodes, nodes + numNodes);
+	scene.bvh.triangleVerts.assign(verts, verts + numRefs);
+	scene.bvh.triangleDatas.assign(bdatas, bdatas + numRefs);
+	scene.sphereLights.resize(numLights);
+	CHECK(phx_scene_lights_random(numLights, 3, scene.sphereLights.data()) == PHX_OK, "lights");
+	std::vector<PhxMaterial> mats(numMats);
+	CHECK(phx_scene_materials_random(numMats, 3, mats.data()) == PHX_OK, "materials");
+	phx_bvh_release(bvh);
+
+	HipTracerRenderer renderer(0);
+	CHECK(renderer.lastStatus() == PHX_OK, "create: %s", renderer.lastError().c_str());
+	// render before a scene is set: recorded error, no crash
+	RenderResult none = renderer.render({}, {});
+	CHECK(none.colorDevice == nullptr && renderer.lastStatus() != PHX_OK, "render without scene");
+
+	renderer.setMaterialsAndTextures(mats, {});
+	renderer.setStaticScene(scene);
+	CHECK(renderer.lastStatus() == PHX_OK, "setStaticScene: %s", renderer.lastError().c_str());
+	const int w = 64, h = 48;
+	renderer.setTargetResolution(vec2i{w, h});
+	CHECK(renderer.lastStatus() == PHX_OK, "resolution: %s", renderer.lastError().c_str());
+
+	// ViewFrustum::set: dir = normalize(target - pos), up orthogonalised against dir.
+	ViewFrustum cam;
+	cam.pos = vec3{0.5f, 0.6f, -0.3f};
+	cam.dir = normalized(0.0, 0.05 - 0.6, 0.6 + 0.3);
+	const double d = double(cam.dir.y);
+	cam.up = normalized(-d * cam.dir.x, 1.0 - d * cam.dir.y, -d * cam.dir.z);
+	cam.verticalFovDeg = 60.0f;
+	renderer.updateCamera(cam);
+	renderer.setSeed(11);
+
+	// The same frames through the C ABI directly.
+	PhxContext* ctx = renderer.context();
+	PhxRenderer* direct = nullptr;
+	CHECK(phx_renderer_create(ctx, w, h, &direct) == PHX_OK, "direct renderer");
+	PhxCameraDef def;
+	const float pos[3] = {cam.pos.x, cam.pos.y, cam.pos.z};
+	const float dir[3] = {cam.dir.x, cam.dir.y, cam.dir.z};
+	const float up[3] = {cam.up.x, cam.up.y, cam.up.z};
+	CHECK(phx_camera_def(pos, dir, up, 60.0f * (3.14159265358979323846f / 180.0f), float(w),
+	                     float(h), &def) == PHX_OK, "camera def");
+	std::vector<float> a(size_t(w) * h * 4), b(a.size());
+	for (uint32_t f = 0; f < 4; f++) {
+		std::printf("hiptracer_check: frame %u\n", f);
+		RenderResult res = renderer.render({}, {});
+		CHECK(res.colorDevice && res.renderedRes.x == w && res.renderedRes.y == h, "render %u: %s",
+		      f, renderer.lastError().c_str());
+		CHECK(phx_copy_to_host(ctx, a.data(), res.colorDevice, a.size() * 4) == PHX_OK, "copy");
+		const uint32_t flags = PHX_FRAME_TAA | (f == 0 ? PHX_FRAME_TAA_RESET : 0u);
+		CHECK(phx_render_frame(ctx, direct, renderer.scene(), &def, f, 11, flags, b.data()) ==
+		          PHX_OK, "direct frame %u: %s", f, phx_last_error());
+		CHECK(std::memcmp(a.data(), b.data(), a.size() * 4) == 0, "frame %u differs", f);
+		double sum = 0.0;
+		for (size_t i = 0; i < a.size(); i += 4) sum += a[i] + a[i + 1] + a[i + 2];
+		CHECK(sum > 0.0 && std::isfinite(sum), "frame %u is black or not finite", f);
+	}
+	phx_renderer_release(direct);
+
+	// Unsupported engine calls record an error and change nothing.
+	renderer.setDynamicMeshes({RawMesh{}});
+	CHECK(renderer.lastStatus() == PHX_ERR_INVALID_ARGUMENT, "dynamic meshes");
+	renderer.addTexture(RawImage{});
+	CHECK(renderer.lastStatus() == PHX_ERR_INVALID_ARGUMENT, "textures");
+
+	// Kernel-level API: primary rays on the device, the three launch functions.
+	std::printf("hiptracer_check: launch functions\n");
+	const uint32_t nr = uint32_t(w) * h;
+	void *rays = nullptr, *hitsA = nullptr, *hitsB = nullptr, *light = nullptr;
+	CHECK(phx_device_alloc(ctx, nr * sizeof(PhxRayIn), &rays) == PHX_OK, "alloc");
+	CHECK(phx_device_alloc(ctx, nr * sizeof(PhxRayHit), &hitsA) == PHX_OK, "alloc");
+	CHECK(phx_device_alloc(ctx, nr * sizeof(PhxRayHit), &hitsB) == PHX_OK, "alloc");
+	CHECK(phx_device_alloc(ctx, nr, &light) == PHX_OK, "alloc");
+	CHECK(phx_gen_primary_rays(ctx, &def, w, h, 1e-4f, 3.0e38f, PHX_RAYS_TILED_8X8,
+	                           static_cast<PhxRayIn*>(rays), PHX_MEM_DEVICE) == PHX_OK, "rays");
+	RayCastKernelInput in;
+	in.scene = renderer.scene();
+	in.numRays = nr;
+	in.rays = static_cast<const PhxRayIn*>(rays);
+	CHECK(launchRayCastKernel(ctx, in, static_cast<PhxRayHit*>(hitsA)) == PHX_OK, "persistent");
+	CHECK(launchRayCastNoPersistenceKernel(ctx, in, static_cast<PhxRayHit*>(hitsB)) == PHX_OK,
+	      "one ray per lane");
+	std::vector<PhxRayIn> hostRays(nr);
+	std::vector<PhxRayHit> ha(nr), hb(nr), hc(nr);
+	CHECK(phx_copy_to_host(ctx, hostRays.data(), rays, nr * sizeof(PhxRayIn)) == PHX_OK, "copy");
+	CHECK(phx_copy_to_host(ctx, ha.data(), hitsA, nr * sizeof(PhxRayHit)) == PHX_OK, "copy");
+	CHECK(phx_copy_to_host(ctx, hb.data(), hitsB, nr * sizeof(PhxRayHit)) == PHX_OK, "copy");
+	CHECK(phx_cast_closest(ctx, renderer.scene(), hostRays.data(), nr, hc.data(), 0) == PHX_OK,
+	      "host cast");
+	CHECK(std::memcmp(ha.data(), hc.data(), nr * sizeof(PhxRayHit)) == 0, "launchRayCastKernel");
+	CHECK(std::memcmp(hb.data(), hc.data(), nr * sizeof(PhxRayHit)) == 0,
+	      "launchRayCastNoPersistenceKernel");
+	uint32_t hitsFound = 0;
+	for (const PhxRayHit& hh : hc) hitsFound += hh.triangleIndex != 0xFFFFFFFFu;
+	CHECK(hitsFound > nr / 2, "only %u of %u primary rays hit", hitsFound, nr);
+	// shadow rays: towards a point above the scene from every hit
+	std::vector<PhxRayIn> shadow(nr);
+	for (uint32_t i = 0; i < nr; i++) {
+		PhxRayIn& s = shadow[i];
+		const PhxRayIn& r = hostRays[i];
+		const float t = hc[i].triangleIndex == 0xFFFFFFFFu ? 1.0f : hc[i].t;
+		float to[3];
+		for (int k = 0; k < 3; k++) s.origin[k] = r.origin[k] + r.dir[k] * t;
+		to[0] = 0.3f - s.origin[0];
+		to[1] = 2.0f - s.origin[1];
+		to[2] = 0.4f - s.origin[2];
+		const float l = std::sqrt(to[0] * to[0] + to[1] * to[1] + to[2] * to[2]);
+		for (int k = 0; k < 3; k++) s.dir[k] = to[k] / l;
+		s.minDist = 1e-3f;
+		s.maxDist = l - 1e-3f;
+	}
+	CHECK(phx_copy_to_device(ctx, rays, shadow.data(), nr * sizeof(PhxRayIn)) == PHX_OK, "copy");
+	CHECK(launchShadowRayCastKernel(ctx, in, static_cast<bool*>(light)) == PHX_OK, "shadow");
+	std::vector<uint8_t> la(nr), lb(nr);
+	CHECK(phx_copy_to_host(ctx, la.data(), light, nr) == PHX_OK, "copy");
+	CHECK(phx_cast_anyhit(ctx, renderer.scene(), shadow.data(), nr, lb.data(), 0) == PHX_OK,
+	      "host anyhit");
+	CHECK(la == lb, "launchShadowRayCastKernel");
+	for (void* p : {rays, hitsA, hitsB, light}) phx_device_free(ctx, p);
+
+	if (failures)
+		std::printf("hiptracer_check: %d check(s) failed\n", failures);
+	else
+		std::printf("hiptracer_check: OK\n");
+	return failures;
+}

@@ -162,3 +162,16 @@ def test_c3_full_frame_bit_identical(ctx):
                                      w, h, jx, jy, SEED, 7)
     bits_equal(gbuf.view(np.float32), ref_gbuf.view(np.float32), "C3 hit records")
     bits_equal(out, ref_color, "C3 frame")
+
+
+def test_cpp_renderer_interface():
+    """The C++ HipTracerRenderer (engine-side interface over the C ABI) renders the same frames as
+    phx_render_frame and its launch functions equal phx_cast_closest / phx_cast_anyhit."""
+    import ctypes
+    from pathlib import Path
+    lib = Path(phx.__file__).resolve().parent / "libhiptracer_check.so"
+    assert lib.exists(), "build the package first (python -m phantasyenginetestbed_amd.build)"
+    say("[gpu-frame] C++ renderer interface check")
+    check = ctypes.CDLL(str(lib))
+    check.hiptracer_check_run.restype = ctypes.c_int
+    assert check.hiptracer_check_run() == 0
