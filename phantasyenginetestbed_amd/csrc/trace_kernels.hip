@@ -23,6 +23,19 @@ constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 
+// Address-space views: records are loaded through pointers the compiler knows to be global (or
+// LDS), so the loads are global_load / ds_read rather than flat loads, whose latency is higher
+// and which wait on both the vector-memory and LDS counters.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const float4 __attribute__((address_space(1))) GlobalF4;
+typedef const float4 __attribute__((address_space(3))) LdsF4;
+#else  // the host pass only needs the declarations to parse
+typedef const float4 GlobalF4;
+typedef const float4 LdsF4;
+#endif
+__device__ __forceinline__ GlobalF4* as_global(const float4* p) { return (GlobalF4*)(p); }
+__device__ __forceinline__ LdsF4* as_lds(const float4* p) { return (LdsF4*)(p); }
+
 struct NodeWords {
 	float4 w0, w1, w2;
 	int4 w3;
@@ -350,8 +363,9 @@ __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t block
 // Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Each
 // iteration every busy lane fetches one record -- its next node (64 B, from the LDS treelet or
 // global memory), its next triangle (48 B) or, if it was just refilled, its ray (32 B) -- through
-// the same 128-bit load instructions, so the fetches of a wave go out together.
-template <bool kAnyHit, bool kCount>
+// the same 128-bit load instructions, so the fetches of a wave go out together. Without a treelet
+// (kTreelet false) every record comes from global memory and no LDS path is compiled.
+template <bool kAnyHit, bool kCount, bool kTreelet>
 __device__ __forceinline__ void persistent_trace(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ inLight,
@@ -361,8 +375,10 @@ __device__ __forceinline__ void persistent_trace(
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
 	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsDepth) * 64u / 4u;
-	for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
-	__syncthreads();
+	if (kTreelet) {
+		for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
+		__syncthreads();
+	}
 
 	const uint32_t lane = __lane_id();
 	LaneStack st;
@@ -400,23 +416,23 @@ __device__ __forceinline__ void persistent_trace(
 		// back; a triangle's word 3 is the next triangle's first word (the triangle array carries
 		// 64 B of padding). Treelet nodes are read from LDS instead.
 		const bool doTri = !fresh && s.triCur < s.triEnd;
-		const bool inLds = !fresh && !doTri && uint32_t(s.next) < topCount;
+		const bool inLds = kTreelet && !fresh && !doTri && uint32_t(s.next) < topCount;
 		const float4* src = fresh  ? rays + 2 * size_t(rayIdx)
 		                    : doTri ? tris + 3u * s.triCur
 		                            : nodes + 4u * uint32_t(s.next);
 		const float4* src23 = fresh ? nodes : src;
 		float4 a0, a1, a2, a3;
 		if (inLds) {
-			const float4* t = treelet + 4u * uint32_t(s.next);
+			const LdsF4* t = as_lds(treelet) + 4u * uint32_t(s.next);
 			a0 = t[0];
 			a1 = t[1];
 			a2 = t[2];
 			a3 = t[3];
 		} else {
-			a0 = src[0];
-			a1 = src[1];
-			a2 = src23[2];
-			a3 = src23[3];
+			a0 = as_global(src)[0];
+			a1 = as_global(src)[1];
+			a2 = as_global(src23)[2];
+			a3 = as_global(src23)[3];
 		}
 
 		bool done = false;
@@ -478,25 +494,26 @@ __device__ __forceinline__ void persistent_trace(
 	retire_wave(work);
 }
 
-template <bool kCount>
+template <bool kCount, bool kTreelet>
 __global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
 	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
 	uint32_t finiteBounds, uint32_t topCount, unsigned long long* __restrict__ counters)
 {
-	persistent_trace<false, kCount>(nodes, tris, rays, hits, nullptr, n, work, spill, refillMin,
-	                                finiteBounds != 0, topCount, counters);
+	persistent_trace<false, kCount, kTreelet>(nodes, tris, rays, hits, nullptr, n, work, spill,
+	                                          refillMin, finiteBounds != 0, topCount, counters);
 }
 
+template <bool kTreelet>
 __global__ __launch_bounds__(kMaxBlock) void k_anyhit_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint8_t* __restrict__ inLight, uint32_t n,
 	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
 	uint32_t finiteBounds, uint32_t topCount)
 {
-	persistent_trace<true, false>(nodes, tris, rays, nullptr, inLight, n, work, spill, refillMin,
-	                              finiteBounds != 0, topCount, nullptr);
+	persistent_trace<true, false, kTreelet>(nodes, tris, rays, nullptr, inLight, n, work, spill,
+	                                        refillMin, finiteBounds != 0, topCount, nullptr);
 }
 
 // ---- fused multi-batch launch ---------------------------------------------------------------------
@@ -528,6 +545,7 @@ __device__ __forceinline__ uint32_t find_batch(const uint32_t* offsets, uint32_t
 	return lo;
 }
 
+template <bool kTreelet>
 __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	const BatchDesc* __restrict__ descs, const uint32_t* __restrict__ offsetsGlobal,
 	uint32_t numBatches, uint32_t total, uint32_t* __restrict__ work,
@@ -600,23 +618,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 		if (rayIdx == kInvalid) continue;
 
 		const bool doTri = !fresh && s.triCur < s.triEnd;
-		const bool inLds = !fresh && !doTri && useTreelet && uint32_t(s.next) < topCount;
+		const bool inLds = kTreelet && !fresh && !doTri && useTreelet && uint32_t(s.next) < topCount;
 		if (!fresh) {
 			src = doTri ? tris + 3u * s.triCur : nodes + 4u * uint32_t(s.next);
 			src23 = src;
 		}
 		float4 a0, a1, a2, a3;
 		if (inLds) {
-			const float4* t = treelet + 4u * uint32_t(s.next);
+			const LdsF4* t = as_lds(treelet) + 4u * uint32_t(s.next);
 			a0 = t[0];
 			a1 = t[1];
 			a2 = t[2];
 			a3 = t[3];
 		} else {
-			a0 = src[0];
-			a1 = src[1];
-			a2 = src23[2];
-			a3 = src23[3];
+			a0 = as_global(src)[0];
+			a1 = as_global(src)[1];
+			a2 = as_global(src23)[2];
+			a3 = as_global(src23)[3];
 		}
 
 		bool done = false;
@@ -726,9 +744,10 @@ uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes)
 hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t* blocks)
 {
 	int a = 0;
+	const void* fn = topNodes ? reinterpret_cast<const void*>(k_closest_persistent<false, true>)
+	                          : reinterpret_cast<const void*>(k_closest_persistent<false, false>);
 	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		&a, reinterpret_cast<const void*>(k_closest_persistent<false>), int(blockThreads),
-		persistent_lds_bytes(blockThreads, topNodes));
+		&a, fn, int(blockThreads), persistent_lds_bytes(blockThreads, topNodes));
 	*blocks = uint32_t(a > 0 ? a : 0);
 	return e;
 }
@@ -752,14 +771,10 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	}
 	uint32_t grid, block, top, lds;
 	persistentShape(L, s, n, grid, block, top, lds);
-	if (counters)
-		hipLaunchKernelGGL(k_closest_persistent<true>, dim3(grid), dim3(block), lds, stream, nodes,
-		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, s.finiteBounds, top,
-		                   counters);
-	else
-		hipLaunchKernelGGL(k_closest_persistent<false>, dim3(grid), dim3(block), lds, stream, nodes,
-		                   tris, r4, h4, n, L.work, L.spill, L.refillMin, s.finiteBounds, top,
-		                   counters);
+	auto kernel = counters ? (top ? k_closest_persistent<true, true> : k_closest_persistent<true, false>)
+	                       : (top ? k_closest_persistent<false, true> : k_closest_persistent<false, false>);
+	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, nodes, tris, r4, h4, n,
+	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, counters);
 	return hipGetLastError();
 }
 
@@ -777,7 +792,8 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	}
 	uint32_t grid, block, top, lds;
 	persistentShape(L, s, n, grid, block, top, lds);
-	hipLaunchKernelGGL(k_anyhit_persistent, dim3(grid), dim3(block), lds, stream, nodes, tris, r4,
+	hipLaunchKernelGGL(top ? k_anyhit_persistent<true> : k_anyhit_persistent<false>, dim3(grid),
+	                   dim3(block), lds, stream, nodes, tris, r4,
 	                   inLight, n, L.work, L.spill, L.refillMin, s.finiteBounds, top);
 	return hipGetLastError();
 }
@@ -799,7 +815,7 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
 	uint32_t grid = L.gridBlocks < gridFor(total, block) ? L.gridBlocks : gridFor(total, block);
 	if (grid > L.spillLanes / block) grid = L.spillLanes / block;
 	if (grid == 0) grid = 1;
-	hipLaunchKernelGGL(k_trace_batches, dim3(grid), dim3(block),
+	hipLaunchKernelGGL(top ? k_trace_batches<true> : k_trace_batches<false>, dim3(grid), dim3(block),
 	                   batchLdsBytes(block, top, numBatches, clock != nullptr), stream, descs,
 	                   offsets, numBatches,
 	                   total, L.work, L.spill, L.refillMin, treeletNodes, top, clock);
