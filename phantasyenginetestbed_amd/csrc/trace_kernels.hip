@@ -311,6 +311,56 @@ __device__ __forceinline__ void decide(const dev::Ray& r, bool fastSlab, const N
 	visitR = (outR >= inR) && outR > r.tMin && inR < tCurrMax;
 }
 
+// A triangle step tests up to kTriBatch consecutive triangles of the current leaf range: the first
+// triangle's words come through the step's unified loads, the others are loaded right after them
+// so all are in flight together, and the tests then run in castRay's order (BVHTraversal.hpp:
+// 78-93), so every acceptance is the one castRay makes. Leaves on the paths of costly incoherent
+// rays hold about five triangles, so this shortens their chains of dependent steps.
+constexpr uint32_t kTriBatch = 2;
+
+struct TriExtra {
+	float4 w[3 * kTriBatch - 4];  // words 4 .. 3*kTriBatch-1 of the range (word 3 is the 4th load)
+};
+
+__device__ __forceinline__ uint32_t load_tri_extra(const float4* src, uint32_t avail, TriExtra& x)
+{
+	const uint32_t cnt = avail < kTriBatch ? avail : kTriBatch;
+#pragma unroll
+	for (uint32_t w = 4; w < 3 * kTriBatch; w++)
+		if (w < 3 * cnt) x.w[w - 4] = as_global(src)[w];
+	return cnt;
+}
+
+// Tests triangles first .. first+cnt-1 (words a0..a3 then x); any hit stops at the first hit.
+template <bool kAnyHit, bool kCount>
+__device__ __forceinline__ void test_tris(const dev::Ray& r, float4 a0, float4 a1, float4 a2,
+                                          float4 a3, const TriExtra& x, uint32_t cnt,
+                                          uint32_t first, Closest& c, Counts& k, bool& done)
+{
+#pragma unroll
+	for (uint32_t j = 0; j < kTriBatch; j++) {
+		if (j >= cnt || done) break;
+		const float4 p0 = j == 0 ? a0 : j == 1 ? a3 : x.w[3 * j - 4];
+		const float4 e1 = j == 0 ? a1 : x.w[3 * j - 3];
+		const float4 e2 = j == 0 ? a2 : x.w[3 * j - 2];
+		float t, u, v;
+		const bool hit = dev::triangle_pre(r, p0, e1, e2, t, u, v);
+		if (kAnyHit) {
+			done = hit && r.tMin <= t && t < r.tMax;
+			if (done) c.tri = 0u;
+		} else {
+			if (hit && t < c.t && r.tMin <= t && t <= r.tMax) {
+				c.tri = first + j;
+				c.t = t;
+				c.u = u;
+				c.v = v;
+				if (kCount) k.accepted++;
+			}
+			if (kCount) k.triTests++;
+		}
+	}
+}
+
 // Per-lane traversal state of the persistent kernels. Every iteration a lane does one unit of
 // work in castRay's per-ray order: expand a node (decide both children, queue the triangles of
 // visited leaf children -- left, then right -- and pick the next node), or test one queued
@@ -434,6 +484,9 @@ __device__ __forceinline__ void persistent_trace(
 			a2 = as_global(src23)[2];
 			a3 = as_global(src23)[3];
 		}
+		TriExtra x;
+		uint32_t cnt = 0;
+		if (doTri) cnt = load_tri_extra(src, s.triEnd - s.triCur, x);
 
 		bool done = false;
 		if (fresh) {
@@ -443,22 +496,8 @@ __device__ __forceinline__ void persistent_trace(
 			st.sp = 0;
 			s = {0u, 0u, 0u, 0u, 0};
 		} else if (doTri) {
-			float t, u, v;
-			const bool hit = dev::triangle_pre(r, a0, a1, a2, t, u, v);
-			if (kAnyHit) {
-				done = hit && r.tMin <= t && t < r.tMax;
-				if (done) c.tri = 0u;
-			} else {
-				if (hit && t < c.t && r.tMin <= t && t <= r.tMax) {
-					c.tri = s.triCur;
-					c.t = t;
-					c.u = u;
-					c.v = v;
-					if (kCount) k.accepted++;
-				}
-				if (kCount) k.triTests++;
-			}
-			s.triCur++;
+			test_tris<kAnyHit, kCount>(r, a0, a1, a2, a3, x, cnt, s.triCur, c, k, done);
+			s.triCur += cnt;
 			if (s.triCur == s.triEnd) {
 				s.triCur = s.pendCur;
 				s.triEnd = s.pendEnd;
@@ -636,6 +675,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 			a2 = as_global(src23)[2];
 			a3 = as_global(src23)[3];
 		}
+		TriExtra x;
+		uint32_t cnt = 0;
+		if (doTri) cnt = load_tri_extra(src, s.triEnd - s.triCur, x);
 
 		bool done = false;
 		if (fresh) {
@@ -645,18 +687,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 			st.sp = 0;
 			s = {0u, 0u, 0u, 0u, 0};
 		} else if (doTri) {
-			float t, u, v;
-			const bool hit = dev::triangle_pre(r, a0, a1, a2, t, u, v);
-			if (anyHit) {
-				done = hit && r.tMin <= t && t < r.tMax;
-				if (done) c.tri = 0u;
-			} else if (hit && t < c.t && r.tMin <= t && t <= r.tMax) {
-				c.tri = s.triCur;
-				c.t = t;
-				c.u = u;
-				c.v = v;
-			}
-			s.triCur++;
+			Counts none;
+			if (anyHit)
+				test_tris<true, false>(r, a0, a1, a2, a3, x, cnt, s.triCur, c, none, done);
+			else
+				test_tris<false, false>(r, a0, a1, a2, a3, x, cnt, s.triCur, c, none, done);
+			s.triCur += cnt;
 			if (s.triCur == s.triEnd) {
 				s.triCur = s.pendCur;
 				s.triEnd = s.pendEnd;
