@@ -214,6 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restri
 //     incoherent rays happen. They are copied once per block and read from LDS afterwards.
 
 constexpr int kLdsDepth = 16;
+constexpr int kLdsRows = kLdsDepth + 1;  // + one row that lanes without a push write to
 constexpr int kPersistentStack = 72;
 constexpr uint32_t kWindow = 64;
 constexpr int kMaxBlock = 1024;
@@ -285,6 +286,16 @@ struct LaneStack {
 		else if (sp < kPersistentStack)
 			spill[size_t(sp - kLdsDepth) * stride] = v;
 		sp++;
+	}
+	// Pushes v if doPush. Lanes that do not push (or whose entry spills) store to the extra row, so
+	// the LDS store needs no branch; only a spilling entry takes one.
+	__device__ __forceinline__ void push_if(bool doPush, uint32_t v)
+	{
+		const bool toLds = doPush && sp < kLdsDepth;
+		lds[(toLds ? sp : kLdsDepth) * 64 + lane] = v;
+		if (doPush && sp >= kLdsDepth && sp < kPersistentStack)
+			spill[size_t(sp - kLdsDepth) * stride] = v;
+		sp += doPush ? 1 : 0;
 	}
 	__device__ __forceinline__ uint32_t pop()
 	{
@@ -375,39 +386,31 @@ struct LaneTrace {
 __device__ __forceinline__ void expand(const NodeWords& nd, bool visitL, bool visitR,
                                        LaneTrace& s, LaneStack& st)
 {
+	// Selects instead of branches: a node step starts with an empty triangle range, visited leaf
+	// children become the range (left first, the right one queued behind it), the right inner
+	// child is castRay's next pop, a left inner child with it is pushed, and with no inner child
+	// the stack is popped.
 	const bool innerL = visitL && nd.w3.z == 0;
 	const bool innerR = visitR && nd.w3.w == 0;
 	const bool leafL = visitL && !innerL;
 	const bool leafR = visitR && !innerR;
 	const uint32_t lFirst = child_index(nd.w3.x), rFirst = child_index(nd.w3.y);
-	if (leafL) {
-		s.triCur = lFirst;
-		s.triEnd = lFirst + uint32_t(nd.w3.z);
-		if (leafR) {
-			s.pendCur = rFirst;
-			s.pendEnd = rFirst + uint32_t(nd.w3.w);
-		}
-	} else if (leafR) {
-		s.triCur = rFirst;
-		s.triEnd = rFirst + uint32_t(nd.w3.w);
-	}
-	if (innerL && innerR) {
-		st.push(lFirst);
-		s.next = int32_t(rFirst);
-	} else if (innerR) {
-		s.next = int32_t(rFirst);
-	} else if (innerL) {
-		s.next = int32_t(lFirst);
-	} else {
-		s.next = st.sp > 0 ? int32_t(st.pop()) : -1;
-	}
+	const uint32_t lEnd = lFirst + uint32_t(nd.w3.z), rEnd = rFirst + uint32_t(nd.w3.w);
+	s.triCur = leafL ? lFirst : leafR ? rFirst : s.triCur;
+	s.triEnd = leafL ? lEnd : leafR ? rEnd : s.triEnd;
+	const bool both = leafL && leafR;
+	s.pendCur = both ? rFirst : s.pendCur;
+	s.pendEnd = both ? rEnd : s.pendEnd;
+	st.push_if(innerL && innerR, lFirst);
+	s.next = innerR ? int32_t(rFirst) : innerL ? int32_t(lFirst) : -1;
+	if (!innerL && !innerR && st.sp > 0) s.next = int32_t(st.pop());
 }
 
 // LDS bytes of a persistent launch: the stacks of every wave plus the treelet.
 __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t blockThreads,
                                                                   uint32_t topCount)
 {
-	return (blockThreads / 64u) * uint32_t(kLdsDepth) * 64u * 4u + topCount * 64u;
+	return (blockThreads / 64u) * uint32_t(kLdsRows) * 64u * 4u + topCount * 64u;
 }
 
 // Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Each
@@ -424,7 +427,7 @@ __device__ __forceinline__ void persistent_trace(
 {
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
-	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsDepth) * 64u / 4u;
+	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
 	if (kTreelet) {
 		for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
 		__syncthreads();
@@ -432,7 +435,7 @@ __device__ __forceinline__ void persistent_trace(
 
 	const uint32_t lane = __lane_id();
 	LaneStack st;
-	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsDepth) * 64u;
+	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
 	st.stride = gridDim.x * blockDim.x;
 	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
 	st.lane = lane;
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 {
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
-	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsDepth) * 64u / 4u;
+	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
 	unsigned long long* blockFinish = reinterpret_cast<unsigned long long*>(treelet + 4u * topCount);
 	uint32_t* blockDone = reinterpret_cast<uint32_t*>(blockFinish + (clock ? numBatches : 0u));
 	uint32_t* offsets = blockDone + (clock ? numBatches : 0u);
@@ -608,7 +611,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	__syncthreads();
 
 	LaneStack st;
-	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsDepth) * 64u;
+	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
 	st.stride = gridDim.x * blockDim.x;
 	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
 	st.lane = __lane_id();
