@@ -335,11 +335,12 @@ struct TriExtra {
 
 __device__ __forceinline__ uint32_t load_tri_extra(const float4* src, uint32_t avail, TriExtra& x)
 {
-	const uint32_t cnt = avail < kTriBatch ? avail : kTriBatch;
-#pragma unroll
-	for (uint32_t w = 4; w < 3 * kTriBatch; w++)
-		if (w < 3 * cnt) x.w[w - 4] = as_global(src)[w];
-	return cnt;
+	// The words past a leaf's last triangle lie in the next triangle or in the 64 B of padding
+	// after the array, so they are loaded whether or not the leaf has a second triangle.
+	static_assert(kTriBatch == 2, "the padding covers the words of one more triangle");
+	x.w[0] = as_global(src)[4];
+	x.w[1] = as_global(src)[5];
+	return avail < kTriBatch ? avail : kTriBatch;
 }
 
 // Tests triangles first .. first+cnt-1 (words a0..a3 then x); any hit stops at the first hit.
@@ -470,9 +471,11 @@ __device__ __forceinline__ void persistent_trace(
 		// 64 B of padding). Treelet nodes are read from LDS instead.
 		const bool doTri = !fresh && s.triCur < s.triEnd;
 		const bool inLds = kTreelet && !fresh && !doTri && uint32_t(s.next) < topCount;
-		const float4* src = fresh  ? rays + 2 * size_t(rayIdx)
-		                    : doTri ? tris + 3u * s.triCur
-		                            : nodes + 4u * uint32_t(s.next);
+		// Record address by integer selects (branches here cost a lone lane more than the math).
+		const uint64_t base = fresh ? uint64_t(rays) : doTri ? uint64_t(tris) : uint64_t(nodes);
+		const uint32_t words = fresh ? 2u : doTri ? 3u : 4u;
+		const uint32_t index = fresh ? rayIdx : doTri ? s.triCur : uint32_t(s.next);
+		const float4* src = reinterpret_cast<const float4*>(base + uint64_t(index * words) * 16u);
 		const float4* src23 = fresh ? nodes : src;
 		float4 a0, a1, a2, a3;
 		if (inLds) {
