@@ -254,6 +254,40 @@ void phx_scene_release(PhxScene* scene);
 size_t phx_scene_device_bytes(const PhxScene* scene);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Scene files: the reference's .bvhcache (three raw array dumps, testbed_application/         */
+/* Main.cpp:105-154) as one versioned file with a checksum, plus an importer of the raw dumps  */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct PhxSceneArrays {
+	const PhxBVHNode* nodes;
+	uint32_t numNodes;
+	uint32_t numTriangles;
+	const PhxTriangleVertices* triangles;
+	const PhxTriangleData* datas; /* may be NULL */
+	const PhxMaterial* materials;
+	uint32_t numMaterials;
+	uint32_t numLights;
+	const PhxSphereLight* lights;
+} PhxSceneArrays;
+
+typedef struct PhxSceneFile PhxSceneFile;
+
+/* Writes the arrays (the BVH is validated first) as a version-1 scene file: a 64-byte header
+ * (magic "PHXSCENE", version, counts, flags, FNV-1a 64 checksum of the payload) followed by the
+ * nodes, triangles, datas, materials and lights, each section 64-byte aligned. */
+phx_status phx_scene_file_write(const char* path, const PhxSceneArrays* arrays);
+/* Reads a scene file; rejects a wrong magic or version, truncation, trailing bytes, a checksum
+ * mismatch and a structurally invalid BVH with PHX_ERR_INVALID_SCENE / PHX_ERR_MALFORMED_BVH. */
+phx_status phx_scene_file_read(const char* path, PhxSceneFile** outFile);
+/* Imports the reference's raw cache files (BVHNode, TriangleVertices and, if datasPath is not
+ * NULL, TriangleData records) with the same validation (Main.cpp:120-134 checks sizes only). */
+phx_status phx_scene_file_read_bvhcache(const char* nodesPath, const char* vertsPath,
+                                        const char* datasPath, PhxSceneFile** outFile);
+/* Views of a read file's arrays (valid until phx_scene_file_release), ready for phx_scene_upload. */
+phx_status phx_scene_file_arrays(const PhxSceneFile* file, PhxSceneArrays* arrays);
+void phx_scene_file_release(PhxSceneFile* file);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Cast rays                                                                                  */
 /* ------------------------------------------------------------------------------------------ */
 

@@ -71,6 +71,12 @@ class Request(C.Structure):
                 ("pad", C.c_uint32)]
 
 
+class SceneArrays(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("numNodes", C.c_uint32), ("numTriangles", C.c_uint32),
+                ("triangles", C.c_void_p), ("datas", C.c_void_p), ("materials", C.c_void_p),
+                ("numMaterials", C.c_uint32), ("numLights", C.c_uint32), ("lights", C.c_void_p)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("requestsSubmitted", C.c_uint64), ("requestsDone", C.c_uint64),
                 ("raysDone", C.c_uint64), ("launches", C.c_uint64), ("rejected", C.c_uint64),
@@ -168,6 +174,12 @@ _SIGNATURES = {
     "phx_server_replay": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_double), _U32,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "phx_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),
+    "phx_scene_file_write": (C.c_int, [C.c_char_p, C.POINTER(SceneArrays)]),
+    "phx_scene_file_read": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "phx_scene_file_read_bvhcache": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p,
+                                               C.POINTER(_P)]),
+    "phx_scene_file_arrays": (C.c_int, [_P, C.POINTER(SceneArrays)]),
+    "phx_scene_file_release": (None, [_P]),
     "phx_host_free": (C.c_int, [_P]),
     "phx_scene_heightfield_boxes": (C.c_int, [_U32, _U32, C.c_uint64, _F32, _F32, _F32, _U32,
                                               _P, _P]),
@@ -232,6 +244,52 @@ def heightfield_boxes(grid: int, num_boxes: int, seed: int, amp: float = 0.1,
     check(lib().phx_scene_heightfield_boxes(grid, num_boxes, seed, amp, box_mu, box_sigma,
                                             num_materials, _ptr(tris), _ptr(datas)))
     return tris, datas
+
+
+def save_scene(path, nodes, tris, datas=None, materials=None, lights=None) -> None:
+    """Writes a version-1 scene file (phx_scene_file_write)."""
+    nodes = np.ascontiguousarray(nodes, NODE)
+    tris = np.ascontiguousarray(tris, TRI)
+    datas = None if datas is None else np.ascontiguousarray(datas, TRIDATA)
+    mats = np.zeros(0, MATERIAL) if materials is None else np.ascontiguousarray(materials, MATERIAL)
+    lts = np.zeros(0, LIGHT) if lights is None else np.ascontiguousarray(lights, LIGHT)
+    a = SceneArrays(_ptr(nodes), len(nodes), len(tris), _ptr(tris), _ptr(datas),
+                    _ptr(mats) if len(mats) else None, len(mats), len(lts),
+                    _ptr(lts) if len(lts) else None)
+    check(lib().phx_scene_file_write(str(path).encode(), C.byref(a)))
+
+
+def _file_arrays(h) -> dict:
+    a = SceneArrays()
+    check(lib().phx_scene_file_arrays(h, C.byref(a)))
+
+    def view(ptr, n, dt):
+        if not ptr or not n:
+            return np.zeros(0, dt)
+        raw = (C.c_uint8 * (n * dt.itemsize)).from_address(ptr)
+        return np.frombuffer(raw, np.uint8).view(dt).copy()
+    out = {"nodes": view(a.nodes, a.numNodes, NODE), "tris": view(a.triangles, a.numTriangles, TRI),
+           "datas": view(a.datas, a.numTriangles, TRIDATA) if a.datas else None,
+           "materials": view(a.materials, a.numMaterials, MATERIAL),
+           "lights": view(a.lights, a.numLights, LIGHT)}
+    lib().phx_scene_file_release(h)
+    return out
+
+
+def load_scene(path) -> dict:
+    """Reads a scene file; returns dict of numpy arrays (nodes, tris, datas, materials, lights)."""
+    h = C.c_void_p()
+    check(lib().phx_scene_file_read(str(path).encode(), C.byref(h)))
+    return _file_arrays(h.value)
+
+
+def load_bvhcache(nodes_path, verts_path, datas_path=None) -> dict:
+    """Imports the reference's three raw .bvhcache files."""
+    h = C.c_void_p()
+    check(lib().phx_scene_file_read_bvhcache(str(nodes_path).encode(), str(verts_path).encode(),
+                                             None if datas_path is None
+                                             else str(datas_path).encode(), C.byref(h)))
+    return _file_arrays(h.value)
 
 
 def materials_random(count: int, seed: int) -> np.ndarray:

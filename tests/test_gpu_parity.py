@@ -283,3 +283,19 @@ def test_fused_plan_matches_per_batch_results(ctx, c1, c2):
         plan.release()
         for b in bufs:
             b.free()
+
+
+def test_scene_file_upload_matches_oracle(ctx, tmp_path):
+    """A scene written to and read back from a scene file traces exactly like the built one."""
+    tris, datas = phx.heightfield_boxes(40, 30, 0xC0FFEE02)
+    b = phx.BVH(tris, datas)
+    p = tmp_path / "hf40.phxscene"
+    phx.save_scene(p, b.nodes, b.tris, b.datas)
+    f = phx.load_scene(p)
+    s = ctx.upload_scene(f["nodes"], f["tris"], f["datas"])
+    cam = phx.camera_look_at(*C2_CAM, math.radians(60), 160, 100)
+    rays = ctx.gen_primary_rays(cam, 160, 100)
+    say("[gpu] scene file upload")
+    got = ctx.cast_closest(s, rays)
+    assert got.tobytes() == O.cast_closest(b.nodes, b.tris, rays).tobytes()
+    s.release()
