@@ -123,7 +123,7 @@ def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
                       f" on {threads} threads ({cpu_model}); hits identical to GPU: {same}"}
 
 
-def batched_form(ctx, scene, pools, ev, reps: int = 3) -> dict:
+def batched_form(ctx, scene, pools, ev, reps: int = 3, plan_launch=None) -> dict:
     from phantasyenginetestbed_amd import workloads as W
     bs = W.batch_set(pools)
     W.upload_batches(ctx, bs)
@@ -153,7 +153,13 @@ def batched_form(ctx, scene, pools, ev, reps: int = 3) -> dict:
     ev.destroy(evs)
 
     plan = W.plan_for(ctx, scene, bs, track=True)
-    tuned = plan.autotune()
+    if plan_launch:
+        grid, block, top, refill = plan_launch
+        plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill)
+        tuned = {"gridBlocksClosest": grid, "blockThreads": block, "topNodes": top,
+                 "refillMin": refill}
+    else:
+        tuned = plan.autotune()
     e2 = ev.create(2)
 
     def timed(graph: bool) -> float:
@@ -295,6 +301,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
                     help="fixed persistent launch shape instead of the autotuner's (profiling)")
+    ap.add_argument("--plan-launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
+                    help="fixed launch shape of the fused 256-batch plan (profiling)")
     ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane"], default=None,
                     help="force a launch model (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -396,11 +404,13 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic() if kernel == "persistent" else None,
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 768x256 "
-                                       "top 512 refill 32 (profiles/closest_persistent_bounce1M.md)",
+                                       "no treelet, refill 8 (profiles/closest_persistent_bounce1M.md)",
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_batched:
-        result["batched"] = batched_form(ctx, scene, pools, ev)
+        plan_launch = tuple(int(x) for x in args.plan_launch.split(",")) if args.plan_launch \
+            else None
+        result["batched"] = batched_form(ctx, scene, pools, ev, plan_launch=plan_launch)
         log(f"batched: {result['batched']}")
     if not args.no_frame:
         result["frame"] = frame_form(ctx, scene, ev, len(lights))
