@@ -127,6 +127,14 @@ struct SpatialSplit {
 	float pos = 0.0f;
 };
 
+// True for a default-constructed box: any growBox leaves at least one bound changed.
+inline bool untouched(const Box& b)
+{
+	for (int k = 0; k < 3; k++)
+		if (b.mn[k] != kF32Max || b.mx[k] != -kF32Max) return false;
+	return true;
+}
+
 struct SpatialBin {
 	Box bounds;
 	int enter = 0;
@@ -148,6 +156,10 @@ struct Shared {
 };
 
 constexpr int kParallelMinRefs = 8192;
+// Nodes at least this large search their splits with several threads (the three axes of the object
+// split in parallel, the spatial binning in chunks). The results are those of the sequential
+// search: sort keys are unique within a node, and bin bounds and counts reduce exactly.
+constexpr int kWideNodeRefs = 1 << 16;
 
 // Per-thread build state: the reference's m_refStack plus scratch arrays.
 class Worker {
@@ -172,12 +184,11 @@ private:
 
 	static float ntf(int n) { return static_cast<float>(n); }  // getTriangleCost(n), unit cost
 
-	// Sorts [start, end) of the stack by (min[dim] + max[dim], tri).
-	void sortRange(int start, int end, int dim)
+	// Sort keys of [start, end) of the stack along dim: (min[dim] + max[dim], tri), sorted.
+	void sortedKeys(int start, int end, int dim, std::vector<KeyPos>& keys) const
 	{
 		int n = end - start;
-		if (n < 2) return;
-		mKeys.resize(n);
+		keys.resize(n);
 		for (int i = 0; i < n; i++) {
 			const Ref& r = mStack[start + i];
 			float c = r.bounds.mn[dim] + r.bounds.mx[dim];
@@ -185,14 +196,78 @@ private:
 			uint32_t bits;
 			std::memcpy(&bits, &c, 4);
 			bits = (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);
-			mKeys[i].key = (uint64_t(bits) << 32) | uint32_t(r.tri);
-			mKeys[i].pos = uint32_t(start + i);
+			keys[i].key = (uint64_t(bits) << 32) | uint32_t(r.tri);
+			keys[i].pos = uint32_t(start + i);
 		}
-		std::sort(mKeys.begin(), mKeys.end(),
+		std::sort(keys.begin(), keys.end(),
 		          [](const KeyPos& a, const KeyPos& b) { return a.key < b.key; });
+	}
+
+	void applyOrder(int start, const std::vector<KeyPos>& keys)
+	{
+		const int n = int(keys.size());
 		mTmp.resize(n);
-		for (int i = 0; i < n; i++) mTmp[i] = mStack[mKeys[i].pos];
+		for (int i = 0; i < n; i++) mTmp[i] = mStack[keys[i].pos];
 		std::copy(mTmp.begin(), mTmp.end(), mStack.begin() + start);
+	}
+
+	// Sorts [start, end) of the stack by (min[dim] + max[dim], tri).
+	void sortRange(int start, int end, int dim)
+	{
+		if (end - start < 2) return;
+		sortedKeys(start, end, dim, mKeys);
+		applyOrder(start, mKeys);
+	}
+
+	// Best object split along one axis over refs in `keys` order: the first split index with the
+	// smallest SAH (SplitBVHBuilder.cpp:211-231).
+	void sweepAxis(int numRef, float nodeSAH, int dim, const std::vector<KeyPos>& keys,
+	               std::vector<Box>& rightBounds, ObjectSplit& best) const
+	{
+		rightBounds.resize(numRef);
+		Box right;
+		for (int i = numRef - 1; i > 0; i--) {
+			right.growBox(mStack[keys[i].pos].bounds);
+			rightBounds[i - 1] = right;
+		}
+		Box left;
+		for (int i = 1; i < numRef; i++) {
+			left.growBox(mStack[keys[i - 1].pos].bounds);
+			float sah = nodeSAH + left.area() * ntf(i);
+			sah = sah + rightBounds[i - 1].area() * ntf(numRef - i);
+			if (sah < best.sah) {
+				best.sah = sah;
+				best.sortDim = dim;
+				best.numLeft = i;
+				best.leftBounds = left;
+				best.rightBounds = rightBounds[i - 1];
+			}
+		}
+	}
+
+	// findObjectSplit for wide nodes: the three axes sorted and swept by three threads, the best
+	// chosen as the sequential loop would (lowest axis among equal SAHs), and the stack left sorted
+	// along the last axis, as the sequential search leaves it.
+	ObjectSplit findObjectSplitWide(const NodeSpec& spec, float nodeSAH)
+	{
+		const int start = int(mStack.size()) - spec.numRef;
+		const int end = int(mStack.size());
+		std::vector<KeyPos> keys[3];
+		std::vector<Box> rb[3];
+		ObjectSplit best[3];
+		auto axis = [&](int dim) {
+			sortedKeys(start, end, dim, keys[dim]);
+			sweepAxis(spec.numRef, nodeSAH, dim, keys[dim], rb[dim], best[dim]);
+		};
+		std::thread t1(axis, 1), t2(axis, 2);
+		axis(0);
+		t1.join();
+		t2.join();
+		ObjectSplit split;
+		for (int dim = 0; dim < 3; dim++)
+			if (best[dim].sah < split.sah) split = best[dim];
+		applyOrder(start, keys[2]);
+		return split;
 	}
 
 	TNode* createLeaf(const NodeSpec& spec)
@@ -209,6 +284,7 @@ private:
 
 	ObjectSplit findObjectSplit(const NodeSpec& spec, float nodeSAH)
 	{
+		if (spec.numRef >= kWideNodeRefs) return findObjectSplitWide(spec, nodeSAH);
 		ObjectSplit split;
 		int start = int(mStack.size()) - spec.numRef;
 		int end = int(mStack.size());
@@ -295,30 +371,65 @@ private:
 		for (int dim = 0; dim < 3; dim++)
 			for (int i = 0; i < kNumBins; i++) mBins[dim][i] = SpatialBin();
 
-		int size = int(mStack.size());
-		for (int refIdx = size - spec.numRef; refIdx < size; refIdx++) {
-			const Ref ref = mStack[refIdx];
-			int first[3], last[3];
-			for (int k = 0; k < 3; k++) {
-				int f = cvt_trunc_i32((ref.bounds.mn[k] - origin[k]) * invBinSize[k]);
-				f = f < 0 ? 0 : (f > kNumBins - 1 ? kNumBins - 1 : f);
-				int l = cvt_trunc_i32((ref.bounds.mx[k] - origin[k]) * invBinSize[k]);
-				l = l < f ? f : (l > kNumBins - 1 ? kNumBins - 1 : l);
-				first[k] = f;
-				last[k] = l;
-			}
-			for (int dim = 0; dim < 3; dim++) {
-				Ref curr = ref;
-				for (int i = first[dim]; i < last[dim]; i++) {
-					Ref l, r;
-					splitReference(l, r, curr, dim, origin[dim] + binSize[dim] * float(i + 1));
-					mBins[dim][i].bounds.growBox(l.bounds);
-					curr = r;
+		const int size = int(mStack.size());
+		auto binRange = [&](int from, int to, SpatialBin (*bins)[kNumBins]) {
+			for (int refIdx = from; refIdx < to; refIdx++) {
+				const Ref ref = mStack[refIdx];
+				int first[3], last[3];
+				for (int k = 0; k < 3; k++) {
+					int f = cvt_trunc_i32((ref.bounds.mn[k] - origin[k]) * invBinSize[k]);
+					f = f < 0 ? 0 : (f > kNumBins - 1 ? kNumBins - 1 : f);
+					int l = cvt_trunc_i32((ref.bounds.mx[k] - origin[k]) * invBinSize[k]);
+					l = l < f ? f : (l > kNumBins - 1 ? kNumBins - 1 : l);
+					first[k] = f;
+					last[k] = l;
 				}
-				mBins[dim][last[dim]].bounds.growBox(curr.bounds);
-				mBins[dim][first[dim]].enter++;
-				mBins[dim][last[dim]].exit++;
+				for (int dim = 0; dim < 3; dim++) {
+					Ref curr = ref;
+					for (int i = first[dim]; i < last[dim]; i++) {
+						Ref l, r;
+						splitReference(l, r, curr, dim, origin[dim] + binSize[dim] * float(i + 1));
+						bins[dim][i].bounds.growBox(l.bounds);
+						curr = r;
+					}
+					bins[dim][last[dim]].bounds.growBox(curr.bounds);
+					bins[dim][first[dim]].enter++;
+					bins[dim][last[dim]].exit++;
+				}
 			}
+		};
+		const int from = size - spec.numRef;
+		const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+		const int chunks = spec.numRef >= kWideNodeRefs ? int(std::min(hw, 16u)) : 1;
+		if (chunks <= 1) {
+			binRange(from, size, mBins);
+		} else {
+			// Bins per chunk, merged after: bounds grow by min/max and counts add, both exact and
+			// independent of order. A chunk bin nothing was added to is skipped: growing a box by
+			// an empty box makes it unbounded (the reference's AABB::grow semantics), which the
+			// sequential binning never does for a bin without references.
+			std::vector<SpatialBin> part(size_t(chunks) * 3 * kNumBins);
+			std::vector<std::thread> threads;
+			const int per = (spec.numRef + chunks - 1) / chunks;
+			for (int c = 0; c < chunks; c++) {
+				const int a = from + c * per, b = std::min(size, a + per);
+				auto* bins = reinterpret_cast<SpatialBin(*)[kNumBins]>(&part[size_t(c) * 3 * kNumBins]);
+				if (a >= b) continue;
+				if (c == 0)
+					continue;  // chunk 0 runs on this thread below
+				threads.emplace_back(binRange, a, b, bins);
+			}
+			binRange(from, std::min(size, from + per),
+			         reinterpret_cast<SpatialBin(*)[kNumBins]>(&part[0]));
+			for (std::thread& t : threads) t.join();
+			for (int c = 0; c < chunks; c++)
+				for (int dim = 0; dim < 3; dim++)
+					for (int i = 0; i < kNumBins; i++) {
+						const SpatialBin& b = part[(size_t(c) * 3 + dim) * kNumBins + i];
+						if (!untouched(b.bounds)) mBins[dim][i].bounds.growBox(b.bounds);
+						mBins[dim][i].enter += b.enter;
+						mBins[dim][i].exit += b.exit;
+					}
 		}
 
 		SpatialSplit split;
