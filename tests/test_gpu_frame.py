@@ -175,3 +175,26 @@ def test_cpp_renderer_interface():
     check = ctypes.CDLL(str(lib))
     check.hiptracer_check_run.restype = ctypes.c_int
     assert check.hiptracer_check_run() == 0
+
+
+def test_frame_without_lights_and_light_growth(ctx, small):
+    """A scene without lights renders black; a renderer that first sees a scene with fewer lights
+    grows its per-light buffers for the next one. Both frames equal the oracle's."""
+    bvh, mats, lts, _ = small
+    w, h = 40, 24
+    cam = W.camera(w, h)
+    dark = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas, mats, None)
+    two = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas, mats, lts[:2])
+    r = ctx.renderer(w, h)
+    try:
+        out = r.render(dark, cam, 0, SEED, 0)
+        assert (out[..., :3] == 0).all() and (out[..., 3] == 1).all()
+        for scene, lights in ((two, lts[:2]), (small[3], lts)):
+            got = r.render(scene, cam, 1, SEED, 0)
+            ref, _, _ = O.frame(bvh.nodes, bvh.tris, bvh.datas, mats, lights, cam.as_array(), w,
+                                h, 0.0, 0.0, SEED, 1)
+            bits_equal(got, ref, f"frame with {len(lights)} lights")
+    finally:
+        r.release()
+        dark.release()
+        two.release()
