@@ -28,12 +28,15 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // and which wait on both the vector-memory and LDS counters.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const float4 __attribute__((address_space(1))) GlobalF4;
+typedef const float __attribute__((address_space(1))) GlobalF1;
 typedef const float4 __attribute__((address_space(3))) LdsF4;
 #else  // the host pass only needs the declarations to parse
 typedef const float4 GlobalF4;
+typedef const float GlobalF1;
 typedef const float4 LdsF4;
 #endif
 __device__ __forceinline__ GlobalF4* as_global(const float4* p) { return (GlobalF4*)(p); }
+__device__ __forceinline__ GlobalF1* as_global(const float* p) { return (GlobalF1*)(p); }
 __device__ __forceinline__ LdsF4* as_lds(const float4* p) { return (LdsF4*)(p); }
 
 struct NodeWords {
@@ -304,6 +307,17 @@ struct LaneStack {
 		if (sp < kPersistentStack) return spill[size_t(sp - kLdsDepth) * stride];
 		return 0u;
 	}
+	// Pops if doPop (the value is meaningless otherwise): every lane reads one LDS word -- lanes
+	// that do not pop, or whose entry spilled, read the extra row -- and only a spilled entry
+	// takes a branch.
+	__device__ __forceinline__ uint32_t pop_if(bool doPop)
+	{
+		sp -= doPop ? 1 : 0;
+		uint32_t v = lds[(doPop && sp < kLdsDepth ? sp : kLdsDepth) * 64 + lane];
+		if (doPop && sp >= kLdsDepth)
+			v = sp < kPersistentStack ? spill[size_t(sp - kLdsDepth) * stride] : 0u;
+		return v;
+	}
 };
 
 // Visit decision of both children of a node (BVHTraversal.hpp:59-64; any-hit: tCurrMax = tMax).
@@ -414,11 +428,429 @@ __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t block
 	return (blockThreads / 64u) * uint32_t(kLdsRows) * 64u * 4u + topCount * 64u;
 }
 
-// Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Each
-// iteration every busy lane fetches one record -- its next node (64 B, from the LDS treelet or
-// global memory), its next triangle (48 B) or, if it was just refilled, its ray (32 B) -- through
-// the same 128-bit load instructions, so the fetches of a wave go out together. Without a treelet
-// (kTreelet false) every record comes from global memory and no LDS path is compiled.
+// ---- software-pipelined persistent loop -------------------------------------------------------
+//
+// Each iteration of a lane consumes the record it loaded in the previous iteration and issues the
+// load of the next one as soon as its address is known: right after a node's expansion, and for
+// a triangle step before the triangles are tested, so the test arithmetic runs while the next
+// record is in flight. The loop body is instantiated twice with the two record buffers swapped
+// (A -> B, then B -> A), so no record is ever copied between registers -- a copy would wait for
+// the load it copies. A new ray is loaded together with the root node and its first iteration
+// expands the root, so a ray costs one iteration less than a node of its own would.
+
+constexpr uint32_t kRecIdle = 0, kRecRay = 1, kRecTri = 2, kRecNode = 3;
+
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt = 0 only: expcnt 7 and lgkmcnt 15 are
+// the "no wait" values of those fields.
+constexpr int kWaitVmcntZero = (0x7 << 4) | (0xF << 8);
+
+// One record's six 16-B words: a node (words 0-3), a triangle pair (p0, e1, e2 of the first
+// triangle, then of the second), or the root node (words 0-3) with a new ray (words 4-5).
+struct Rec {
+	float4 w0, w1, w2, w3, w4, w5;
+};
+
+struct PipeArgs {
+	const float4* nodes;
+	const float4* tris;
+	const float4* rays;
+	uint4* hits;
+	uint8_t* inLight;
+	uint32_t n;
+	uint32_t* work;
+	uint32_t refillMin;
+	uint32_t topCount;
+	bool finiteBounds;
+	const float4* treelet;  // LDS copy of nodes [0, topCount)
+};
+
+// Loop-carried state of one lane.
+struct Tracer {
+	dev::Ray r;
+	Closest c;
+	LaneTrace s;
+	uint32_t rayIdx;   // ray being traced
+	uint32_t doneIdx;  // ray that ended in the previous step, its result still to be written
+	uint32_t kind;    // record in flight (kRec*)
+	bool fastSlab;
+};
+
+// Writes a ray's result: its closest hit, or (any hit) 0 if occluded -- c.tri == 0 marks a hit
+// there -- and 1 if not.
+template <bool kAnyHit>
+__device__ __forceinline__ void store_result(const PipeArgs& P, uint32_t idx, const Closest& c)
+{
+	if (kAnyHit)
+		P.inLight[idx] = c.tri == 0u ? 0 : 1;
+	else
+		P.hits[idx] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u),
+		                         __float_as_uint(c.v));
+}
+
+// Both triangles of a pair, branch-free, accepted in castRay's order (BVHTraversal.hpp:78-93).
+// The second is skipped when the pair holds one triangle (its words are the next triangle's or
+// the array's padding).
+template <bool kCount>
+__device__ __forceinline__ void test_pair_closest(const dev::Ray& r, const Rec& A, uint32_t cnt,
+                                                  uint32_t first, Closest& c, Counts& k)
+{
+	const dev::TriDet d0 = dev::tri_det(r, A.w1, A.w2);
+	const dev::TriDet d1 = dev::tri_det(r, A.w4, A.w5);
+	float f0 = dev::recip_newton(d0.a), f1 = dev::recip_newton(d1.a);
+	if (!(dev::recip_newton_ok(d0.a) && dev::recip_newton_ok(d1.a))) {
+		f0 = 1.0f / d0.a;
+		f1 = 1.0f / d1.a;
+	}
+	float t0, u0, v0, t1, u1, v1;
+	const bool h0 = dev::tri_finish(r, d0, f0, A.w0, A.w1, A.w2, t0, u0, v0);
+	const bool h1 = dev::tri_finish(r, d1, f1, A.w3, A.w4, A.w5, t1, u1, v1) && cnt > 1u;
+	const bool acc0 = h0 && t0 < c.t && r.tMin <= t0 && t0 <= r.tMax;
+	c.tri = acc0 ? first : c.tri;
+	c.t = acc0 ? t0 : c.t;
+	c.u = acc0 ? u0 : c.u;
+	c.v = acc0 ? v0 : c.v;
+	const bool acc1 = h1 && t1 < c.t && r.tMin <= t1 && t1 <= r.tMax;
+	c.tri = acc1 ? first + 1u : c.tri;
+	c.t = acc1 ? t1 : c.t;
+	c.u = acc1 ? u1 : c.u;
+	c.v = acc1 ? v1 : c.v;
+	if (kCount) {
+		k.accepted += uint32_t(acc0) + uint32_t(acc1);
+		k.triTests += cnt;
+	}
+}
+
+// Any-hit pair: true iff one of the triangles has tMin <= t < tMax (RayCastKernel.cu:759-765).
+__device__ __forceinline__ bool test_pair_any(const dev::Ray& r, const Rec& A, uint32_t cnt)
+{
+	const dev::TriDet d0 = dev::tri_det(r, A.w1, A.w2);
+	const dev::TriDet d1 = dev::tri_det(r, A.w4, A.w5);
+	float f0 = dev::recip_newton(d0.a), f1 = dev::recip_newton(d1.a);
+	if (!(dev::recip_newton_ok(d0.a) && dev::recip_newton_ok(d1.a))) {
+		f0 = 1.0f / d0.a;
+		f1 = 1.0f / d1.a;
+	}
+	float t0, u0, v0, t1, u1, v1;
+	const bool h0 = dev::tri_finish(r, d0, f0, A.w0, A.w1, A.w2, t0, u0, v0);
+	const bool h1 = dev::tri_finish(r, d1, f1, A.w3, A.w4, A.w5, t1, u1, v1) && cnt > 1u;
+	return (h0 && r.tMin <= t0 && t0 < r.tMax) || (h1 && r.tMin <= t1 && t1 < r.tMax);
+}
+
+// Node expansion for the pipelined loop: as expand(), with the pop done by selects around one
+// LDS read (lanes that do not pop read the spare row); only a spilled entry takes a branch.
+__device__ __forceinline__ void expand_pipe(const NodeWords& nd, bool visitL, bool visitR,
+                                            LaneTrace& s, LaneStack& st)
+{
+	const bool innerL = visitL && nd.w3.z == 0;
+	const bool innerR = visitR && nd.w3.w == 0;
+	const bool leafL = visitL && !innerL;
+	const bool leafR = visitR && !innerR;
+	const uint32_t lFirst = child_index(nd.w3.x), rFirst = child_index(nd.w3.y);
+	const uint32_t lEnd = lFirst + uint32_t(nd.w3.z), rEnd = rFirst + uint32_t(nd.w3.w);
+	s.triCur = leafL ? lFirst : leafR ? rFirst : s.triCur;
+	s.triEnd = leafL ? lEnd : leafR ? rEnd : s.triEnd;
+	const bool both = leafL && leafR;
+	s.pendCur = both ? rFirst : s.pendCur;
+	s.pendEnd = both ? rEnd : s.pendEnd;
+	st.push_if(innerL && innerR, lFirst);
+	const bool doPop = !innerL && !innerR && st.sp > 0;
+	const uint32_t popped = st.pop_if(doPop);
+	s.next = innerR ? int32_t(rFirst) : innerL ? int32_t(lFirst) : doPop ? int32_t(popped) : -1;
+}
+
+// What the wave does after a step: go on, stop (no ray left), or trace its last few rays solo.
+constexpr int kStepGo = 0, kStepDone = 1, kStepSolo = 2;
+
+// Rays left in a wave, once the ray queue is empty, at or below which the wave traces them one at
+// a time with all 64 lanes (solo_trace) instead of one lane each.
+constexpr uint32_t kSoloRays = 4;
+
+// One iteration: consume record A, issue the load of the lane's next record into B, test A's
+// triangles, retire finished rays and refill lanes. Every path a lone lane can take is kept short
+// -- once few rays are left, a wave's time per step is the number of instructions it issues --
+// so work that only some lanes need sits in the branch of those lanes. kExitCheck: whether this
+// step tests if the wave is done or should go solo (kStepGo otherwise); the loop runs two steps
+// per turn and checks after the second.
+template <bool kAnyHit, bool kCount, bool kTreelet, bool kExitCheck>
+__device__ __forceinline__ int pipe_step(const PipeArgs& P, Tracer& L, LaneStack& st,
+                                          WaveQueue& q, const Rec& A, Rec& B, Counts& k)
+{
+	// Wait for A -- and for the stores of the previous step -- here, before anything else. The
+	// compiler's own waits treat the vector-memory counter as unordered while a store is pending
+	// among loads and then wait for everything, B's loads included; with nothing outstanding at
+	// the top of the step, the only loads in flight during the triangle tests are B's.
+	__builtin_amdgcn_s_waitcnt(kWaitVmcntZero);
+	const uint32_t kind = L.kind;
+	// The result of a ray that ended in the previous step is still in L.c until a new ray starts.
+	const Closest ended = L.c;
+	uint32_t triFirst = 0u, triCnt = 0u;
+	bool anyHit = false;
+	if (kind == kRecTri) {
+		// Up to kTriBatch triangles of the current range; when the range is used up, the right
+		// leaf's range queued behind it (if any) becomes current.
+		triFirst = L.s.triCur;
+		const uint32_t left = L.s.triEnd - triFirst;
+		triCnt = left < kTriBatch ? left : kTriBatch;
+		const bool rangeDone = left <= kTriBatch;
+		L.s.triCur = rangeDone ? L.s.pendCur : triFirst + kTriBatch;
+		L.s.triEnd = rangeDone ? L.s.pendEnd : L.s.triEnd;
+		L.s.pendCur = rangeDone ? 0u : L.s.pendCur;
+		L.s.pendEnd = rangeDone ? 0u : L.s.pendEnd;
+		if (kAnyHit) anyHit = test_pair_any(L.r, A, triCnt);
+	} else if (kind != kRecIdle) {
+		if (kind == kRecRay) {
+			L.r = dev::make_ray(A.w4, A.w5);
+			L.fastSlab = P.finiteBounds && L.r.nanFree;
+			L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+			st.sp = 0;
+			L.s = {0u, 0u, 0u, 0u, -1};
+		}
+		NodeWords nd;
+		nd.w0 = A.w0;
+		nd.w1 = A.w1;
+		nd.w2 = A.w2;
+		nd.w3 = make_int4(__float_as_int(A.w3.x), __float_as_int(A.w3.y), __float_as_int(A.w3.z),
+		                  __float_as_int(A.w3.w));
+		const float tCurrMax = kAnyHit ? L.r.tMax : dev::min_first(L.r.tMax, L.c.t);
+		bool visitL, visitR;
+		decide(L.r, L.fastSlab, nd, tCurrMax, visitL, visitR);
+		if (kCount) {
+			k.nodes++;
+			k.boxHits += uint32_t(visitL) + uint32_t(visitR);
+		}
+		expand_pipe(nd, visitL, visitR, L.s, st);
+	}
+
+	// Next record: more of this ray, or (for lanes whose ray ends here, and idle lanes) a new ray.
+	const bool more = kind != kRecIdle && !anyHit && (L.s.triCur < L.s.triEnd || L.s.next >= 0);
+	uint32_t fresh = kInvalid;
+	if (!q.exhausted) {
+		const uint64_t freeLanes = __ballot(!more);
+		if (freeLanes != 0 && (uint32_t(__popcll(freeLanes)) >= P.refillMin || freeLanes == ~0ull))
+			fresh = refill(q, freeLanes, P.work, P.n, !more);
+	}
+	const bool tri = more && L.s.triCur < L.s.triEnd;
+	const bool node = more && !tri;
+	const bool ray = fresh != kInvalid;
+	const uint32_t nextKind = tri ? kRecTri : node ? kRecNode : ray ? kRecRay : kRecIdle;
+	// The ended ray's result is written just before the loads, after the waits of this step (the
+	// stack pop's, the refill's), so its latency overlaps the triangle tests, not a wait.
+	if (L.doneIdx != kInvalid) store_result<kAnyHit>(P, L.doneIdx, ended);
+	// Every lane issues the same six loads: a node (words 0-3), a triangle pair (words 0-5), or
+	// the root node and a new ray's 32 B; idle lanes and words 4-5 of a node read the root node
+	// (always cached). Offsets are in 16-B words: 4 per node, 3 per triangle.
+	const uint32_t idx = tri ? L.s.triCur : node ? uint32_t(L.s.next) : 0u;
+	const uint32_t word = tri ? idx + (idx << 1) : idx << 2;
+	const float4* rec = (tri ? P.tris : P.nodes) + word;
+	const float4* src45 = tri ? rec + 4 : ray ? P.rays + 2u * size_t(ray ? fresh : 0u) : P.nodes;
+	if (kTreelet && !tri && idx < P.topCount) {
+		const LdsF4* t = as_lds(P.treelet) + 4u * idx;
+		B.w0 = t[0];
+		B.w1 = t[1];
+		B.w2 = t[2];
+		B.w3 = t[3];
+	} else {
+		B.w0 = as_global(rec)[0];
+		B.w1 = as_global(rec)[1];
+		B.w2 = as_global(rec)[2];
+		B.w3 = as_global(rec)[3];
+	}
+	B.w4 = as_global(src45)[0];
+	B.w5 = as_global(src45)[1];
+
+	if (!kAnyHit && kind == kRecTri) test_pair_closest<kCount>(L.r, A, triCnt, triFirst, L.c, k);
+
+	if (kAnyHit && anyHit) L.c.tri = 0u;
+	L.doneIdx = kind != kRecIdle && !more ? L.rayIdx : kInvalid;
+	if (!more) L.rayIdx = fresh;
+	L.kind = nextKind;
+	if (!kExitCheck || !q.exhausted) return kStepGo;
+	const uint64_t busy = __ballot(nextKind != kRecIdle);
+	if (busy == 0) return kStepDone;
+	return !kCount && uint32_t(__popcll(busy)) <= kSoloRays ? kStepSolo : kStepGo;
+}
+
+// ---- solo mode: one ray per wave ---------------------------------------------------------------
+//
+// Once the ray queue is empty, a wave's last rays are the costliest ones, and a wave with one busy
+// lane still issues every instruction of a step. A wave with at most kSoloRays rays left therefore
+// traces them one after the other with all 64 lanes: a node's twelve slab planes are computed by
+// lanes 0-11 at once and combined across lanes (DPP), the triangles of visited leaves are tested
+// one per lane in a single pass, and the traversal state -- the ray, the closest hit, the stack
+// (entry e in lane e of two registers) -- is uniform, so every decision is a scalar branch.
+// Visiting order and arithmetic are castRay's: the same slab operations with std::min/max
+// semantics, the same triangle test, and a leaf pass accepts candidates in castRay's order
+// (ascending lane = ascending position in the left-then-right leaf sequence), each against the
+// closest hit so far.
+
+__device__ __forceinline__ uint32_t lane_u(uint32_t v, uint32_t l)
+{
+	return uint32_t(__builtin_amdgcn_readlane(int(v), int(l)));
+}
+__device__ __forceinline__ float lane_f(float v, uint32_t l)
+{
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(l)));
+}
+// The value of lane (this lane + k) of the same 16-lane row (DPP row_shl:k).
+template <int k>
+__device__ __forceinline__ float from_lane_plus(float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 | k, 0xF, 0xF, false));
+}
+
+struct SoloHit {
+	uint32_t tri;
+	float t, u, v;
+	bool any;  // any hit found (any-hit traversal)
+};
+
+// Tests the n1 triangles from a1 and then the n2 from a2 -- castRay's order -- one per lane, 64 per
+// pass. Closest hit: each candidate (hit, t < closest t on entry, tMin <= t <= tMax) is accepted
+// in order if t is still below the closest t, exactly as the sequential loop does. Any hit:
+// records a hit with tMin <= t < tMax.
+template <bool kAnyHit>
+__device__ __forceinline__ void solo_tris(const PipeArgs& P, const dev::Ray& r, uint32_t a1,
+                                          uint32_t n1, uint32_t a2, uint32_t n2, SoloHit& c)
+{
+	const uint32_t lane = __lane_id();
+	for (uint32_t base = 0; base < n1 + n2; base += 64u) {
+		const uint32_t j = base + lane;
+		const bool have = j < n1 + n2;
+		const uint32_t tri = j < n1 ? a1 + j : a2 + (j - n1);
+		float t = 0.0f, u = 0.0f, v = 0.0f;
+		bool hit = false;
+		if (have) {
+			const float4* p = P.tris + 3u * tri;
+			const float4 p0 = as_global(p)[0], e1 = as_global(p)[1], e2 = as_global(p)[2];
+			const dev::TriDet d = dev::tri_det(r, e1, e2);
+			float f = dev::recip_newton(d.a);
+			if (!dev::recip_newton_ok(d.a)) f = 1.0f / d.a;
+			hit = dev::tri_finish(r, d, f, p0, e1, e2, t, u, v);
+		}
+		if (kAnyHit) {
+			if (__ballot(have && hit && r.tMin <= t && t < r.tMax) != 0) {
+				c.any = true;
+				return;
+			}
+		} else {
+			uint64_t cand = __ballot(have && hit && t < c.t && r.tMin <= t && t <= r.tMax);
+			while (cand != 0) {
+				const uint32_t w = uint32_t(__builtin_ctzll(cand));
+				cand &= cand - 1u;
+				const float tw = lane_f(t, w);
+				if (tw < c.t) {
+					c.t = tw;
+					c.u = lane_f(u, w);
+					c.v = lane_f(v, w);
+					c.tri = lane_u(tri, w);
+				}
+			}
+		}
+	}
+}
+
+// Traces lane `lsel`'s ray to its end from the state the pipelined loop left (the record in flight
+// is loaded again here) and writes its result.
+template <bool kAnyHit>
+__device__ __forceinline__ void solo_trace(const PipeArgs& P, const Tracer& L, const LaneStack& st,
+                                           uint32_t lsel)
+{
+	const uint32_t lane = __lane_id();
+	const uint32_t kind = lane_u(L.kind, lsel);
+	const uint32_t rayIdx = lane_u(L.rayIdx, lsel);
+	dev::Ray r;
+	SoloHit c;
+	uint32_t triCur, triEnd, pendCur, pendEnd;
+	int32_t cur;
+	int sp;
+	if (kind == kRecRay) {
+		const float4* src = P.rays + 2u * size_t(rayIdx);
+		const float4 a = as_global(src)[0], b = as_global(src)[1];
+		r = dev::make_ray(make_float4(lane_f(a.x, 0), lane_f(a.y, 0), lane_f(a.z, 0), lane_f(a.w, 0)),
+		                  make_float4(lane_f(b.x, 0), lane_f(b.y, 0), lane_f(b.z, 0), lane_f(b.w, 0)));
+		c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX, false};
+		triCur = triEnd = pendCur = pendEnd = 0u;
+		cur = 0;
+		sp = 0;
+	} else {
+		r.ox = lane_f(L.r.ox, lsel), r.oy = lane_f(L.r.oy, lsel), r.oz = lane_f(L.r.oz, lsel);
+		r.dx = lane_f(L.r.dx, lsel), r.dy = lane_f(L.r.dy, lsel), r.dz = lane_f(L.r.dz, lsel);
+		r.ix = lane_f(L.r.ix, lsel), r.iy = lane_f(L.r.iy, lsel), r.iz = lane_f(L.r.iz, lsel);
+		r.tMin = lane_f(L.r.tMin, lsel), r.tMax = lane_f(L.r.tMax, lsel);
+		r.nanFree = false;
+		c = {lane_u(L.c.tri, lsel), lane_f(L.c.t, lsel), lane_f(L.c.u, lsel), lane_f(L.c.v, lsel),
+		     false};
+		triCur = lane_u(L.s.triCur, lsel), triEnd = lane_u(L.s.triEnd, lsel);
+		pendCur = lane_u(L.s.pendCur, lsel), pendEnd = lane_u(L.s.pendEnd, lsel);
+		cur = int32_t(lane_u(uint32_t(L.s.next), lsel));
+		sp = int(lane_u(uint32_t(st.sp), lsel));
+	}
+	// The stack of lane lsel (LDS column, then its spill column) into two registers: entry e in
+	// lane e of stk0 (e < 64) or of stk1.
+	uint32_t stk0 = 0u, stk1 = 0u;
+	{
+		const uint32_t* spillCol = st.spill - lane + lsel;
+		for (int e = int(lane), h = 0; h < 2; h++, e += 64) {
+			uint32_t v = 0u;
+			if (e < sp)
+				v = e < kLdsDepth ? st.lds[e * 64 + int(lsel)] : spillCol[size_t(e - kLdsDepth) * st.stride];
+			if (h == 0)
+				stk0 = v;
+			else
+				stk1 = v;
+		}
+	}
+	// Slab lanes: lane i < 12 holds plane i of a node -- lc.min.xyz, lc.max.xyz, rc.min.xyz,
+	// rc.max.xyz -- whose axis is i mod 3.
+	const uint32_t axis = lane % 3u;
+	const float oA = axis == 0u ? r.ox : axis == 1u ? r.oy : r.oz;
+	const float iA = axis == 0u ? r.ix : axis == 1u ? r.iy : r.iz;
+	const float* nodeWords = reinterpret_cast<const float*>(P.nodes);
+
+	// Triangles queued when the loop handed over (a leaf pair's ranges), then the nodes.
+	solo_tris<kAnyHit>(P, r, triCur, triEnd - triCur, pendCur, pendEnd - pendCur, c);
+	while (!(kAnyHit && c.any)) {
+		if (cur < 0) {
+			if (sp == 0) break;
+			sp--;
+			cur = int32_t(sp < 64 ? lane_u(stk0, uint32_t(sp)) : lane_u(stk1, uint32_t(sp - 64)));
+		}
+		const float w = as_global(nodeWords)[16u * uint32_t(cur) + (lane & 15u)];
+		// t of plane i (lane i), and of plane i + 3 (the max plane of the same axis and box).
+		const float t1 = (w - oA) * iA;
+		const float t2 = from_lane_plus<3>(t1);
+		// min/max with std::min/std::max semantics (BVHTraversal via Intersection.hpp:77-90).
+		const float lo = dev::min_first(t1, t2), hi = dev::max_first(t1, t2);
+		// lanes 0 (left box) and 6 (right box): combine the x, y, z slabs.
+		const float tIn = dev::max_first(dev::max_first(lo, from_lane_plus<1>(lo)), from_lane_plus<2>(lo));
+		const float tOut = dev::min_first(dev::min_first(hi, from_lane_plus<1>(hi)), from_lane_plus<2>(hi));
+		const float tCurrMax = kAnyHit ? r.tMax : dev::min_first(r.tMax, c.t);
+		const uint64_t vis = __ballot((tOut >= tIn) && tOut > r.tMin && tIn < tCurrMax);
+		const bool visitL = (vis & 1ull) != 0, visitR = (vis & (1ull << 6)) != 0;
+		const int32_t lRaw = int32_t(lane_u(__float_as_uint(w), 12u));
+		const int32_t rRaw = int32_t(lane_u(__float_as_uint(w), 13u));
+		const uint32_t lCnt = lane_u(__float_as_uint(w), 14u);
+		const uint32_t rCnt = lane_u(__float_as_uint(w), 15u);
+		const uint32_t lFirst = child_index(lRaw), rFirst = child_index(rRaw);
+		const bool innerL = visitL && lCnt == 0u, innerR = visitR && rCnt == 0u;
+		if (innerL && innerR) {
+			stk0 = int(lane) == sp ? lFirst : stk0;
+			stk1 = int(lane) + 64 == sp ? lFirst : stk1;
+			sp++;
+		}
+		cur = innerR ? int32_t(rFirst) : innerL ? int32_t(lFirst) : -1;
+		const bool leafL = visitL && !innerL, leafR = visitR && !innerR;
+		if (leafL || leafR)
+			solo_tris<kAnyHit>(P, r, lFirst, leafL ? lCnt : 0u, rFirst, leafR ? rCnt : 0u, c);
+	}
+	if (lane == 0u) {
+		const Closest res = {kAnyHit ? (c.any ? 0u : kInvalid) : c.tri, c.t, c.u, c.v};
+		store_result<kAnyHit>(P, rayIdx, res);
+	}
+}
+
+// Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Without a
+// treelet (kTreelet false) every record comes from global memory and no LDS path is compiled.
 template <bool kAnyHit, bool kCount, bool kTreelet>
 __device__ __forceinline__ void persistent_trace(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
@@ -434,111 +866,42 @@ __device__ __forceinline__ void persistent_trace(
 		__syncthreads();
 	}
 
-	const uint32_t lane = __lane_id();
 	LaneStack st;
 	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
 	st.stride = gridDim.x * blockDim.x;
 	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
-	st.lane = lane;
+	st.lane = __lane_id();
 	st.sp = 0;
 
+	const PipeArgs P = {nodes, tris, rays, hits, inLight, n, work, refillMin,
+	                    kTreelet ? topCount : 0u, finiteBounds, treelet};
 	WaveQueue q;
-	uint32_t rayIdx = kInvalid;
-	LaneTrace s = {0u, 0u, 0u, 0u, -1};
-	dev::Ray r;
-	bool fastSlab = false;
-	Closest c;
+	Tracer L;
+	L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+	L.s = {0u, 0u, 0u, 0u, -1};
+	L.rayIdx = kInvalid;
+	L.doneIdx = kInvalid;
+	L.kind = kRecIdle;
+	L.fastSlab = false;
+	L.r = dev::make_ray(make_float4(0.f, 0.f, 0.f, 0.f), make_float4(1.f, 1.f, 1.f, 0.f));
+	const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+	Rec A = {z, z, z, z, z, z}, B = A;
 	Counts k;
-
-	while (true) {
-		const uint64_t idle = __ballot(rayIdx == kInvalid);
-		bool fresh = false;
-		if (!q.exhausted && idle != 0 &&
-		    (uint32_t(__popcll(idle)) >= refillMin || idle == ~0ull)) {
-			const uint32_t got = refill(q, idle, work, n, rayIdx == kInvalid);
-			if (got != kInvalid) {
-				rayIdx = got;
-				fresh = true;
-			}
-		}
-		if (q.exhausted && __ballot(rayIdx != kInvalid) == 0) break;
-		if (rayIdx == kInvalid) continue;
-
-		// A refilled lane only loads and prepares its ray this iteration; its root node then comes
-		// through the same path as every other node. Words 2-3 of a refilled lane come from the root
-		// node instead of past its 32-B ray, so the global loads need no mask and issue back to
-		// back; a triangle's word 3 is the next triangle's first word (the triangle array carries
-		// 64 B of padding). Treelet nodes are read from LDS instead.
-		const bool doTri = !fresh && s.triCur < s.triEnd;
-		const bool inLds = kTreelet && !fresh && !doTri && uint32_t(s.next) < topCount;
-		// Record address by integer selects (branches here cost a lone lane more than the math).
-		const uint64_t base = fresh ? uint64_t(rays) : doTri ? uint64_t(tris) : uint64_t(nodes);
-		const uint32_t words = fresh ? 2u : doTri ? 3u : 4u;
-		const uint32_t index = fresh ? rayIdx : doTri ? s.triCur : uint32_t(s.next);
-		const float4* src = reinterpret_cast<const float4*>(base + uint64_t(index * words) * 16u);
-		const float4* src23 = fresh ? nodes : src;
-		float4 a0, a1, a2, a3;
-		if (inLds) {
-			const LdsF4* t = as_lds(treelet) + 4u * uint32_t(s.next);
-			a0 = t[0];
-			a1 = t[1];
-			a2 = t[2];
-			a3 = t[3];
-		} else {
-			a0 = as_global(src)[0];
-			a1 = as_global(src)[1];
-			a2 = as_global(src23)[2];
-			a3 = as_global(src23)[3];
-		}
-		TriExtra x;
-		uint32_t cnt = 0;
-		if (doTri) cnt = load_tri_extra(src, s.triEnd - s.triCur, x);
-
-		bool done = false;
-		if (fresh) {
-			r = dev::make_ray(a0, a1);
-			fastSlab = finiteBounds && r.nanFree;
-			c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
-			st.sp = 0;
-			s = {0u, 0u, 0u, 0u, 0};
-		} else if (doTri) {
-			test_tris<kAnyHit, kCount>(r, a0, a1, a2, a3, x, cnt, s.triCur, c, k, done);
-			s.triCur += cnt;
-			if (s.triCur == s.triEnd) {
-				s.triCur = s.pendCur;
-				s.triEnd = s.pendEnd;
-				s.pendCur = s.pendEnd = 0u;
-			}
-		} else {
-			NodeWords nd;
-			nd.w0 = a0;
-			nd.w1 = a1;
-			nd.w2 = a2;
-			nd.w3 = make_int4(__float_as_int(a3.x), __float_as_int(a3.y), __float_as_int(a3.z),
-			                  __float_as_int(a3.w));
-			const float tCurrMax = kAnyHit ? r.tMax : dev::min_first(r.tMax, c.t);
-			bool visitL, visitR;
-			decide(r, fastSlab, nd, tCurrMax, visitL, visitR);
-			if (kCount) {
-				k.nodes++;
-				k.boxHits += uint32_t(visitL) + uint32_t(visitR);
-			}
-			expand(nd, visitL, visitR, s, st);
-		}
-		if (!done) done = s.triCur >= s.triEnd && s.next < 0;
-		if (done) {
-			if (kAnyHit)
-				inLight[rayIdx] = (c.tri == 0u) ? 0 : 1;
-			else
-				hits[rayIdx] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u),
-				                          __float_as_uint(c.v));
-			rayIdx = kInvalid;
-		}
+	int next = kStepGo;
+	while (next == kStepGo) {
+		pipe_step<kAnyHit, kCount, kTreelet, false>(P, L, st, q, A, B, k);
+		next = pipe_step<kAnyHit, kCount, kTreelet, true>(P, L, st, q, B, A, k);
+	}
+	if (L.doneIdx != kInvalid) store_result<kAnyHit>(P, L.doneIdx, L.c);
+	if (!kCount && next == kStepSolo) {
+		// The wave's last rays, one after the other (the loads still in flight land in A, which
+		// is not read again).
+		for (uint64_t busy = __ballot(L.kind != kRecIdle); busy != 0; busy &= busy - 1u)
+			solo_trace<kAnyHit>(P, L, st, uint32_t(__builtin_ctzll(busy)));
 	}
 	if (kCount) add_counts(counters, k);
 	retire_wave(work);
 }
-
 template <bool kCount, bool kTreelet>
 __global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
