@@ -39,7 +39,8 @@ typedef enum phx_status {
 	PHX_ERR_BUDGET = 7,           /* served form: request can never fit the memory budget       */
 	PHX_ERR_NOT_READY = 8,        /* served form: request still in flight (phx_poll)            */
 	PHX_ERR_CANCELLED = 9,        /* served form: request was cancelled before it ran           */
-	PHX_ERR_INVALID_SCENE = 10    /* scene input unusable (no triangles, non-finite vertex ...) */
+	PHX_ERR_INVALID_SCENE = 10,   /* scene input unusable (no triangles, non-finite vertex ...) */
+	PHX_ERR_DEVICE = 11           /* device arithmetic differs from what the kernels rely on     */
 } phx_status;
 
 /* Returns the message of the last failed call on this thread ("" if none). */
@@ -226,6 +227,15 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* params)
 #define PHX_AUTOTUNE_ANYHIT 0x1000u
 phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                         uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen);
+
+/* Device self-check; no counterpart in the reference. The traversal kernels compute 1/a as
+ * v_rcp_f32 refined by one Newton-Raphson step, which on gfx950 equals IEEE division for every a
+ * with 2^-126 <= |a| < 2^126; outside that range they divide. This compares the two on the device:
+ * every significand at the exponents of 2^-126, 2^-1, 2^0 and 2^125, both signs (flags 0; phx_create
+ * runs this and fails with PHX_ERR_DEVICE on a mismatch), or every float of the range
+ * (PHX_CHECK_EXHAUSTIVE, about 4.2e9 inputs). counts[0] = inputs compared, counts[1] = mismatches. */
+#define PHX_CHECK_EXHAUSTIVE 0x1u
+phx_status phx_check_reciprocal(PhxContext* ctx, uint32_t flags, uint64_t counts[2]);
 
 /* Device buffers (CudaBuffer.hpp:93-108 equivalents). Allocations are 256-B aligned. */
 phx_status phx_device_alloc(PhxContext* ctx, size_t bytes, void** outPtr);

@@ -261,7 +261,46 @@ phx_status phx_create(int device, PhxContext** outCtx)
 		return hipFail(e, "phx_create: persistent-kernel state");
 	}
 	registry().add(ctx);
+	// The traversal's reciprocal must equal IEEE division on this device (see phx_check_reciprocal).
+	uint64_t counts[2] = {0, 0};
+	const phx_status st = phx_check_reciprocal(ctx, 0, counts);
+	if (st != PHX_OK || counts[1] != 0) {
+		registry().remove(ctx);
+		hipFree(ctx->work);
+		hipFree(ctx->spill);
+		hipStreamDestroy(ctx->stream);
+		delete ctx;
+		if (st != PHX_OK) return st;
+		return fail(PHX_ERR_DEVICE,
+		            "phx_create: the device's reciprocal differs from IEEE division for %llu of %llu "
+		            "inputs; traversal results would not match the reference",
+		            (unsigned long long)counts[1], (unsigned long long)counts[0]);
+	}
 	*outCtx = ctx;
+	return ok();
+}
+
+phx_status phx_check_reciprocal(PhxContext* ctx, uint32_t flags, uint64_t counts[2])
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	if (!counts) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_check_reciprocal: counts is NULL");
+	if (flags & ~PHX_CHECK_EXHAUSTIVE)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_check_reciprocal: unknown flags 0x%x", flags);
+	PHX_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+	unsigned long long* bad = nullptr;
+	PHX_HIP(hipMalloc(&bad, sizeof(*bad)), "hipMalloc");
+	uint64_t tested = 0;
+	unsigned long long host = 0;
+	hipError_t e = hipMemsetAsync(bad, 0, sizeof(*bad), ctx->stream);
+	if (e == hipSuccess)
+		e = launchCheckReciprocal((flags & PHX_CHECK_EXHAUSTIVE) != 0, bad, &tested, ctx->stream);
+	if (e == hipSuccess) e = hipMemcpyAsync(&host, bad, sizeof(host), hipMemcpyDeviceToHost, ctx->stream);
+	if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+	hipFree(bad);
+	if (e != hipSuccess) return hipFail(e, "phx_check_reciprocal");
+	counts[0] = tested;
+	counts[1] = host;
 	return ok();
 }
 

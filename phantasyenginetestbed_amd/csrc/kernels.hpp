@@ -45,6 +45,12 @@ hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint3
 
 // Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
 // counters of the batch are added to counters[0..4] (device memory).
+// Counts the floats a (exhaustive: all of 2^-126 <= |a| < 2^126; else every significand at four
+// exponents, both signs) whose fast reciprocal differs from IEEE 1/a; adds them to *mismatches.
+// Returns the number of inputs compared in *tested.
+hipError_t launchCheckReciprocal(bool exhaustive, unsigned long long* mismatches, uint64_t* tested,
+                                 hipStream_t stream);
+
 hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, PhxRayHit* hits,
                          const TraceLaunch& launch, unsigned long long* counters,
                          hipStream_t stream);

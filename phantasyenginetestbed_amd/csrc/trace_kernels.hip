@@ -1119,6 +1119,29 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	}
 }
 
+// ---- device self-check of the fast reciprocal -----------------------------------------------------
+
+constexpr uint64_t kBinade = 1ull << 23;           // significands per exponent
+constexpr uint64_t kNormalBinades = 252;            // exponent fields 1 .. 252: 2^-126 <= |a| < 2^126
+constexpr uint32_t kSampleExponents[4] = {1u, 126u, 127u, 252u};
+
+__global__ __launch_bounds__(kBlock) void k_check_reciprocal(bool exhaustive, uint64_t total,
+                                                             unsigned long long* mismatches)
+{
+	const uint64_t perSign = total / 2u;
+	unsigned long long bad = 0;
+	for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total;
+	     i += uint64_t(gridDim.x) * blockDim.x) {
+		const uint32_t sign = i >= perSign ? 0x80000000u : 0u;
+		const uint64_t j = i % perSign;
+		const uint32_t binade = uint32_t(j / kBinade), mant = uint32_t(j % kBinade);
+		const uint32_t expField = exhaustive ? 1u + binade : kSampleExponents[binade];
+		const float a = __uint_as_float(sign | (expField << 23) | mant);
+		if (__float_as_uint(dev::recip_newton(a)) != __float_as_uint(1.0f / a)) bad++;
+	}
+	if (bad) atomicAdd(mismatches, bad);
+}
+
 uint32_t gridFor(uint32_t n, uint32_t block) { return n / block + (n % block != 0u); }
 
 // Persistent launch shape: grid capped by the work and by the spill area's lane count.
@@ -1155,6 +1178,16 @@ hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint3
 		&a, fn, int(blockThreads), persistent_lds_bytes(blockThreads, topNodes));
 	*blocks = uint32_t(a > 0 ? a : 0);
 	return e;
+}
+
+hipError_t launchCheckReciprocal(bool exhaustive, unsigned long long* mismatches, uint64_t* tested,
+                                 hipStream_t stream)
+{
+	const uint64_t total = 2u * kBinade * (exhaustive ? kNormalBinades : 4u);
+	*tested = total;
+	hipLaunchKernelGGL(k_check_reciprocal, dim3(2048), dim3(kBlock), 0, stream, exhaustive, total,
+	                   mismatches);
+	return hipGetLastError();
 }
 
 hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, PhxRayHit* hits,

@@ -34,7 +34,7 @@ FLT_MAX = float(np.finfo(np.float32).max)
 
 # status codes
 OK, ERR_INVALID_ARGUMENT, ERR_INVALID_HANDLE, ERR_OUT_OF_MEMORY, ERR_MALFORMED_BVH, ERR_HIP, \
-    ERR_NO_DEVICE, ERR_BUDGET, ERR_NOT_READY, ERR_CANCELLED, ERR_INVALID_SCENE = range(11)
+    ERR_NO_DEVICE, ERR_BUDGET, ERR_NOT_READY, ERR_CANCELLED, ERR_INVALID_SCENE, ERR_DEVICE = range(12)
 
 # flags
 MEM_DEVICE = 0x1
@@ -47,6 +47,7 @@ BATCH_ANYHIT = 1
 PLAN_TRACK_COMPLETION = 0x10
 PLAN_USE_GRAPH = 0x20
 AUTOTUNE_ANYHIT = 0x1000
+CHECK_EXHAUSTIVE = 0x1
 SERVE_PLAIN = 0x1
 FRAME_TAA = 0x10000
 FRAME_TAA_RESET = 0x20000
@@ -149,6 +150,7 @@ _SIGNATURES = {
     "phx_plan_set_launch_params": (C.c_int, [_P, _P, C.POINTER(LaunchParams)]),
     "phx_plan_autotune": (C.c_int, [_P, _P, C.POINTER(LaunchParams)]),
     "phx_autotune": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(LaunchParams)]),
+    "phx_check_reciprocal": (C.c_int, [_P, _U32, C.POINTER(C.c_uint64)]),
     "phx_camera_def": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
     "phx_camera_look_at": (C.c_int, [_PF, _PF, _PF, _F32, _F32, _F32, C.POINTER(CameraDef)]),
     "phx_gen_primary_rays": (C.c_int, [_P, C.POINTER(CameraDef), _U32, _U32, _F32, _F32, _U32,
@@ -590,6 +592,12 @@ class Context:
         p = LaunchParams(grid_closest, grid_anyhit, refill_min, block_threads,
                          0xFFFFFFFF if top_nodes is None else top_nodes, 0)
         check(lib().phx_set_launch_params(self.handle, C.byref(p)))
+
+    def check_reciprocal(self, exhaustive: bool = False) -> tuple[int, int]:
+        """(inputs compared, mismatches) of the device's fast reciprocal against IEEE division."""
+        c = (C.c_uint64 * 2)()
+        check(lib().phx_check_reciprocal(self.handle, CHECK_EXHAUSTIVE if exhaustive else 0, c))
+        return int(c[0]), int(c[1])
 
     def autotune(self, scene: Scene, rays_ptr: int, n: int, anyhit: bool = False) -> dict:
         """Times candidate launch shapes on device rays and applies the fastest. Returns it;

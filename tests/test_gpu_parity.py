@@ -299,3 +299,62 @@ def test_scene_file_upload_matches_oracle(ctx, tmp_path):
     got = ctx.cast_closest(s, rays)
     assert got.tobytes() == O.cast_closest(b.nodes, b.tris, rays).tobytes()
     s.release()
+
+
+def caterpillar(depth=60, seed=5):
+    """A valid BVH `depth` levels deep whose boxes all cover the scene: spine node i has the side
+    node depth+i (an inner node over two one-triangle leaves) on its left and spine node i+1 on
+    its right, so castRay pushes one entry per level and its stack reaches `depth` entries --
+    past what the persistent kernels keep in LDS."""
+    rng = np.random.default_rng(seed)
+    n = 2 * depth
+    nodes = np.zeros(n, phx.NODE)
+    box = np.array([-10, -10, -10, 10, 10, 10, -10, -10, -10, 10, 10, 10], np.float32)
+    nodes["f"][:] = box
+    ntri = 2 * depth + 1
+    centers = rng.uniform(-1, 1, (ntri, 1, 3)).astype(np.float32)
+    v = centers + rng.uniform(-0.3, 0.3, (ntri, 3, 3)).astype(np.float32)
+    tris = np.zeros(ntri, phx.TRI)
+    tris["v"][:, :, :3] = v
+    tris["v"][:, 0, 3] = -1.0  # each leaf holds one triangle: it is its leaf's last
+    for i in range(depth):
+        q = depth + i
+        nodes["i"][i] = (q, i + 1, 0, 0) if i + 1 < depth else (q, ~(2 * depth), 0, 1)
+        nodes["i"][q] = (~(2 * i), ~(2 * i + 1), 1, 1)
+    return nodes, tris
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
+def test_deep_stack_matches_oracle(ctx, kernel):
+    """Stacks 60 entries deep spill past the LDS part in the persistent kernel, and the wave's
+    last rays are traced by the solo path, which reads the spilled entries back."""
+    nodes, tris = caterpillar()
+    assert phx.validate(nodes, tris) == 61
+    rng = np.random.default_rng(11)
+    n = 3000
+    rays = np.zeros(n, phx.RAY)
+    rays["o"] = rng.uniform(-3, 3, (n, 3))
+    aim = tris["v"][rng.integers(0, len(tris), n), :, :3].mean(axis=1)  # toward a triangle
+    d = aim + rng.normal(0, 0.05, (n, 3)) - rays["o"]
+    rays["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmin"] = 0.0
+    rays["tmax"] = phx.FLT_MAX
+    s = ctx.upload_scene(nodes, tris)
+    try:
+        ref = O.cast_closest(nodes, tris, rays)
+        assert (ref["tri"] != phx.MISS).sum() > n // 10
+        assert_same_hits(ctx.cast_closest(s, rays, kernel), ref, f"deep stack kernel {kernel}")
+        lit = ctx.cast_anyhit(s, rays, kernel)
+        assert (lit == O.cast_anyhit(nodes, tris, rays)).all()
+    finally:
+        s.release()
+
+
+def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
+    """The traversal's 1/a (v_rcp_f32 + one Newton step) equals IEEE division for every float
+    with 2^-126 <= |a| < 2^126 on this device."""
+    tested, bad = ctx.check_reciprocal(exhaustive=True)
+    assert tested == 2 * 252 * (1 << 23)
+    assert bad == 0, f"{bad} of {tested} reciprocals differ"
+    tested, bad = ctx.check_reciprocal()
+    assert tested == 2 * 4 * (1 << 23) and bad == 0
