@@ -21,7 +21,6 @@ namespace {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / 64;
 
 // Address-space views: records are loaded through pointers the compiler knows to be global (or
 // LDS), so the loads are global_load / ds_read rather than flat loads, whose latency is higher
@@ -282,14 +281,6 @@ struct LaneStack {
 	uint32_t lane;
 	int sp;
 
-	__device__ __forceinline__ void push(uint32_t v)
-	{
-		if (sp < kLdsDepth)
-			lds[sp * 64 + lane] = v;
-		else if (sp < kPersistentStack)
-			spill[size_t(sp - kLdsDepth) * stride] = v;
-		sp++;
-	}
 	// Pushes v if doPush. Lanes that do not push (or whose entry spills) store to the extra row, so
 	// the LDS store needs no branch; only a spilling entry takes one.
 	__device__ __forceinline__ void push_if(bool doPush, uint32_t v)
@@ -299,13 +290,6 @@ struct LaneStack {
 		if (doPush && sp >= kLdsDepth && sp < kPersistentStack)
 			spill[size_t(sp - kLdsDepth) * stride] = v;
 		sp += doPush ? 1 : 0;
-	}
-	__device__ __forceinline__ uint32_t pop()
-	{
-		sp--;
-		if (sp < kLdsDepth) return lds[sp * 64 + lane];
-		if (sp < kPersistentStack) return spill[size_t(sp - kLdsDepth) * stride];
-		return 0u;
 	}
 	// Pops if doPop (the value is meaningless otherwise): every lane reads one LDS word -- lanes
 	// that do not pop, or whose entry spilled, read the extra row -- and only a spilled entry
@@ -336,90 +320,23 @@ __device__ __forceinline__ void decide(const dev::Ray& r, bool fastSlab, const N
 	visitR = (outR >= inR) && outR > r.tMin && inR < tCurrMax;
 }
 
-// A triangle step tests up to kTriBatch consecutive triangles of the current leaf range: the first
-// triangle's words come through the step's unified loads, the others are loaded right after them
-// so all are in flight together, and the tests then run in castRay's order (BVHTraversal.hpp:
-// 78-93), so every acceptance is the one castRay makes. Leaves on the paths of costly incoherent
-// rays hold about five triangles, so this shortens their chains of dependent steps.
+// A triangle step tests up to kTriBatch consecutive triangles of the current leaf range, all
+// loaded by the same step, in castRay's order (BVHTraversal.hpp:78-93). Leaves on the paths of
+// costly incoherent rays hold about five triangles, so this shortens their chains of dependent
+// steps. The words of a second triangle past a leaf's last one lie in the next triangle or in the
+// 64 B of padding after the array.
 constexpr uint32_t kTriBatch = 2;
-
-struct TriExtra {
-	float4 w[3 * kTriBatch - 4];  // words 4 .. 3*kTriBatch-1 of the range (word 3 is the 4th load)
-};
-
-__device__ __forceinline__ uint32_t load_tri_extra(const float4* src, uint32_t avail, TriExtra& x)
-{
-	// The words past a leaf's last triangle lie in the next triangle or in the 64 B of padding
-	// after the array, so they are loaded whether or not the leaf has a second triangle.
-	static_assert(kTriBatch == 2, "the padding covers the words of one more triangle");
-	x.w[0] = as_global(src)[4];
-	x.w[1] = as_global(src)[5];
-	return avail < kTriBatch ? avail : kTriBatch;
-}
-
-// Tests triangles first .. first+cnt-1 (words a0..a3 then x); any hit stops at the first hit.
-template <bool kAnyHit, bool kCount>
-__device__ __forceinline__ void test_tris(const dev::Ray& r, float4 a0, float4 a1, float4 a2,
-                                          float4 a3, const TriExtra& x, uint32_t cnt,
-                                          uint32_t first, Closest& c, Counts& k, bool& done)
-{
-#pragma unroll
-	for (uint32_t j = 0; j < kTriBatch; j++) {
-		if (j >= cnt || done) break;
-		const float4 p0 = j == 0 ? a0 : j == 1 ? a3 : x.w[3 * j - 4];
-		const float4 e1 = j == 0 ? a1 : x.w[3 * j - 3];
-		const float4 e2 = j == 0 ? a2 : x.w[3 * j - 2];
-		float t, u, v;
-		const bool hit = dev::triangle_pre(r, p0, e1, e2, t, u, v);
-		if (kAnyHit) {
-			done = hit && r.tMin <= t && t < r.tMax;
-			if (done) c.tri = 0u;
-		} else {
-			if (hit && t < c.t && r.tMin <= t && t <= r.tMax) {
-				c.tri = first + j;
-				c.t = t;
-				c.u = u;
-				c.v = v;
-				if (kCount) k.accepted++;
-			}
-			if (kCount) k.triTests++;
-		}
-	}
-}
 
 // Per-lane traversal state of the persistent kernels. Every iteration a lane does one unit of
 // work in castRay's per-ray order: expand a node (decide both children, queue the triangles of
-// visited leaf children -- left, then right -- and pick the next node), or test one queued
-// triangle. Nodes are expanded only once the queued triangles are done, so each box decision
-// sees the closest.t that castRay would see.
+// visited leaf children -- left, then right -- and pick the next node), or test up to kTriBatch
+// queued triangles. Nodes are expanded only once the queued triangles are done, so each box
+// decision sees the closest.t that castRay would see.
 struct LaneTrace {
 	uint32_t triCur, triEnd;    // triangles being tested
 	uint32_t pendCur, pendEnd;  // right-leaf triangles queued behind them
 	int32_t next;               // node castRay pops next, or -1
 };
-
-__device__ __forceinline__ void expand(const NodeWords& nd, bool visitL, bool visitR,
-                                       LaneTrace& s, LaneStack& st)
-{
-	// Selects instead of branches: a node step starts with an empty triangle range, visited leaf
-	// children become the range (left first, the right one queued behind it), the right inner
-	// child is castRay's next pop, a left inner child with it is pushed, and with no inner child
-	// the stack is popped.
-	const bool innerL = visitL && nd.w3.z == 0;
-	const bool innerR = visitR && nd.w3.w == 0;
-	const bool leafL = visitL && !innerL;
-	const bool leafR = visitR && !innerR;
-	const uint32_t lFirst = child_index(nd.w3.x), rFirst = child_index(nd.w3.y);
-	const uint32_t lEnd = lFirst + uint32_t(nd.w3.z), rEnd = rFirst + uint32_t(nd.w3.w);
-	s.triCur = leafL ? lFirst : leafR ? rFirst : s.triCur;
-	s.triEnd = leafL ? lEnd : leafR ? rEnd : s.triEnd;
-	const bool both = leafL && leafR;
-	s.pendCur = both ? rFirst : s.pendCur;
-	s.pendEnd = both ? rEnd : s.pendEnd;
-	st.push_if(innerL && innerR, lFirst);
-	s.next = innerR ? int32_t(rFirst) : innerL ? int32_t(lFirst) : -1;
-	if (!innerL && !innerR && st.sp > 0) s.next = int32_t(st.pop());
-}
 
 // LDS bytes of a persistent launch: the stacks of every wave plus the treelet.
 __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t blockThreads,
@@ -478,13 +395,19 @@ struct Tracer {
 // Writes a ray's result: its closest hit, or (any hit) 0 if occluded -- c.tri == 0 marks a hit
 // there -- and 1 if not.
 template <bool kAnyHit>
-__device__ __forceinline__ void store_result(const PipeArgs& P, uint32_t idx, const Closest& c)
+__device__ __forceinline__ void store_result(void* out, uint32_t idx, const Closest& c)
 {
 	if (kAnyHit)
-		P.inLight[idx] = c.tri == 0u ? 0 : 1;
+		static_cast<uint8_t*>(out)[idx] = c.tri == 0u ? 0 : 1;
 	else
-		P.hits[idx] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u),
-		                         __float_as_uint(c.v));
+		static_cast<uint4*>(out)[idx] = make_uint4(c.tri, __float_as_uint(c.t),
+		                                           __float_as_uint(c.u), __float_as_uint(c.v));
+}
+template <bool kAnyHit>
+__device__ __forceinline__ void store_result(const PipeArgs& P, uint32_t idx, const Closest& c)
+{
+	store_result<kAnyHit>(kAnyHit ? static_cast<void*>(P.inLight) : static_cast<void*>(P.hits),
+	                      idx, c);
 }
 
 // Both triangles of a pair, branch-free, accepted in castRay's order (BVHTraversal.hpp:78-93).
@@ -536,8 +459,10 @@ __device__ __forceinline__ bool test_pair_any(const dev::Ray& r, const Rec& A, u
 	return (h0 && r.tMin <= t0 && t0 < r.tMax) || (h1 && r.tMin <= t1 && t1 < r.tMax);
 }
 
-// Node expansion for the pipelined loop: as expand(), with the pop done by selects around one
-// LDS read (lanes that do not pop read the spare row); only a spilled entry takes a branch.
+// Node expansion (selects, not branches): visited leaf children become the triangle range (left
+// first, the right one queued behind it), the right inner child is castRay's next pop, a left
+// inner child with it is pushed, and with no inner child the stack is popped -- one LDS read for
+// every lane (lanes that do not pop read the spare row); only a spilled entry takes a branch.
 __device__ __forceinline__ void expand_pipe(const NodeWords& nd, bool visitL, bool visitR,
                                             LaneTrace& s, LaneStack& st)
 {
@@ -709,7 +634,7 @@ struct SoloHit {
 // in order if t is still below the closest t, exactly as the sequential loop does. Any hit:
 // records a hit with tMin <= t < tMax.
 template <bool kAnyHit>
-__device__ __forceinline__ void solo_tris(const PipeArgs& P, const dev::Ray& r, uint32_t a1,
+__device__ __forceinline__ void solo_tris(const float4* tris, const dev::Ray& r, uint32_t a1,
                                           uint32_t n1, uint32_t a2, uint32_t n2, SoloHit& c)
 {
 	const uint32_t lane = __lane_id();
@@ -720,7 +645,7 @@ __device__ __forceinline__ void solo_tris(const PipeArgs& P, const dev::Ray& r, 
 		float t = 0.0f, u = 0.0f, v = 0.0f;
 		bool hit = false;
 		if (have) {
-			const float4* p = P.tris + 3u * tri;
+			const float4* p = tris + 3u * tri;
 			const float4 p0 = as_global(p)[0], e1 = as_global(p)[1], e2 = as_global(p)[2];
 			const dev::TriDet d = dev::tri_det(r, e1, e2);
 			float f = dev::recip_newton(d.a);
@@ -749,10 +674,18 @@ __device__ __forceinline__ void solo_tris(const PipeArgs& P, const dev::Ray& r, 
 	}
 }
 
+// The scene and output of a solo ray (uniform).
+struct SoloScene {
+	const float4* nodes;
+	const float4* tris;
+	const float4* rays;
+	void* out;  // uint4 hits (closest hit) or uint8_t (any hit), indexed by the ray's index
+};
+
 // Traces lane `lsel`'s ray to its end from the state the pipelined loop left (the record in flight
 // is loaded again here) and writes its result.
 template <bool kAnyHit>
-__device__ __forceinline__ void solo_trace(const PipeArgs& P, const Tracer& L, const LaneStack& st,
+__device__ __forceinline__ void solo_trace(const SoloScene& P, const Tracer& L, const LaneStack& st,
                                            uint32_t lsel)
 {
 	const uint32_t lane = __lane_id();
@@ -808,7 +741,7 @@ __device__ __forceinline__ void solo_trace(const PipeArgs& P, const Tracer& L, c
 	const float* nodeWords = reinterpret_cast<const float*>(P.nodes);
 
 	// Triangles queued when the loop handed over (a leaf pair's ranges), then the nodes.
-	solo_tris<kAnyHit>(P, r, triCur, triEnd - triCur, pendCur, pendEnd - pendCur, c);
+	solo_tris<kAnyHit>(P.tris, r, triCur, triEnd - triCur, pendCur, pendEnd - pendCur, c);
 	while (!(kAnyHit && c.any)) {
 		if (cur < 0) {
 			if (sp == 0) break;
@@ -841,11 +774,11 @@ __device__ __forceinline__ void solo_trace(const PipeArgs& P, const Tracer& L, c
 		cur = innerR ? int32_t(rFirst) : innerL ? int32_t(lFirst) : -1;
 		const bool leafL = visitL && !innerL, leafR = visitR && !innerR;
 		if (leafL || leafR)
-			solo_tris<kAnyHit>(P, r, lFirst, leafL ? lCnt : 0u, rFirst, leafR ? rCnt : 0u, c);
+			solo_tris<kAnyHit>(P.tris, r, lFirst, leafL ? lCnt : 0u, rFirst, leafR ? rCnt : 0u, c);
 	}
 	if (lane == 0u) {
 		const Closest res = {kAnyHit ? (c.any ? 0u : kInvalid) : c.tri, c.t, c.u, c.v};
-		store_result<kAnyHit>(P, rayIdx, res);
+		store_result<kAnyHit>(P.out, rayIdx, res);
 	}
 }
 
@@ -896,8 +829,10 @@ __device__ __forceinline__ void persistent_trace(
 	if (!kCount && next == kStepSolo) {
 		// The wave's last rays, one after the other (the loads still in flight land in A, which
 		// is not read again).
+		const SoloScene S = {nodes, tris, rays,
+		                     kAnyHit ? static_cast<void*>(inLight) : static_cast<void*>(hits)};
 		for (uint64_t busy = __ballot(L.kind != kRecIdle); busy != 0; busy &= busy - 1u)
-			solo_trace<kAnyHit>(P, L, st, uint32_t(__builtin_ctzll(busy)));
+			solo_trace<kAnyHit>(S, L, st, uint32_t(__builtin_ctzll(busy)));
 	}
 	if (kCount) add_counts(counters, k);
 	retire_wave(work);
@@ -953,23 +888,172 @@ __device__ __forceinline__ uint32_t find_batch(const uint32_t* offsets, uint32_t
 	return lo;
 }
 
-template <bool kTreelet>
-__global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
-	const BatchDesc* __restrict__ descs, const uint32_t* __restrict__ offsetsGlobal,
-	uint32_t numBatches, uint32_t total, uint32_t* __restrict__ work,
-	uint32_t* __restrict__ spill, uint32_t refillMin, const float4* __restrict__ treeletNodes,
-	uint32_t topCount, BatchClock* __restrict__ clock)
+// Launch-wide arguments of the fused kernel.
+struct MultiArgs {
+	const BatchDesc* descs;
+	const uint32_t* offsets;  // LDS copy: offsets[b] = first global ray index of batch b
+	uint32_t numBatches;
+	uint32_t total;
+	uint32_t* work;
+	uint32_t refillMin;
+	uint32_t topCount;
+	const float4* treeletNodes;  // the scene whose top nodes are in the LDS treelet
+	const float4* treelet;       // LDS
+	unsigned long long* blockFinish;  // LDS, per batch (completion tracking)
+	uint32_t* blockDone;              // LDS, per batch
+};
+
+// Per-lane state of the fused kernel: the pipelined loop's, plus the scene, output and kind of
+// the lane's batch.
+struct MultiTracer {
+	Tracer t;
+	const float4* nodes;
+	const float4* tris;
+	void* out;
+	uint32_t batch;
+	bool anyHit;
+	bool finiteBounds;
+	bool inTreelet;
+};
+
+__device__ __forceinline__ void track_done(const MultiArgs& P, uint32_t batch)
+{
+	// Block-local LDS atomics; flushed to the batch table when the block retires.
+	atomicAdd(&P.blockDone[batch], 1u);
+	atomicMax(&P.blockFinish[batch], wall_clock64());
+}
+
+// pipe_step for the fused kernel: the same iteration, with the scene, output and kind taken from
+// each lane's batch (looked up in the batch table when the lane takes a new ray). A lane's result
+// is written with its own batch's output before a new ray's batch replaces it.
+template <bool kTreelet, bool kTrack, bool kExitCheck>
+__device__ __forceinline__ int pipe_step_multi(const MultiArgs& P, MultiTracer& M, LaneStack& st,
+                                               WaveQueue& q, const Rec& A, Rec& B)
+{
+	__builtin_amdgcn_s_waitcnt(kWaitVmcntZero);
+	Tracer& L = M.t;
+	Counts none;
+	const uint32_t kind = L.kind;
+	uint32_t triFirst = 0u, triCnt = 0u;
+	bool anyHit = false;
+	if (kind == kRecTri) {
+		triFirst = L.s.triCur;
+		const uint32_t left = L.s.triEnd - triFirst;
+		triCnt = left < kTriBatch ? left : kTriBatch;
+		const bool rangeDone = left <= kTriBatch;
+		L.s.triCur = rangeDone ? L.s.pendCur : triFirst + kTriBatch;
+		L.s.triEnd = rangeDone ? L.s.pendEnd : L.s.triEnd;
+		L.s.pendCur = rangeDone ? 0u : L.s.pendCur;
+		L.s.pendEnd = rangeDone ? 0u : L.s.pendEnd;
+		if (M.anyHit) anyHit = test_pair_any(L.r, A, triCnt);
+	} else if (kind != kRecIdle) {
+		if (kind == kRecRay) {
+			L.r = dev::make_ray(A.w4, A.w5);
+			L.fastSlab = M.finiteBounds && L.r.nanFree;
+			L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+			st.sp = 0;
+			L.s = {0u, 0u, 0u, 0u, -1};
+		}
+		NodeWords nd;
+		nd.w0 = A.w0;
+		nd.w1 = A.w1;
+		nd.w2 = A.w2;
+		nd.w3 = make_int4(__float_as_int(A.w3.x), __float_as_int(A.w3.y), __float_as_int(A.w3.z),
+		                  __float_as_int(A.w3.w));
+		const float tCurrMax = M.anyHit ? L.r.tMax : dev::min_first(L.r.tMax, L.c.t);
+		bool visitL, visitR;
+		decide(L.r, L.fastSlab, nd, tCurrMax, visitL, visitR);
+		expand_pipe(nd, visitL, visitR, L.s, st);
+	}
+
+	const bool more = kind != kRecIdle && !anyHit && (L.s.triCur < L.s.triEnd || L.s.next >= 0);
+	uint32_t fresh = kInvalid;
+	if (!q.exhausted) {
+		const uint64_t freeLanes = __ballot(!more);
+		if (freeLanes != 0 && (uint32_t(__popcll(freeLanes)) >= P.refillMin || freeLanes == ~0ull))
+			fresh = refill(q, freeLanes, P.work, P.total, !more);
+	}
+	const bool tri = more && L.s.triCur < L.s.triEnd;
+	const bool node = more && !tri;
+	const bool ray = fresh != kInvalid;
+	// A new ray's batch: the largest b with offsets[b] <= its global index.
+	uint32_t nb = 0u, local = 0u;
+	BatchDesc nd{};
+	if (ray) {
+		nb = find_batch(P.offsets, P.numBatches, fresh);
+		local = fresh - P.offsets[nb];
+		nd = P.descs[nb];
+	}
+	const uint32_t nextKind = tri ? kRecTri : node ? kRecNode : ray ? kRecRay : kRecIdle;
+	// The same six loads for every lane: a node, a triangle pair, or a new ray's root node and
+	// 32 B; idle lanes and words 4-5 of a node read their scene's root.
+	const float4* nodes = ray ? nd.nodes : M.nodes;
+	const uint32_t idx = tri ? L.s.triCur : node ? uint32_t(L.s.next) : 0u;
+	const uint32_t word = tri ? idx + (idx << 1) : idx << 2;
+	const float4* rec = (tri ? M.tris : nodes) + word;
+	const float4* src45 = tri ? rec + 4 : ray ? nd.rays + 2u * size_t(local) : nodes;
+	const bool inTreelet = ray ? nd.nodes == P.treeletNodes : M.inTreelet;
+	if (kTreelet && !tri && inTreelet && idx < P.topCount) {
+		const LdsF4* t = as_lds(P.treelet) + 4u * idx;
+		B.w0 = t[0];
+		B.w1 = t[1];
+		B.w2 = t[2];
+		B.w3 = t[3];
+	} else {
+		B.w0 = as_global(rec)[0];
+		B.w1 = as_global(rec)[1];
+		B.w2 = as_global(rec)[2];
+		B.w3 = as_global(rec)[3];
+	}
+	B.w4 = as_global(src45)[0];
+	B.w5 = as_global(src45)[1];
+
+	if (kind == kRecTri && !M.anyHit) test_pair_closest<false>(L.r, A, triCnt, triFirst, L.c, none);
+
+	if (kind != kRecIdle && !more) {
+		if (M.anyHit)
+			static_cast<uint8_t*>(M.out)[L.rayIdx] = anyHit ? 0 : 1;
+		else
+			static_cast<uint4*>(M.out)[L.rayIdx] = make_uint4(
+				L.c.tri, __float_as_uint(L.c.t), __float_as_uint(L.c.u), __float_as_uint(L.c.v));
+		if (kTrack) track_done(P, M.batch);
+	}
+	if (ray) {
+		M.nodes = nd.nodes;
+		M.tris = nd.tris;
+		M.out = nd.out;
+		M.batch = nb;
+		M.anyHit = nd.kind != 0u;
+		M.finiteBounds = nd.finiteBounds != 0u;
+		M.inTreelet = inTreelet;
+	}
+	if (!more) L.rayIdx = ray ? local : kInvalid;
+	L.kind = nextKind;
+	if (!kExitCheck || !q.exhausted) return kStepGo;
+	const uint64_t busy = __ballot(nextKind != kRecIdle);
+	if (busy == 0) return kStepDone;
+	return uint32_t(__popcll(busy)) <= kSoloRays ? kStepSolo : kStepGo;
+}
+
+template <bool kTreelet, bool kTrack>
+__device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ descs,
+                                              const uint32_t* __restrict__ offsetsGlobal,
+                                              uint32_t numBatches, uint32_t total,
+                                              uint32_t* __restrict__ work,
+                                              uint32_t* __restrict__ spill, uint32_t refillMin,
+                                              const float4* __restrict__ treeletNodes,
+                                              uint32_t topCount, BatchClock* __restrict__ clock)
 {
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
 	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
 	unsigned long long* blockFinish = reinterpret_cast<unsigned long long*>(treelet + 4u * topCount);
-	uint32_t* blockDone = reinterpret_cast<uint32_t*>(blockFinish + (clock ? numBatches : 0u));
-	uint32_t* offsets = blockDone + (clock ? numBatches : 0u);
-	if (clock && threadIdx.x == 0) atomicMin(&clock->start, wall_clock64());
+	uint32_t* blockDone = reinterpret_cast<uint32_t*>(blockFinish + (kTrack ? numBatches : 0u));
+	uint32_t* offsets = blockDone + (kTrack ? numBatches : 0u);
+	if (kTrack && threadIdx.x == 0) atomicMin(&clock->start, wall_clock64());
 	for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = treeletNodes[i];
 	for (uint32_t i = threadIdx.x; i <= numBatches; i += blockDim.x) offsets[i] = offsetsGlobal[i];
-	if (clock)
+	if (kTrack)
 		for (uint32_t i = threadIdx.x; i < numBatches; i += blockDim.x) {
 			blockFinish[i] = 0ull;
 			blockDone[i] = 0u;
@@ -983,118 +1067,47 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	st.lane = __lane_id();
 	st.sp = 0;
 
+	const MultiArgs P = {descs, offsets, numBatches, total, work, refillMin,
+	                     kTreelet ? topCount : 0u, treeletNodes, treelet, blockFinish, blockDone};
 	WaveQueue q;
-	uint32_t rayIdx = kInvalid;  // index within the batch
-	uint32_t batch = 0;
-	LaneTrace s = {0u, 0u, 0u, 0u, -1};
-	dev::Ray r;
-	bool fastSlab = false, anyHit = false, useTreelet = false;
-	const float4* nodes = nullptr;
-	const float4* tris = nullptr;
-	void* out = nullptr;
-	Closest c;
-
-	while (true) {
-		const uint64_t idle = __ballot(rayIdx == kInvalid);
-		bool fresh = false;
-		uint32_t g = kInvalid;
-		if (!q.exhausted && idle != 0 &&
-		    (uint32_t(__popcll(idle)) >= refillMin || idle == ~0ull)) {
-			g = refill(q, idle, work, total, rayIdx == kInvalid);
-			fresh = g != kInvalid;
-		}
-		if (q.exhausted && __ballot(rayIdx != kInvalid || fresh) == 0) break;
-
-		const float4* src;
-		const float4* src23;
-		if (fresh) {
-			batch = find_batch(offsets, numBatches, g);
-			rayIdx = g - offsets[batch];
-			const BatchDesc d = descs[batch];
-			nodes = d.nodes;
-			tris = d.tris;
-			out = d.out;
-			anyHit = d.kind != 0;
-			useTreelet = d.nodes == treeletNodes;
-			fastSlab = d.finiteBounds != 0;
-			src = d.rays + 2 * size_t(rayIdx);
-			src23 = nodes;
-		} else {
-			src = nullptr;
-			src23 = nullptr;
-		}
-		if (rayIdx == kInvalid) continue;
-
-		const bool doTri = !fresh && s.triCur < s.triEnd;
-		const bool inLds = kTreelet && !fresh && !doTri && useTreelet && uint32_t(s.next) < topCount;
-		if (!fresh) {
-			src = doTri ? tris + 3u * s.triCur : nodes + 4u * uint32_t(s.next);
-			src23 = src;
-		}
-		float4 a0, a1, a2, a3;
-		if (inLds) {
-			const LdsF4* t = as_lds(treelet) + 4u * uint32_t(s.next);
-			a0 = t[0];
-			a1 = t[1];
-			a2 = t[2];
-			a3 = t[3];
-		} else {
-			a0 = as_global(src)[0];
-			a1 = as_global(src)[1];
-			a2 = as_global(src23)[2];
-			a3 = as_global(src23)[3];
-		}
-		TriExtra x;
-		uint32_t cnt = 0;
-		if (doTri) cnt = load_tri_extra(src, s.triEnd - s.triCur, x);
-
-		bool done = false;
-		if (fresh) {
-			r = dev::make_ray(a0, a1);
-			fastSlab = fastSlab && r.nanFree;
-			c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
-			st.sp = 0;
-			s = {0u, 0u, 0u, 0u, 0};
-		} else if (doTri) {
-			Counts none;
-			if (anyHit)
-				test_tris<true, false>(r, a0, a1, a2, a3, x, cnt, s.triCur, c, none, done);
+	MultiTracer M;
+	Tracer& L = M.t;
+	L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+	L.s = {0u, 0u, 0u, 0u, -1};
+	L.rayIdx = kInvalid;
+	L.doneIdx = kInvalid;
+	L.kind = kRecIdle;
+	L.fastSlab = false;
+	L.r = dev::make_ray(make_float4(0.f, 0.f, 0.f, 0.f), make_float4(1.f, 1.f, 1.f, 0.f));
+	// Until a lane takes its first ray, it reads batch 0's root node.
+	M.nodes = descs[0].nodes;
+	M.tris = descs[0].tris;
+	M.out = descs[0].out;
+	M.batch = 0u;
+	M.anyHit = false;
+	M.finiteBounds = false;
+	M.inTreelet = false;
+	const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+	Rec A = {z, z, z, z, z, z}, B = A;
+	int next = kStepGo;
+	while (next == kStepGo) {
+		pipe_step_multi<kTreelet, kTrack, false>(P, M, st, q, A, B);
+		next = pipe_step_multi<kTreelet, kTrack, true>(P, M, st, q, B, A);
+	}
+	if (next == kStepSolo) {
+		for (uint64_t busy = __ballot(L.kind != kRecIdle); busy != 0; busy &= busy - 1u) {
+			const uint32_t lsel = uint32_t(__builtin_ctzll(busy));
+			const uint32_t b = lane_u(M.batch, lsel);
+			const BatchDesc d = descs[b];
+			const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
+			if (d.kind != 0u)
+				solo_trace<true>(S, L, st, lsel);
 			else
-				test_tris<false, false>(r, a0, a1, a2, a3, x, cnt, s.triCur, c, none, done);
-			s.triCur += cnt;
-			if (s.triCur == s.triEnd) {
-				s.triCur = s.pendCur;
-				s.triEnd = s.pendEnd;
-				s.pendCur = s.pendEnd = 0u;
-			}
-		} else {
-			NodeWords nd;
-			nd.w0 = a0;
-			nd.w1 = a1;
-			nd.w2 = a2;
-			nd.w3 = make_int4(__float_as_int(a3.x), __float_as_int(a3.y), __float_as_int(a3.z),
-			                  __float_as_int(a3.w));
-			const float tCurrMax = anyHit ? r.tMax : dev::min_first(r.tMax, c.t);
-			bool visitL, visitR;
-			decide(r, fastSlab, nd, tCurrMax, visitL, visitR);
-			expand(nd, visitL, visitR, s, st);
-		}
-		if (!done) done = s.triCur >= s.triEnd && s.next < 0;
-		if (done) {
-			if (anyHit)
-				static_cast<uint8_t*>(out)[rayIdx] = (c.tri == 0u) ? 0 : 1;
-			else
-				static_cast<uint4*>(out)[rayIdx] = make_uint4(
-					c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
-			rayIdx = kInvalid;
-		}
-		if (clock && done) {
-			// Block-local LDS atomics; flushed to the batch table when the block retires.
-			atomicAdd(&blockDone[batch], 1u);
-			atomicMax(&blockFinish[batch], wall_clock64());
+				solo_trace<false>(S, L, st, lsel);
+			if (kTrack && __lane_id() == 0u) track_done(P, b);
 		}
 	}
-	if (clock) {
+	if (kTrack) {
 		__syncthreads();
 		for (uint32_t b = threadIdx.x; b < numBatches; b += blockDim.x)
 			if (blockDone[b]) {
@@ -1110,13 +1123,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 		if (last) {
 			atomicExch(&work[0], 0u);
 			atomicExch(&work[1], 0u);
-			if (clock) {
+			if (kTrack) {
 				for (uint32_t b = 0; b < numBatches; b++) atomicExch(&clock->done[b], 0u);
 				clock->lastStart = clock->start;
 				atomicExch(&clock->start, ~0ull);
 			}
 		}
 	}
+}
+
+template <bool kTreelet, bool kTrack>
+__global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
+	const BatchDesc* __restrict__ descs, const uint32_t* __restrict__ offsetsGlobal,
+	uint32_t numBatches, uint32_t total, uint32_t* __restrict__ work,
+	uint32_t* __restrict__ spill, uint32_t refillMin, const float4* __restrict__ treeletNodes,
+	uint32_t topCount, BatchClock* __restrict__ clock)
+{
+	trace_batches<kTreelet, kTrack>(descs, offsetsGlobal, numBatches, total, work, spill, refillMin,
+	                                treeletNodes, topCount, clock);
 }
 
 // ---- device self-check of the fast reciprocal -----------------------------------------------------
@@ -1253,10 +1277,12 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
 	uint32_t grid = L.gridBlocks < gridFor(total, block) ? L.gridBlocks : gridFor(total, block);
 	if (grid > L.spillLanes / block) grid = L.spillLanes / block;
 	if (grid == 0) grid = 1;
-	hipLaunchKernelGGL(top ? k_trace_batches<true> : k_trace_batches<false>, dim3(grid), dim3(block),
+	auto kernel = clock ? (top ? k_trace_batches<true, true> : k_trace_batches<false, true>)
+	                    : (top ? k_trace_batches<true, false> : k_trace_batches<false, false>);
+	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block),
 	                   batchLdsBytes(block, top, numBatches, clock != nullptr), stream, descs,
-	                   offsets, numBatches,
-	                   total, L.work, L.spill, L.refillMin, treeletNodes, top, clock);
+	                   offsets, numBatches, total, L.work, L.spill, L.refillMin, treeletNodes, top,
+	                   clock);
 	return hipGetLastError();
 }
 
