@@ -338,11 +338,14 @@ struct LaneTrace {
 	int32_t next;               // node castRay pops next, or -1
 };
 
-// LDS bytes of a persistent launch: the stacks of every wave plus the treelet.
+__host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t blockThreads);
+
+// LDS bytes of a persistent launch: the stacks of every wave, the treelet, the handoff area.
 __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t blockThreads,
                                                                   uint32_t topCount)
 {
-	return (blockThreads / 64u) * uint32_t(kLdsRows) * 64u * 4u + topCount * 64u;
+	return (blockThreads / 64u) * uint32_t(kLdsRows) * 64u * 4u + topCount * 64u +
+	       handoff_bytes(blockThreads);
 }
 
 // ---- software-pipelined persistent loop -------------------------------------------------------
@@ -682,21 +685,57 @@ struct SoloScene {
 	void* out;  // uint4 hits (closest hit) or uint8_t (any hit), indexed by the ray's index
 };
 
-// Traces lane `lsel`'s ray to its end from the state the pipelined loop left (the record in flight
-// is loaded again here) and writes its result.
+// Where a solo ray starts: the state the pipelined loop left for it (the record in flight is
+// loaded again). Uniform values.
+struct SoloStart {
+	float4 oMin, dMax;            // origin and tMin, direction and tMax (1/dir is recomputed)
+	Closest c;                    // closest hit so far
+	uint32_t triCur, triEnd, pendCur, pendEnd;  // queued triangle ranges
+	int32_t next;                 // node to expand next, or -1
+	int sp;                       // stack entries
+	uint32_t rayIdx;
+	uint32_t kind;                // kRecRay: the ray is not set up yet (load it from rays[rayIdx])
+	const uint32_t* ldsCol;       // the owner lane's LDS stack column (entry e at ldsCol[64 e])
+	const uint32_t* spillCol;     // its spill column (entry e >= kLdsDepth at [(e - kLdsDepth) stride])
+	uint32_t stride;
+};
+
+// The start state of lane lsel of this wave.
+__device__ __forceinline__ SoloStart solo_start_lane(const Tracer& L, const LaneStack& st,
+                                                     uint32_t lsel)
+{
+	SoloStart S;
+	S.oMin = make_float4(lane_f(L.r.ox, lsel), lane_f(L.r.oy, lsel), lane_f(L.r.oz, lsel),
+	                     lane_f(L.r.tMin, lsel));
+	S.dMax = make_float4(lane_f(L.r.dx, lsel), lane_f(L.r.dy, lsel), lane_f(L.r.dz, lsel),
+	                     lane_f(L.r.tMax, lsel));
+	S.c = {lane_u(L.c.tri, lsel), lane_f(L.c.t, lsel), lane_f(L.c.u, lsel), lane_f(L.c.v, lsel)};
+	S.triCur = lane_u(L.s.triCur, lsel);
+	S.triEnd = lane_u(L.s.triEnd, lsel);
+	S.pendCur = lane_u(L.s.pendCur, lsel);
+	S.pendEnd = lane_u(L.s.pendEnd, lsel);
+	S.next = int32_t(lane_u(uint32_t(L.s.next), lsel));
+	S.sp = int(lane_u(uint32_t(st.sp), lsel));
+	S.rayIdx = lane_u(L.rayIdx, lsel);
+	S.kind = lane_u(L.kind, lsel);
+	S.ldsCol = st.lds + lsel;
+	S.spillCol = st.spill - __lane_id() + lsel;
+	S.stride = st.stride;
+	return S;
+}
+
+// Traces a ray to its end from S and writes its result.
 template <bool kAnyHit>
-__device__ __forceinline__ void solo_trace(const SoloScene& P, const Tracer& L, const LaneStack& st,
-                                           uint32_t lsel)
+__device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 {
 	const uint32_t lane = __lane_id();
-	const uint32_t kind = lane_u(L.kind, lsel);
-	const uint32_t rayIdx = lane_u(L.rayIdx, lsel);
+	const uint32_t rayIdx = S.rayIdx;
 	dev::Ray r;
 	SoloHit c;
 	uint32_t triCur, triEnd, pendCur, pendEnd;
 	int32_t cur;
 	int sp;
-	if (kind == kRecRay) {
+	if (S.kind == kRecRay) {
 		const float4* src = P.rays + 2u * size_t(rayIdx);
 		const float4 a = as_global(src)[0], b = as_global(src)[1];
 		r = dev::make_ray(make_float4(lane_f(a.x, 0), lane_f(a.y, 0), lane_f(a.z, 0), lane_f(a.w, 0)),
@@ -706,32 +745,23 @@ __device__ __forceinline__ void solo_trace(const SoloScene& P, const Tracer& L, 
 		cur = 0;
 		sp = 0;
 	} else {
-		r.ox = lane_f(L.r.ox, lsel), r.oy = lane_f(L.r.oy, lsel), r.oz = lane_f(L.r.oz, lsel);
-		r.dx = lane_f(L.r.dx, lsel), r.dy = lane_f(L.r.dy, lsel), r.dz = lane_f(L.r.dz, lsel);
-		r.ix = lane_f(L.r.ix, lsel), r.iy = lane_f(L.r.iy, lsel), r.iz = lane_f(L.r.iz, lsel);
-		r.tMin = lane_f(L.r.tMin, lsel), r.tMax = lane_f(L.r.tMax, lsel);
-		r.nanFree = false;
-		c = {lane_u(L.c.tri, lsel), lane_f(L.c.t, lsel), lane_f(L.c.u, lsel), lane_f(L.c.v, lsel),
-		     false};
-		triCur = lane_u(L.s.triCur, lsel), triEnd = lane_u(L.s.triEnd, lsel);
-		pendCur = lane_u(L.s.pendCur, lsel), pendEnd = lane_u(L.s.pendEnd, lsel);
-		cur = int32_t(lane_u(uint32_t(L.s.next), lsel));
-		sp = int(lane_u(uint32_t(st.sp), lsel));
+		r = dev::make_ray(S.oMin, S.dMax);
+		c = {S.c.tri, S.c.t, S.c.u, S.c.v, false};
+		triCur = S.triCur, triEnd = S.triEnd, pendCur = S.pendCur, pendEnd = S.pendEnd;
+		cur = S.next;
+		sp = S.sp;
 	}
-	// The stack of lane lsel (LDS column, then its spill column) into two registers: entry e in
-	// lane e of stk0 (e < 64) or of stk1.
+	// The owner's stack (LDS column, then spill column) into two registers: entry e in lane e of
+	// stk0 (e < 64) or of stk1.
 	uint32_t stk0 = 0u, stk1 = 0u;
-	{
-		const uint32_t* spillCol = st.spill - lane + lsel;
-		for (int e = int(lane), h = 0; h < 2; h++, e += 64) {
-			uint32_t v = 0u;
-			if (e < sp)
-				v = e < kLdsDepth ? st.lds[e * 64 + int(lsel)] : spillCol[size_t(e - kLdsDepth) * st.stride];
-			if (h == 0)
-				stk0 = v;
-			else
-				stk1 = v;
-		}
+	for (int e = int(lane), h = 0; h < 2; h++, e += 64) {
+		uint32_t v = 0u;
+		if (e < sp)
+			v = e < kLdsDepth ? S.ldsCol[e * 64] : S.spillCol[size_t(e - kLdsDepth) * S.stride];
+		if (h == 0)
+			stk0 = v;
+		else
+			stk1 = v;
 	}
 	// Slab lanes: lane i < 12 holds plane i of a node -- lc.min.xyz, lc.max.xyz, rc.min.xyz,
 	// rc.max.xyz -- whose axis is i mod 3.
@@ -782,6 +812,143 @@ __device__ __forceinline__ void solo_trace(const SoloScene& P, const Tracer& L, 
 	}
 }
 
+// ---- handing solo rays to the block's other waves ------------------------------------------------
+//
+// A wave that goes solo keeps one ray and offers the others in LDS slots; waves of the block that
+// have nothing left claim them and trace them solo, so a wave's last rays run side by side
+// instead of one after another. The stack of an offered ray stays where it is (its owner's LDS
+// column and spill column, which the owner no longer writes). Area layout (32-bit words):
+// [0] slots allocated, [1] slots claimed, [2] waves that may still offer, [3] unused, then
+// kSlotWords per slot. A claimed slot index is below the allocated count, and its owner marks it
+// ready right after writing it, so a claimant's wait ends; the last claim attempts end once no
+// wave may offer and every slot is claimed.
+
+constexpr uint32_t kHandoffPerWave = kSoloRays - 1u;  // a wave offers at most this many rays
+constexpr uint32_t kSlotWords = 24;
+
+__host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t blockThreads)
+{
+	return 16u + (blockThreads / 64u) * kHandoffPerWave * kSlotWords * 4u;
+}
+
+__device__ __forceinline__ uint32_t lds_load(const uint32_t* p)
+{
+	return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Every thread of the block calls this before the block's first barrier: resets the counters and
+// every slot's ready word (LDS holds whatever an earlier block left there, and a stale ready word
+// would let a claimant read a slot before its owner writes it).
+__device__ __forceinline__ void handoff_init(uint32_t* H)
+{
+	if (threadIdx.x == 0) {
+		H[0] = 0u;
+		H[1] = 0u;
+		H[2] = blockDim.x / 64u;
+		H[3] = 0u;
+	}
+	const uint32_t slots = (blockDim.x / 64u) * kHandoffPerWave;
+	for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) H[4u + i * kSlotWords + 22u] = 0u;
+}
+
+// Offers the rays of the lanes in `offer` (each lane writes its own slot), then retires this wave
+// as a producer.
+__device__ __forceinline__ void handoff_offer(uint32_t* H, uint64_t offer, const Tracer& L,
+                                              const LaneStack& st, uint32_t batch)
+{
+	const uint32_t lane = __lane_id();
+	const uint32_t n = uint32_t(__popcll(offer));
+	if (n != 0u) {
+		uint32_t base = 0u;
+		if (lane == 0u) base = atomicAdd(&H[0], n);
+		base = lane_u(base, 0u);
+		const bool mine = ((offer >> lane) & 1ull) != 0;
+		uint32_t* w = H + 4u + (base + uint32_t(__popcll(offer & __lanemask_lt()))) * kSlotWords;
+		if (mine) {
+			w[0] = __float_as_uint(L.r.ox);
+			w[1] = __float_as_uint(L.r.oy);
+			w[2] = __float_as_uint(L.r.oz);
+			w[3] = __float_as_uint(L.r.tMin);
+			w[4] = __float_as_uint(L.r.dx);
+			w[5] = __float_as_uint(L.r.dy);
+			w[6] = __float_as_uint(L.r.dz);
+			w[7] = __float_as_uint(L.r.tMax);
+			w[8] = L.c.tri;
+			w[9] = __float_as_uint(L.c.t);
+			w[10] = __float_as_uint(L.c.u);
+			w[11] = __float_as_uint(L.c.v);
+			w[12] = L.s.triCur;
+			w[13] = L.s.triEnd;
+			w[14] = L.s.pendCur;
+			w[15] = L.s.pendEnd;
+			w[16] = uint32_t(L.s.next);
+			w[17] = uint32_t(st.sp);
+			w[18] = L.rayIdx;
+			w[19] = L.kind;
+			w[20] = ((threadIdx.x >> 6) << 6) | lane;  // owner: wave in block, lane
+			w[21] = batch;
+		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+		if (mine) __hip_atomic_store(&w[22], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+	}
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+	if (lane == 0u) atomicSub(&H[2], 1u);
+}
+
+// Claims an offered ray: its slot, or kInvalid once none is left and no wave can offer more.
+__device__ __forceinline__ uint32_t handoff_claim(uint32_t* H)
+{
+	uint32_t got = kInvalid;
+	if (__lane_id() == 0u) {
+		while (true) {
+			const uint32_t c = lds_load(&H[1]);
+			if (c < lds_load(&H[0])) {
+				if (atomicCAS(&H[1], c, c + 1u) == c) {
+					got = c;
+					break;
+				}
+				continue;
+			}
+			if (lds_load(&H[2]) == 0u && lds_load(&H[1]) >= lds_load(&H[0])) break;
+			__builtin_amdgcn_s_sleep(2);
+		}
+	}
+	return lane_u(got, 0u);
+}
+
+// The start state in a claimed slot (waits for its owner to mark it ready).
+__device__ __forceinline__ SoloStart solo_start_slot(const uint32_t* H, uint32_t slot,
+                                                     const uint32_t* ldsStacks,
+                                                     const uint32_t* spill, uint32_t stride,
+                                                     uint32_t& batch)
+{
+	const uint32_t* w = H + 4u + slot * kSlotWords;
+	while (lane_u(lds_load(&w[22]), 0u) == 0u) __builtin_amdgcn_s_sleep(1);
+	uint32_t v[22];
+#pragma unroll
+	for (int i = 0; i < 22; i++) v[i] = __builtin_amdgcn_readfirstlane(w[i]);
+	SoloStart S;
+	S.oMin = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+	                     __uint_as_float(v[3]));
+	S.dMax = make_float4(__uint_as_float(v[4]), __uint_as_float(v[5]), __uint_as_float(v[6]),
+	                     __uint_as_float(v[7]));
+	S.c = {v[8], __uint_as_float(v[9]), __uint_as_float(v[10]), __uint_as_float(v[11])};
+	S.triCur = v[12];
+	S.triEnd = v[13];
+	S.pendCur = v[14];
+	S.pendEnd = v[15];
+	S.next = int32_t(v[16]);
+	S.sp = int(v[17]);
+	S.rayIdx = v[18];
+	S.kind = v[19];
+	const uint32_t wave = v[20] >> 6, lane = v[20] & 63u;
+	S.ldsCol = ldsStacks + wave * uint32_t(kLdsRows) * 64u + lane;
+	S.spillCol = spill + (blockIdx.x * blockDim.x + wave * 64u + lane);
+	S.stride = stride;
+	batch = v[21];
+	return S;
+}
+
 // Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Without a
 // treelet (kTreelet false) every record comes from global memory and no LDS path is compiled.
 template <bool kAnyHit, bool kCount, bool kTreelet>
@@ -794,10 +961,11 @@ __device__ __forceinline__ void persistent_trace(
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
 	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
-	if (kTreelet) {
+	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
+	if (kTreelet)
 		for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
-		__syncthreads();
-	}
+	handoff_init(handoff);
+	__syncthreads();
 
 	LaneStack st;
 	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
@@ -826,13 +994,18 @@ __device__ __forceinline__ void persistent_trace(
 		next = pipe_step<kAnyHit, kCount, kTreelet, true>(P, L, st, q, B, A, k);
 	}
 	if (L.doneIdx != kInvalid) store_result<kAnyHit>(P, L.doneIdx, L.c);
-	if (!kCount && next == kStepSolo) {
-		// The wave's last rays, one after the other (the loads still in flight land in A, which
-		// is not read again).
+	if (!kCount) {
+		// Solo: keep the first busy lane's ray, offer the others to the block, then trace offered
+		// rays until none is left (the loads still in flight land in A, which is not read again).
 		const SoloScene S = {nodes, tris, rays,
 		                     kAnyHit ? static_cast<void*>(inLight) : static_cast<void*>(hits)};
-		for (uint64_t busy = __ballot(L.kind != kRecIdle); busy != 0; busy &= busy - 1u)
-			solo_trace<kAnyHit>(S, L, st, uint32_t(__builtin_ctzll(busy)));
+		const uint64_t busy = next == kStepSolo ? __ballot(L.kind != kRecIdle) : 0ull;
+		handoff_offer(handoff, busy & (busy - 1u), L, st, 0u);
+		if (busy != 0) solo_run<kAnyHit>(S, solo_start_lane(L, st, uint32_t(__builtin_ctzll(busy))));
+		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
+			uint32_t batch;
+			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, batch));
+		}
 	}
 	if (kCount) add_counts(counters, k);
 	retire_wave(work);
@@ -1047,7 +1220,9 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 	extern __shared__ float4 ldsRaw[];
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
 	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
-	unsigned long long* blockFinish = reinterpret_cast<unsigned long long*>(treelet + 4u * topCount);
+	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
+	unsigned long long* blockFinish =
+		reinterpret_cast<unsigned long long*>(handoff + handoff_bytes(blockDim.x) / 4u);
 	uint32_t* blockDone = reinterpret_cast<uint32_t*>(blockFinish + (kTrack ? numBatches : 0u));
 	uint32_t* offsets = blockDone + (kTrack ? numBatches : 0u);
 	if (kTrack && threadIdx.x == 0) atomicMin(&clock->start, wall_clock64());
@@ -1058,6 +1233,7 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 			blockFinish[i] = 0ull;
 			blockDone[i] = 0u;
 		}
+	handoff_init(handoff);
 	__syncthreads();
 
 	LaneStack st;
@@ -1094,18 +1270,45 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 		pipe_step_multi<kTreelet, kTrack, false>(P, M, st, q, A, B);
 		next = pipe_step_multi<kTreelet, kTrack, true>(P, M, st, q, B, A);
 	}
-	if (next == kStepSolo) {
-		for (uint64_t busy = __ballot(L.kind != kRecIdle); busy != 0; busy &= busy - 1u) {
-			const uint32_t lsel = uint32_t(__builtin_ctzll(busy));
-			const uint32_t b = lane_u(M.batch, lsel);
+	{
+		// Solo: keep the first busy lane's ray, offer the others to the block, then trace offered
+		// rays until none is left.
+		const uint64_t busy = next == kStepSolo ? __ballot(L.kind != kRecIdle) : 0ull;
+		handoff_offer(handoff, busy & (busy - 1u), L, st, M.batch);
+		const uint32_t lsel = busy != 0 ? uint32_t(__builtin_ctzll(busy)) : 0u;
+		uint32_t slot = kInvalid;
+		bool have = busy != 0;
+		while (have) {
+			uint32_t b;
+			SoloStart S0;
+			if (slot == kInvalid) {
+				b = lane_u(M.batch, lsel);
+				S0 = solo_start_lane(L, st, lsel);
+			} else {
+				S0 = solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, b);
+			}
 			const BatchDesc d = descs[b];
 			const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
 			if (d.kind != 0u)
-				solo_trace<true>(S, L, st, lsel);
+				solo_run<true>(S, S0);
 			else
-				solo_trace<false>(S, L, st, lsel);
+				solo_run<false>(S, S0);
 			if (kTrack && __lane_id() == 0u) track_done(P, b);
+			slot = handoff_claim(handoff);
+			have = slot != kInvalid;
 		}
+		if (busy == 0)
+			for (slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
+				uint32_t b;
+				const SoloStart S0 = solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, b);
+				const BatchDesc d = descs[b];
+				const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
+				if (d.kind != 0u)
+					solo_run<true>(S, S0);
+				else
+					solo_run<false>(S, S0);
+				if (kTrack && __lane_id() == 0u) track_done(P, b);
+			}
 	}
 	if (kTrack) {
 		__syncthreads();
