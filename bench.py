@@ -403,8 +403,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic() if kernel == "persistent" else None,
-                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 768x256 "
-                                       "no treelet, refill 8 (profiles/closest_persistent_bounce1M.md)",
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 256x512 "
+                                       "no treelet, refill 32 (profiles/closest_persistent_bounce1M.md)",
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_batched:

@@ -201,8 +201,8 @@ uint32_t phx_context_num_cus(PhxContext* ctx);
  * closest-hit and any-hit, block size (threads, a multiple of 64 up to 1024), the number of top
  * tree nodes each block keeps in LDS (0..2048; the upload lays out the top of the tree breadth
  * first), and the refill threshold (a wave refills its idle lanes once at least `refillMin` of its
- * 64 lanes are idle; 1..64). phx_create picks defaults from the device's occupancy; phx_autotune
- * picks them per scene and ray mix. In phx_set_launch_params a zero field (0xFFFFFFFF for
+ * 64 lanes are idle; 1..64). phx_create's defaults: one block of 512 threads per CU, no treelet,
+ * refill 8; phx_autotune picks them per scene and ray mix. In phx_set_launch_params a zero field (0xFFFFFFFF for
  * topNodes) keeps the current value; combinations that do not fit a CU are rejected. Results do
  * not depend on these parameters. */
 typedef struct PhxLaunchParams {

@@ -242,11 +242,14 @@ phx_status phx_create(int device, PhxContext** outCtx)
 		delete ctx;
 		return hipFail(e, "hipStreamCreate");
 	}
+	// Default shape: one block of 512 threads per CU, no treelet, refill at 8 idle lanes -- the
+	// fastest or close to it for incoherent closest-hit and any-hit casts on MI355X;
+	// phx_autotune refines it per scene and ray mix.
 	uint32_t bpc = 0;
 	e = persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, &bpc);
-	if (bpc == 0) bpc = 1;
-	ctx->gridClosest = ctx->numCUs * bpc;
-	ctx->gridAnyhit = ctx->numCUs * bpc;
+	if (e == hipSuccess && bpc == 0) e = hipErrorInvalidConfiguration;
+	ctx->gridClosest = ctx->numCUs;
+	ctx->gridAnyhit = ctx->numCUs;
 	// The spill area covers every lane the device can hold (2048 per CU).
 	ctx->spillLanes = ctx->numCUs * 2048u;
 	if (e == hipSuccess) e = hipMalloc(&ctx->work, 64);

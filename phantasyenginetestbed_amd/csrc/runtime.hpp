@@ -67,8 +67,8 @@ struct PhxContext {
 	uint32_t* spill = nullptr;
 	uint32_t gridClosest = 0;
 	uint32_t gridAnyhit = 0;
-	uint32_t blockThreads = 256;
-	uint32_t topNodes = 1024;
+	uint32_t blockThreads = 512;
+	uint32_t topNodes = 0;
 	uint32_t refillMin = 8;
 	uint32_t spillLanes = 0;
 	// Staging buffers for calls that pass host pointers, grown on demand.

@@ -491,7 +491,7 @@ constexpr int kStepGo = 0, kStepDone = 1, kStepSolo = 2;
 
 // Rays left in a wave, once the ray queue is empty, at or below which the wave traces them one at
 // a time with all 64 lanes (solo_trace) instead of one lane each.
-constexpr uint32_t kSoloRays = 4;
+constexpr uint32_t kSoloRays = 6;
 
 // One iteration: consume record A, issue the load of the lane's next record into B, test A's
 // triangles, retire finished rays and refill lanes. Every path a lone lane can take is kept short

@@ -358,3 +358,50 @@ def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
     assert bad == 0, f"{bad} of {tested} reciprocals differ"
     tested, bad = ctx.check_reciprocal()
     assert tested == 2 * 4 * (1 << 23) and bad == 0
+
+
+@pytest.mark.parametrize("shape", [(512, 256, 512, 8), (300, 256, 1024, 32), (128, 1024, 0, 16)],
+                         ids=["b256-treelet512", "b256-treelet1024", "b1024-notreelet"])
+def test_launch_shapes_match_oracle(ctx, c2, shape):
+    """Other launch shapes -- the LDS-treelet kernels, larger blocks -- give the oracle's hits,
+    single-batch and fused."""
+    grid, block, top, refill = shape
+    say(f"[gpu] launch shape grid {grid} block {block} treelet {top} refill {refill}")
+    b2, s2 = c2
+    rng = np.random.default_rng(13)
+    n = 40000
+    rays = np.zeros(n, phx.RAY)
+    rays["o"] = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmin"] = np.float32(1e-4)
+    rays["tmax"] = np.float32(phx.FLT_MAX)
+    short = rays.copy()
+    short["tmax"] = np.float32(0.3)
+    ref = O.cast_closest(b2.nodes, b2.tris, rays)
+    ref_lit = O.cast_anyhit(b2.nodes, b2.tris, short)
+    old = ctx.launch_params()
+    bufs = []
+    try:
+        ctx.set_launch_params(grid_closest=grid, grid_anyhit=grid, refill_min=refill,
+                              block_threads=block, top_nodes=top)
+        assert_same_hits(ctx.cast_closest(s2, rays), ref, f"closest, shape {shape}")
+        assert (ctx.cast_anyhit(s2, short) == ref_lit).all()
+        rb, sb = ctx.alloc(n * 32).upload(rays), ctx.alloc(n * 32).upload(short)
+        hb, lb = ctx.alloc(n * 16), ctx.alloc(n)
+        bufs = [rb, sb, hb, lb]
+        plan = phx.Plan(ctx, [(s2, rb.ptr, hb.ptr, n, phx.BATCH_CLOSEST),
+                              (s2, sb.ptr, lb.ptr, n, phx.BATCH_ANYHIT)])
+        try:
+            plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill)
+            plan.launch()
+            assert_same_hits(hb.download(phx.HIT, n), ref, f"fused, shape {shape}")
+            assert (lb.download(np.uint8, n) == ref_lit).all()
+        finally:
+            plan.release()
+    finally:
+        for b in bufs:
+            b.free()
+        ctx.set_launch_params(grid_closest=old["gridBlocksClosest"],
+                              grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
+                              block_threads=old["blockThreads"], top_nodes=old["topNodes"])
