@@ -382,6 +382,7 @@ struct PipeArgs {
 	uint32_t topCount;
 	bool finiteBounds;
 	const float4* treelet;  // LDS copy of nodes [0, topCount)
+	const uint32_t* handoff;  // the block's handoff area (LDS)
 };
 
 // Loop-carried state of one lane.
@@ -489,9 +490,13 @@ __device__ __forceinline__ void expand_pipe(const NodeWords& nd, bool visitL, bo
 // What the wave does after a step: go on, stop (no ray left), or trace its last few rays solo.
 constexpr int kStepGo = 0, kStepDone = 1, kStepSolo = 2;
 
-// Rays left in a wave, once the ray queue is empty, at or below which the wave traces them one at
-// a time with all 64 lanes (solo_trace) instead of one lane each.
+// Rays left in a wave, once the ray queue is empty, at or below which the wave goes solo: it traces
+// them one at a time with all 64 lanes instead of one lane each, keeping one and offering the
+// others to the block's other waves (below). While waves of the block are idle, the threshold is
+// one more than their number, up to kMaxSoloRays, so each idle wave can take a ray at once.
 constexpr uint32_t kSoloRays = 6;
+constexpr uint32_t kMaxSoloRays = 16;
+__device__ __forceinline__ uint32_t solo_threshold(const uint32_t* handoff);
 
 // One iteration: consume record A, issue the load of the lane's next record into B, test A's
 // triangles, retire finished rays and refill lanes. Every path a lone lane can take is kept short
@@ -595,7 +600,7 @@ __device__ __forceinline__ int pipe_step(const PipeArgs& P, Tracer& L, LaneStack
 	if (!kExitCheck || !q.exhausted) return kStepGo;
 	const uint64_t busy = __ballot(nextKind != kRecIdle);
 	if (busy == 0) return kStepDone;
-	return !kCount && uint32_t(__popcll(busy)) <= kSoloRays ? kStepSolo : kStepGo;
+	return !kCount && uint32_t(__popcll(busy)) <= solo_threshold(P.handoff) ? kStepSolo : kStepGo;
 }
 
 // ---- solo mode: one ray per wave ---------------------------------------------------------------
@@ -818,12 +823,13 @@ __device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 // have nothing left claim them and trace them solo, so a wave's last rays run side by side
 // instead of one after another. The stack of an offered ray stays where it is (its owner's LDS
 // column and spill column, which the owner no longer writes). Area layout (32-bit words):
-// [0] slots allocated, [1] slots claimed, [2] waves that may still offer, [3] unused, then
+// [0] slots allocated, [1] slots claimed, [2] waves that may still offer, [3] waves waiting to
+// claim, then
 // kSlotWords per slot. A claimed slot index is below the allocated count, and its owner marks it
 // ready right after writing it, so a claimant's wait ends; the last claim attempts end once no
 // wave may offer and every slot is claimed.
 
-constexpr uint32_t kHandoffPerWave = kSoloRays - 1u;  // a wave offers at most this many rays
+constexpr uint32_t kHandoffPerWave = kMaxSoloRays - 1u;  // a wave offers at most this many rays
 constexpr uint32_t kSlotWords = 24;
 
 __host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t blockThreads)
@@ -834,6 +840,13 @@ __host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t blockThreads
 __device__ __forceinline__ uint32_t lds_load(const uint32_t* p)
 {
 	return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The solo threshold of a wave of the block (see kSoloRays).
+__device__ __forceinline__ uint32_t solo_threshold(const uint32_t* H)
+{
+	const uint32_t idle = __builtin_amdgcn_readfirstlane(lds_load(&H[3]));
+	return idle + 1u > kSoloRays ? (idle + 1u < kMaxSoloRays ? idle + 1u : kMaxSoloRays) : kSoloRays;
 }
 
 // Every thread of the block calls this before the block's first barrier: resets the counters and
@@ -900,6 +913,7 @@ __device__ __forceinline__ uint32_t handoff_claim(uint32_t* H)
 {
 	uint32_t got = kInvalid;
 	if (__lane_id() == 0u) {
+		atomicAdd(&H[3], 1u);
 		while (true) {
 			const uint32_t c = lds_load(&H[1]);
 			if (c < lds_load(&H[0])) {
@@ -912,6 +926,7 @@ __device__ __forceinline__ uint32_t handoff_claim(uint32_t* H)
 			if (lds_load(&H[2]) == 0u && lds_load(&H[1]) >= lds_load(&H[0])) break;
 			__builtin_amdgcn_s_sleep(2);
 		}
+		atomicSub(&H[3], 1u);
 	}
 	return lane_u(got, 0u);
 }
@@ -975,7 +990,7 @@ __device__ __forceinline__ void persistent_trace(
 	st.sp = 0;
 
 	const PipeArgs P = {nodes, tris, rays, hits, inLight, n, work, refillMin,
-	                    kTreelet ? topCount : 0u, finiteBounds, treelet};
+	                    kTreelet ? topCount : 0u, finiteBounds, treelet, handoff};
 	WaveQueue q;
 	Tracer L;
 	L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
@@ -1074,6 +1089,7 @@ struct MultiArgs {
 	const float4* treelet;       // LDS
 	unsigned long long* blockFinish;  // LDS, per batch (completion tracking)
 	uint32_t* blockDone;              // LDS, per batch
+	const uint32_t* handoff;          // the block's handoff area (LDS)
 };
 
 // Per-lane state of the fused kernel: the pipelined loop's, plus the scene, output and kind of
@@ -1205,7 +1221,7 @@ __device__ __forceinline__ int pipe_step_multi(const MultiArgs& P, MultiTracer& 
 	if (!kExitCheck || !q.exhausted) return kStepGo;
 	const uint64_t busy = __ballot(nextKind != kRecIdle);
 	if (busy == 0) return kStepDone;
-	return uint32_t(__popcll(busy)) <= kSoloRays ? kStepSolo : kStepGo;
+	return uint32_t(__popcll(busy)) <= solo_threshold(P.handoff) ? kStepSolo : kStepGo;
 }
 
 template <bool kTreelet, bool kTrack>
@@ -1244,7 +1260,8 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 	st.sp = 0;
 
 	const MultiArgs P = {descs, offsets, numBatches, total, work, refillMin,
-	                     kTreelet ? topCount : 0u, treeletNodes, treelet, blockFinish, blockDone};
+	                     kTreelet ? topCount : 0u, treeletNodes, treelet, blockFinish, blockDone,
+	                     handoff};
 	WaveQueue q;
 	MultiTracer M;
 	Tracer& L = M.t;
