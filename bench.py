@@ -201,27 +201,37 @@ def batched_form(ctx, scene, pools, ev, reps: int = 3, plan_launch=None) -> dict
 
 
 def frame_form(ctx, scene, ev, lights: int, frames: int = 16) -> dict:
-    """The wavefront frame at 1920x1080 with TAA over 16 Halton-jittered frames."""
+    """The wavefront frame at 1920x1080 with TAA over 16 Halton-jittered frames, replayed from the
+    captured hipGraph (steady frames), beside the same frames launched directly; identical
+    outputs."""
     from phantasyenginetestbed_amd import phx, workloads as W
     w, h = 1920, 1080
     cam = W.camera(w, h)
-    r = ctx.renderer(w, h)
-    out = ctx.alloc(w * h * 16)
-    flags = phx.FRAME_TAA
-    for f in range(2):  # warm-up (allocates the per-light buffers)
-        r.render(scene, cam, f, 3, flags | phx.FRAME_TAA_RESET, out_ptr=out.ptr)
-    evs = ev.create(frames + 1)
-    ev.record(evs[0], ctx.stream)
-    for f in range(frames):
-        r.render(scene, cam, f, 3, flags | (phx.FRAME_TAA_RESET if f == 0 else 0),
-                 out_ptr=out.ptr, asynchronous=True)
-        ev.record(evs[f + 1], ctx.stream)
-    ctx.synchronize()
-    per = np.array([ev.elapsed_ms(evs[i], evs[i + 1]) for i in range(frames)])
-    ev.destroy(evs)
-    hist = r.download(phx.BUFFER_HISTORY, np.float32).reshape(h, w, 4)
-    r.release()
-    out.free()
+    outs = {}
+
+    def run(extra):
+        r = ctx.renderer(w, h)
+        out = ctx.alloc(w * h * 16)
+        flags = phx.FRAME_TAA | extra
+        for f in range(2):  # warm-up (allocates the per-light buffers, captures the graph)
+            r.render(scene, cam, f, 3, flags | phx.FRAME_TAA_RESET, out_ptr=out.ptr)
+        evs = ev.create(frames + 1)
+        ev.record(evs[0], ctx.stream)
+        for f in range(frames):
+            r.render(scene, cam, f, 3, flags | (phx.FRAME_TAA_RESET if f == 0 else 0),
+                     out_ptr=out.ptr, asynchronous=True)
+            ev.record(evs[f + 1], ctx.stream)
+        ctx.synchronize()
+        per = np.array([ev.elapsed_ms(evs[i], evs[i + 1]) for i in range(frames)])
+        ev.destroy(evs)
+        hist = r.download(phx.BUFFER_HISTORY, np.float32).reshape(h, w, 4)
+        outs[extra] = out.download(np.float32, w * h * 4)
+        r.release()
+        out.free()
+        return per, hist
+
+    direct, _ = run(0)
+    per, hist = run(phx.FRAME_GRAPH)
     n, m = w * h, (w // 2) * (h // 2)
     rays = n + n * lights + m + m * lights
     ms = float(per.mean())
@@ -229,6 +239,8 @@ def frame_form(ctx, scene, ev, lights: int, frames: int = 16) -> dict:
             "ms_per_frame": round(ms, 3), "p50_ms": round(float(np.percentile(per, 50)), 3),
             "p99_ms": round(float(np.percentile(per, 99)), 3),
             "rays_per_frame": int(rays), "mrays": round(rays / ms / 1e3, 1),
+            "replay": "hipGraph", "direct_ms_per_frame": round(float(direct.mean()), 3),
+            "identical_outputs": bool(outs[0].tobytes() == outs[phx.FRAME_GRAPH].tobytes()),
             "taa_samples_after": float(hist[..., 3].min())}
 
 

@@ -480,6 +480,7 @@ typedef struct PhxRenderer PhxRenderer;
 #define PHX_FRAME_TAA 0x10000u       /* jitter by Halton(2,3)[frame % 16], resolve into history  */
 #define PHX_FRAME_TAA_RESET 0x20000u /* start the TAA history at this frame                      */
 #define PHX_FRAME_NO_CLAMP 0x40000u  /* TAA without the 3x3 neighbourhood clamp of the history   */
+#define PHX_FRAME_GRAPH 0x80000u     /* replay the frame's launches as a captured hipGraph         */
 
 /* Frame buffers of one resolution (targetResolutionUpdated, CudaTracerRenderer.cpp:735-793).
  * Per-light buffers grow on the first frame that needs them. */
@@ -496,7 +497,11 @@ void phx_renderer_release(PhxRenderer* renderer);
  * camera, FP32 history, 3x3 clamp). The primary shadow rays and the bounce rays are traced in one
  * fused persistent launch. `rgba` (width*height*4 floats, row-major; host or device with
  * PHX_MEM_DEVICE; may be NULL) receives the output. Random numbers: PCG32 streams keyed by
- * (seed, frameIndex, pass, record). */
+ * (seed, frameIndex, pass, record). The frame's kernels read the per-frame values (jitter, frame
+ * index, seed, TAA reset and history buffer) from a small device block written before the frame,
+ * so with PHX_FRAME_GRAPH the launches are captured once as a hipGraph -- again whenever the
+ * scene, camera, PHX_FRAME_TAA / PHX_FRAME_NO_CLAMP or the context's launch shape change -- and
+ * replayed for the following frames; the output is the same as without it. */
 phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* renderer, const PhxScene* scene,
                             const PhxCameraDef* cam, uint32_t frameIndex, uint64_t seed,
                             uint32_t flags, float* rgba);

@@ -106,13 +106,15 @@ __global__ __launch_bounds__(kBlock1D) void k_shade(const float4* __restrict__ i
                                                     float cx, float cy, float cz,
                                                     const PhxSphereLight* __restrict__ lights,
                                                     uint32_t numLights, uint32_t count,
-                                                    uint64_t seed, uint32_t frame,
+                                                    const FrameParams* __restrict__ fp,
                                                     uint32_t purpose,
                                                     float4* __restrict__ contrib,
                                                     float4* __restrict__ shadow)
 {
 	const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
 	if (id >= count) return;
+	const uint64_t seed = fp->seed;
+	const uint32_t frame = fp->frame;
 	const float4 d1 = info[3 * size_t(id)];
 	const float4 d2 = info[3 * size_t(id) + 1];
 	const float4 d3 = info[3 * size_t(id) + 2];
@@ -174,8 +176,8 @@ __global__ __launch_bounds__(kBlock1D) void k_shade(const float4* __restrict__ i
 // path. A block pixel without a hit gets the dummy ray and zero throughput.
 __global__ __launch_bounds__(kBlock1D) void k_gen_secondary(const float4* __restrict__ gbuf,
                                                             uint32_t w, uint32_t h, float cx,
-                                                            float cy, float cz, uint64_t seed,
-                                                            uint32_t frame,
+                                                            float cy, float cz,
+                                                            const FrameParams* __restrict__ fp,
                                                             float4* __restrict__ rays,
                                                             float4* __restrict__ throughput)
 {
@@ -183,6 +185,8 @@ __global__ __launch_bounds__(kBlock1D) void k_gen_secondary(const float4* __rest
 	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
 	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
 	if (x >= hw || y >= hh) return;
+	const uint64_t seed = fp->seed;
+	const uint32_t frame = fp->frame;
 	const uint32_t hid = pixel_id(x, y, hw, hh);
 	Pcg32 rng = frame_rng(seed, frame, 2u, hid);
 	const uint32_t pix = rng.next() % 4u;
@@ -257,16 +261,18 @@ __global__ __launch_bounds__(kBlock1D) void k_shadow_logic(float4* __restrict__ 
 // same pixel), FP32 history with the effective sample count in alpha, and the history colour
 // clamped to the current frame's 3x3 neighbourhood min/max (clampHistory).
 __global__ __launch_bounds__(kBlock1D) void k_taa(const float4* __restrict__ cur,
-                                                  const float4* __restrict__ histIn,
+                                                  float4* hist0, float4* hist1,
                                                   const float4* __restrict__ gbuf, uint32_t w,
-                                                  uint32_t h, uint32_t reset,
-                                                  uint32_t clampHistory,
-                                                  float4* __restrict__ histOut,
+                                                  uint32_t h, uint32_t clampHistory,
+                                                  const FrameParams* __restrict__ fp,
                                                   float4* __restrict__ out)
 {
 	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
 	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
 	if (x >= w || y >= h) return;
+	const uint32_t reset = fp->taaReset;
+	const float4* histIn = fp->histIn ? hist1 : hist0;
+	float4* histOut = fp->histIn ? hist0 : hist1;
 	const size_t p = size_t(y) * w + x;
 	const float4 c = cur[p];
 	float o0, o1, o2, n2;
@@ -328,25 +334,25 @@ hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t
 
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
                        const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
-                       uint32_t count, uint64_t seed, uint32_t frame, uint32_t purpose,
-                       float4* contrib, PhxRayIn* shadow, hipStream_t stream)
+                       uint32_t count, const FrameParams* fp, uint32_t purpose, float4* contrib,
+                       PhxRayIn* shadow, hipStream_t stream)
 {
 	if (count == 0 || numLights == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_shade, dim3(blocks1d(count)), dim3(kBlock1D), 0, stream, info,
 	                   reinterpret_cast<const float4*>(rays), throughput, camPos[0], camPos[1],
-	                   camPos[2], lights, numLights, count, seed, frame, purpose, contrib,
+	                   camPos[2], lights, numLights, count, fp, purpose, contrib,
 	                   reinterpret_cast<float4*>(shadow));
 	return hipGetLastError();
 }
 
 hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const float camPos[3],
-                              uint64_t seed, uint32_t frame, PhxRayIn* rays, float4* throughput,
+                              const FrameParams* fp, PhxRayIn* rays, float4* throughput,
                               hipStream_t stream)
 {
 	if (w < 2u || h < 2u) return hipSuccess;
 	hipLaunchKernelGGL(k_gen_secondary, grid2d(w / 2u, h / 2u), dim3(kBlock1D), 0, stream, gbuf, w,
-	                   h, camPos[0], camPos[1], camPos[2], seed, frame,
-	                   reinterpret_cast<float4*>(rays), throughput);
+	                   h, camPos[0], camPos[1], camPos[2], fp, reinterpret_cast<float4*>(rays),
+	                   throughput);
 	return hipGetLastError();
 }
 
@@ -360,13 +366,26 @@ hipError_t launchShadowLogic(float4* img, uint32_t w, uint32_t h, uint32_t scale
 	return hipGetLastError();
 }
 
-hipError_t launchTaa(const float4* cur, const float4* histIn, const float4* gbuf, uint32_t w,
-                     uint32_t h, bool reset, bool clampHistory, float4* histOut, float4* out,
+hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, const float4* gbuf,
+                     uint32_t w, uint32_t h, bool clampHistory, const FrameParams* fp, float4* out,
                      hipStream_t stream)
 {
 	if (w == 0 || h == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_taa, grid2d(w, h), dim3(kBlock1D), 0, stream, cur, histIn, gbuf, w, h,
-	                   uint32_t(reset), uint32_t(clampHistory), histOut, out);
+	hipLaunchKernelGGL(k_taa, grid2d(w, h), dim3(kBlock1D), 0, stream, cur, hist0, hist1, gbuf, w,
+	                   h, uint32_t(clampHistory), fp, out);
+	return hipGetLastError();
+}
+
+namespace {
+__global__ void k_set_frame_params(FrameParams* dst, FrameParams value)
+{
+	if (threadIdx.x == 0) *dst = value;  // per-lane condition: an ordinary vector store
+}
+}  // namespace
+
+hipError_t launchSetFrameParams(FrameParams* dst, const FrameParams& value, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_set_frame_params, dim3(1), dim3(1), 0, stream, dst, value);
 	return hipGetLastError();
 }
 

@@ -91,28 +91,46 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
                          uint32_t total, const TraceLaunch& launch, const float4* treeletNodes,
                          uint32_t treeletCount, BatchClock* clock, hipStream_t stream);
 
-// Pinhole primary rays.
+// Per-frame values of the wavefront frame. Its kernels read them from device memory, so the
+// frame's launches are the same for every frame and can be captured once and replayed as a
+// hipGraph with only this block rewritten (by launchSetFrameParams, before the frame).
+struct FrameParams {
+	float jx, jy;       // sub-pixel jitter of the primary rays
+	uint32_t frame;     // frame index (random-number streams)
+	uint32_t taaReset;  // TAA: start the history afresh
+	uint64_t seed;
+	uint32_t histIn;    // TAA: history buffer read (0 or 1); the other one is written
+	uint32_t pad;
+};
+static_assert(sizeof(FrameParams) == 32, "FrameParams layout");
+
+// Writes `value` to *dst on the stream (a one-thread kernel: its argument is copied at launch).
+hipError_t launchSetFrameParams(FrameParams* dst, const FrameParams& value, hipStream_t stream);
+
+// Pinhole primary rays. With `fp` the jitter is read from fp->jx, fp->jy instead of jx, jy.
 hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float jx,
                                 float jy, float tMin, float tMax, uint32_t order, PhxRayIn* rays,
-                                hipStream_t stream);
+                                hipStream_t stream, const FrameParams* fp = nullptr);
 
 // Wavefront frame kernels (frame_kernels.hip). Hit records are 3 float4 each:
 // (position, roughness) (normal, metallic) (albedo, alpha; alpha < 0 = no hit).
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
                            const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
                            float4* info, hipStream_t stream);
+// The seed and frame index of the random-number streams come from fp.
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
                        const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
-                       uint32_t count, uint64_t seed, uint32_t frame, uint32_t purpose,
-                       float4* contrib, PhxRayIn* shadow, hipStream_t stream);
+                       uint32_t count, const FrameParams* fp, uint32_t purpose, float4* contrib,
+                       PhxRayIn* shadow, hipStream_t stream);
 hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const float camPos[3],
-                              uint64_t seed, uint32_t frame, PhxRayIn* rays, float4* throughput,
+                              const FrameParams* fp, PhxRayIn* rays, float4* throughput,
                               hipStream_t stream);
 hipError_t launchShadowLogic(float4* img, uint32_t w, uint32_t h, uint32_t scale, bool addTo,
                              const uint8_t* inLight, const float4* contrib, uint32_t numLights,
                              hipStream_t stream);
-hipError_t launchTaa(const float4* cur, const float4* histIn, const float4* gbuf, uint32_t w,
-                     uint32_t h, bool reset, bool clampHistory, float4* histOut, float4* out,
+// TAA resolve: reads history hist[fp->histIn] (ignored if fp->taaReset), writes the other one.
+hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, const float4* gbuf,
+                     uint32_t w, uint32_t h, bool clampHistory, const FrameParams* fp, float4* out,
                      hipStream_t stream);
 
 }  // namespace phx

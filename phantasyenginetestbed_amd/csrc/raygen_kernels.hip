@@ -17,11 +17,16 @@ namespace {
 // A jitter (jx, jy) in pixels moves the sample to (x + jx, y + jy); (0, 0) is the formula above.
 __global__ __launch_bounds__(256) void k_gen_primary(PhxCameraDef cam, uint32_t w, uint32_t h,
                                                      float jx, float jy, float tMin, float tMax,
-                                                     uint32_t order, float4* __restrict__ rays)
+                                                     uint32_t order, float4* __restrict__ rays,
+                                                     const FrameParams* __restrict__ fp)
 {
 	const uint32_t x = blockIdx.x * 16u + (threadIdx.x & 15u);
 	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
 	if (x >= w || y >= h) return;
+	if (fp) {
+		jx = fp->jx;
+		jy = fp->jy;
+	}
 	const float cx = ((float(x) + jx) / float(w)) * 2.0f - 1.0f;
 	const float cy = ((float(y) + jy) / float(h)) * 2.0f - 1.0f;
 	float d[3];
@@ -124,11 +129,11 @@ hipError_t launchGenBounceRays(const PhxRayIn* rays, const PhxRayHit* hits,
 
 hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float jx,
                                 float jy, float tMin, float tMax, uint32_t order, PhxRayIn* rays,
-                                hipStream_t stream)
+                                hipStream_t stream, const FrameParams* fp)
 {
 	dim3 grid((w + 15u) / 16u, (h + 15u) / 16u);
 	hipLaunchKernelGGL(k_gen_primary, grid, dim3(256), 0, stream, cam, w, h, jx, jy, tMin, tMax,
-	                   order, reinterpret_cast<float4*>(rays));
+	                   order, reinterpret_cast<float4*>(rays), fp);
 	return hipGetLastError();
 }
 

@@ -36,16 +36,38 @@ struct PhxRenderer {
 	uint32_t hostOff[3] = {0u, 0u, 0u};
 	bool descsValid = false;
 	float4* lastOut = nullptr;
+	FrameParams* params = nullptr;  // per-frame values the frame kernels read (device)
+	// The captured frame (PHX_FRAME_GRAPH) and what it was captured for.
+	struct GraphKey {
+		const PhxScene* scene;
+		PhxCameraDef cam;
+		uint32_t flags;  // PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP
+		uint32_t numLights;
+		const void* lightBuffers;
+		uint32_t launch[5];  // the context's launch shape
+	};
+	GraphKey key{};
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graphExec = nullptr;
 };
 
 namespace {
 
+void dropGraph(PhxRenderer* r)
+{
+	if (r->graphExec) hipGraphExecDestroy(r->graphExec);
+	if (r->graph) hipGraphDestroy(r->graph);
+	r->graphExec = nullptr;
+	r->graph = nullptr;
+}
+
 void freeRenderer(PhxRenderer* r)
 {
+	dropGraph(r);
 	void* bufs[] = {r->prim,    r->primHits, r->gbuf,   r->bounce,  r->throughput,
 	                r->bounceHits, r->bounceInfo, r->contrib, r->shadow, r->inLight,
 	                r->color,   r->hist[0],  r->hist[1], r->out,    r->descs,
-	                r->offsets};
+	                r->offsets, r->params};
 	for (void* b : bufs) hipFree(b);
 	delete r;
 }
@@ -67,6 +89,7 @@ phx_status growLights(PhxRenderer* r, uint32_t lights)
 {
 	if (lights <= r->lightCap) return PHX_OK;
 	hipStreamSynchronize(r->ctx->stream);  // earlier frames may still use the buffers
+	dropGraph(r);                           // a captured frame uses them too
 	hipFree(r->contrib);
 	hipFree(r->shadow);
 	hipFree(r->inLight);
@@ -80,6 +103,94 @@ phx_status growLights(PhxRenderer* r, uint32_t lights)
 	if (st == PHX_OK) st = devAlloc(r->inLight, k, "phx_render_frame");
 	if (st == PHX_OK) r->lightCap = lights;
 	return st;
+}
+
+// The frame's launches, all on stream s; per-frame values come from r->params, so this sequence is
+// the same for every frame of a scene, camera and flags (and can be captured as a graph).
+phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
+                        const PhxCameraDef* cam, uint32_t flags, hipStream_t s)
+{
+	const uint32_t L = scene->numLights;
+	const bool taa = (flags & PHX_FRAME_TAA) != 0;
+	const float camPos[3] = {cam->origin[0], cam->origin[1], cam->origin[2]};
+	const uint32_t order = (r->w % 8u == 0u && r->h % 8u == 0u) ? PHX_RAYS_TILED_8X8 : PHX_RAYS_ROW_MAJOR;
+	const SceneView sv = view(scene);
+	const TraceLaunch closest = traceLaunch(ctx, 0, false);
+	const TraceLaunch anyhit = traceLaunch(ctx, 0, true);
+	const char* what = "phx_render_frame";
+
+	// Primary hits -> hit records -> contributions and shadow rays of every light.
+	PHX_HIP(launchGenPrimaryRays(*cam, r->w, r->h, 0.0f, 0.0f, 1e-4f, FLT_MAX, order, r->prim, s,
+	                             r->params),
+	        what);
+	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, closest, nullptr, s), what);
+	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->materials,
+	                        scene->numMaterials, r->gbuf, s),
+	        what);
+	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, r->params, 1u,
+	                    r->contrib, r->shadow, s),
+	        what);
+	// Bounce rays (they need only the hit records), traced together with the primary shadow rays
+	// in one fused persistent launch.
+	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, camPos, r->params, r->bounce, r->throughput,
+	                           s),
+	        what);
+	{
+		const uint32_t total = r->n * L + r->m;
+		PHX_HIP(launchBatches(r->descs, r->offsets, 2u, total, closest,
+		                      reinterpret_cast<const float4*>(scene->nodes), scene->topCount,
+		                      nullptr, s),
+		        what);
+	}
+	PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 1u, false, r->inLight, r->contrib, L, s), what);
+
+	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
+	if (r->m) {
+		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->materials,
+		                        scene->numMaterials, r->bounceInfo, s),
+		        what);
+		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
+		                    r->m, r->params, 3u, r->contrib, r->shadow, s),
+		        what);
+		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, anyhit, s), what);
+		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
+		        what);
+	}
+
+	if (taa)
+		PHX_HIP(launchTaa(r->color, r->hist[0], r->hist[1], r->gbuf, r->w, r->h,
+		                  !(flags & PHX_FRAME_NO_CLAMP), r->params, r->out, s),
+		        what);
+	return PHX_OK;
+}
+
+// The two batch descriptors of the fused shadow + bounce launch, uploaded when they change (before
+// any capture: the copies read the renderer's host copies, which change only after the stream has
+// drained the frames that used them).
+phx_status uploadDescs(PhxRenderer* r, const PhxScene* scene)
+{
+	hipStream_t s = r->ctx->stream;
+	const char* what = "phx_render_frame";
+	const uint32_t L = scene->numLights;
+	BatchDesc d[2];
+	const float4* nodes = reinterpret_cast<const float4*>(scene->nodes);
+	const float4* tris = reinterpret_cast<const float4*>(scene->tris);
+	d[0] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->shadow), r->inLight, r->n * L,
+	                 1u, scene->finiteBounds, 0u};
+	d[1] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->bounce), r->bounceHits, r->m,
+	                 0u, scene->finiteBounds, 0u};
+	const uint32_t off[3] = {0u, r->n * L, r->n * L + r->m};
+	if (!r->descsValid || std::memcmp(d, r->hostDescs, sizeof(d)) != 0 ||
+	    std::memcmp(off, r->hostOff, sizeof(off)) != 0) {
+		PHX_HIP(hipStreamSynchronize(s), what);
+		std::memcpy(r->hostDescs, d, sizeof(d));
+		std::memcpy(r->hostOff, off, sizeof(off));
+		PHX_HIP(hipMemcpyAsync(r->descs, r->hostDescs, sizeof(d), hipMemcpyHostToDevice, s), what);
+		PHX_HIP(hipMemcpyAsync(r->offsets, r->hostOff, sizeof(off), hipMemcpyHostToDevice, s),
+		        what);
+		r->descsValid = true;
+	}
+	return PHX_OK;
 }
 
 }  // namespace
@@ -131,6 +242,7 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 	if (st == PHX_OK) st = devAlloc(r->out, r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->descs, 2, what);
 	if (st == PHX_OK) st = devAlloc(r->offsets, 3, what);
+	if (st == PHX_OK) st = devAlloc(r->params, 1, what);
 	if (st != PHX_OK) {
 		freeRenderer(r);
 		return st;
@@ -190,81 +302,52 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 	if (st != PHX_OK) return st;
 
 	hipStream_t s = ctx->stream;
-	const uint32_t L = scene->numLights;
 	const bool taa = (flags & PHX_FRAME_TAA) != 0;
-	float jx = 0.0f, jy = 0.0f;
-	if (taa) phx_halton23(frameIndex, &jx, &jy);
-	const float camPos[3] = {cam->origin[0], cam->origin[1], cam->origin[2]};
-	const uint32_t order = (r->w % 8u == 0u && r->h % 8u == 0u) ? PHX_RAYS_TILED_8X8 : PHX_RAYS_ROW_MAJOR;
-	const SceneView sv = view(scene);
-	const TraceLaunch closest = traceLaunch(ctx, 0, false);
-	const TraceLaunch anyhit = traceLaunch(ctx, 0, true);
 	const char* what = "phx_render_frame";
-
-	// Primary hits -> hit records -> contributions and shadow rays of every light.
-	PHX_HIP(launchGenPrimaryRays(*cam, r->w, r->h, jx, jy, 1e-4f, FLT_MAX, order, r->prim, s), what);
-	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, closest, nullptr, s), what);
-	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->materials,
-	                        scene->numMaterials, r->gbuf, s),
-	        what);
-	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, seed,
-	                    frameIndex, 1u, r->contrib, r->shadow, s),
-	        what);
-	// Bounce rays (they need only the hit records), traced together with the primary shadow rays
-	// in one fused persistent launch.
-	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, camPos, seed, frameIndex, r->bounce,
-	                           r->throughput, s),
-	        what);
-	{
-		BatchDesc d[2];
-		const float4* nodes = reinterpret_cast<const float4*>(scene->nodes);
-		const float4* tris = reinterpret_cast<const float4*>(scene->tris);
-		d[0] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->shadow), r->inLight,
-		                 r->n * L, 1u, scene->finiteBounds, 0u};
-		d[1] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->bounce), r->bounceHits,
-		                 r->m, 0u, scene->finiteBounds, 0u};
-		const uint32_t off[3] = {0u, r->n * L, r->n * L + r->m};
-		if (!r->descsValid || std::memcmp(d, r->hostDescs, sizeof(d)) != 0 ||
-		    std::memcmp(off, r->hostOff, sizeof(off)) != 0) {
-			// The copies read the renderer's host copies, which change only after the stream
-			// has drained the frames that used them.
-			PHX_HIP(hipStreamSynchronize(s), what);
-			std::memcpy(r->hostDescs, d, sizeof(d));
-			std::memcpy(r->hostOff, off, sizeof(off));
-			PHX_HIP(hipMemcpyAsync(r->descs, r->hostDescs, sizeof(d), hipMemcpyHostToDevice, s),
-			        what);
-			PHX_HIP(hipMemcpyAsync(r->offsets, r->hostOff, sizeof(off), hipMemcpyHostToDevice, s),
-			        what);
-			r->descsValid = true;
+	FrameParams fp{};
+	if (taa) phx_halton23(frameIndex, &fp.jx, &fp.jy);
+	fp.frame = frameIndex;
+	fp.taaReset = (flags & PHX_FRAME_TAA_RESET) || !r->haveHistory ? 1u : 0u;
+	fp.seed = seed;
+	fp.histIn = r->histIndex;
+	st = uploadDescs(r, scene);
+	if (st != PHX_OK) return st;
+	PHX_HIP(launchSetFrameParams(r->params, fp, s), what);
+	if (flags & PHX_FRAME_GRAPH) {
+		PhxRenderer::GraphKey k{};
+		k.scene = scene;
+		k.cam = *cam;
+		k.flags = flags & (PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP);
+		k.numLights = scene->numLights;
+		k.lightBuffers = r->contrib;
+		const uint32_t launch[5] = {ctx->gridClosest, ctx->gridAnyhit, ctx->blockThreads,
+		                            ctx->topNodes, ctx->refillMin};
+		std::memcpy(k.launch, launch, sizeof(launch));
+		if (!r->graphExec || std::memcmp(&k, &r->key, sizeof(k)) != 0) {
+			dropGraph(r);
+			PHX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), what);
+			st = enqueueFrame(ctx, r, scene, cam, flags, s);
+			hipGraph_t g = nullptr;
+			const hipError_t e = hipStreamEndCapture(s, &g);
+			if (st != PHX_OK) {
+				if (g) hipGraphDestroy(g);
+				return st;
+			}
+			PHX_HIP(e, "phx_render_frame: capture");
+			r->graph = g;
+			PHX_HIP(hipGraphInstantiate(&r->graphExec, g, nullptr, nullptr, 0),
+			        "phx_render_frame: graph instantiate");
+			r->key = k;
 		}
-		PHX_HIP(launchBatches(r->descs, r->offsets, 2u, off[2], closest,
-		                      reinterpret_cast<const float4*>(scene->nodes), scene->topCount,
-		                      nullptr, s),
-		        what);
-	}
-	PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 1u, false, r->inLight, r->contrib, L, s), what);
-
-	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
-	if (r->m) {
-		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->materials,
-		                        scene->numMaterials, r->bounceInfo, s),
-		        what);
-		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
-		                    r->m, seed, frameIndex, 3u, r->contrib, r->shadow, s),
-		        what);
-		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, anyhit, s), what);
-		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
-		        what);
+		PHX_HIP(hipGraphLaunch(r->graphExec, s), "phx_render_frame: graph launch");
+	} else {
+		st = enqueueFrame(ctx, r, scene, cam, flags, s);
+		if (st != PHX_OK) return st;
 	}
 
 	r->lastOut = r->color;
 	if (taa) {
-		const bool reset = (flags & PHX_FRAME_TAA_RESET) || !r->haveHistory;
-		const uint32_t next = r->histIndex ^ 1u;
-		PHX_HIP(launchTaa(r->color, r->hist[r->histIndex], r->gbuf, r->w, r->h, reset,
-		                  !(flags & PHX_FRAME_NO_CLAMP), r->hist[next], r->out, s),
-		        what);
-		r->histIndex = next;
+		r->histIndex ^= 1u;
 		r->haveHistory = true;
 		r->lastOut = r->out;
 	}

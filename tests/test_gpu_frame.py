@@ -198,3 +198,39 @@ def test_frame_without_lights_and_light_growth(ctx, small):
         r.release()
         dark.release()
         two.release()
+
+
+def test_frame_graph_replays_match_direct_frames(ctx, small):
+    """Frames replayed from the captured hipGraph (PHX_FRAME_GRAPH) equal frames launched directly,
+    replay after replay, with other device work and synchronisations between the replays, across
+    a camera change (a new capture) and a TAA reset; the last one also equals the oracle's TAA."""
+    bvh, mats, lts, scene = small
+    w, h = 64, 40
+    cams = [W.camera(w, h), phx.camera_look_at((0.45, 0.55, -0.25), (0.5, 0.05, 0.6),
+                                                (0.0, 1.0, 0.0), math.radians(55), w, h)]
+    rays = np.zeros(4096, phx.RAY)
+    rng = np.random.default_rng(3)
+    rays["o"] = rng.uniform(0.0, 1.0, (4096, 3))
+    rays["d"] = (0.0, -1.0, 0.0)
+    rays["tmax"] = phx.FLT_MAX
+    direct, graph = ctx.renderer(w, h), ctx.renderer(w, h)
+    try:
+        for f in range(12):
+            cam = cams[0] if f < 7 else cams[1]
+            flags = phx.FRAME_TAA | (phx.FRAME_TAA_RESET if f in (0, 7) else 0)
+            want = direct.render(scene, cam, f, SEED, flags)
+            got = graph.render(scene, cam, f, SEED, flags | phx.FRAME_GRAPH)
+            bits_equal(got, want, f"graph replay, frame {f}")
+            ctx.cast_closest(scene, rays)  # eager work between replays
+            ctx.synchronize()
+        # the replayed TAA sequence after the reset at frame 7 is the oracle's
+        hist, ids, cam = None, O.pixel_ids(w, h), cams[1]
+        for f in range(7, 12):
+            jx, jy = phx.halton23(f)
+            color, gbuf, _ = O.frame(bvh.nodes, bvh.tris, bvh.datas, mats, lts, cam.as_array(), w,
+                                     h, jx, jy, SEED, f)
+            hist, ref = O.taa(color, hist, gbuf["roughness"][ids], reset=(f == 7), clamp=True)
+        bits_equal(got, ref, "graph replay vs oracle TAA")
+    finally:
+        direct.release()
+        graph.release()
