@@ -263,6 +263,31 @@ void phx_scene_release(PhxScene* scene);
 /* Device bytes held by a scene. */
 size_t phx_scene_device_bytes(const PhxScene* scene);
 
+/* RawImage (rendering/RawImage.hpp:21-29): `height` rows of `pitch` bytes, row 0 first, each
+ * texel `bytesPerPixel` consecutive bytes. */
+typedef struct PhxTexture {
+	const uint8_t* data;
+	uint32_t width;
+	uint32_t height;
+	uint32_t bytesPerPixel; /* 1, 2 or 4 (CudaBindlessTexture.cpp:42-53 rejects 3) */
+	uint32_t pitch;         /* bytes per row, >= width * bytesPerPixel */
+} PhxTexture;
+
+/* Material textures. Replaces the texture half of setMaterialsAndTextures
+ * (CudaTracerRenderer.cpp:192-239) and CudaBindlessTexture::load (CudaBindlessTexture.cpp:33-67).
+ * Copies `count` images (rows packed to width * bytesPerPixel) into one device arena owned by the
+ * scene, replacing the textures set before; count 0 removes them. A material's iData[0..2]
+ * index this list; an index < 0 or >= count means no texture for that channel. Sampling (used
+ * by phx_interpret_hits and phx_render_frame): nearest texel, wrapping, normalised coordinates,
+ * texel x = floor(fmod(fmod(u * width, width) + width, width)) and y likewise from v with
+ * height (CPURayTracerRenderer::sampleImage, CPUTracerRenderer.cpp:160-180; non-finite
+ * coordinates read texel 0). Albedo reads rgba from 4-byte textures, rgb linearised as
+ * pow(c / 255, 2.2) and rounded once to float, alpha c / 255 (InterpretRayHitKernel.cu:86-93);
+ * textures of 1 or 2 bytes per texel leave the constant albedo (CPUTracerRenderer.cpp:200-206).
+ * Roughness and metallic read byte 0 of a texel as c / 255 (:96-111). */
+phx_status phx_scene_set_textures(PhxContext* ctx, PhxScene* scene, const PhxTexture* textures,
+                                  uint32_t count);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Scene files: the reference's .bvhcache (three raw array dumps, testbed_application/         */
 /* Main.cpp:105-154) as one versioned file with a checksum, plus an importer of the raw dumps  */
@@ -403,6 +428,24 @@ phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32
 phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                                const PhxRayHit* hits, uint32_t numRays, uint64_t seed,
                                uint32_t frame, PhxRayIn* outRays, uint32_t flags);
+
+/* RayHitInfo -- kernels/InterpretRayHitKernel.hpp:21-58 (48 B): [pos, roughness],
+ * [normal, metallic], [albedo rgb, alpha]; alpha < 0 marks a miss. */
+typedef struct PhxRayHitInfo {
+	float data1[4];
+	float data2[4];
+	float data3[4];
+} PhxRayHitInfo;
+
+/* Replaces launchInterpretRayHitKernel (InterpretRayHitKernel.hpp:76-78, .cu:58-136). A hit
+ * gets position o + d*t, normal normalize((n0 + (n1-n0)u) + (n2-n0)v), uv interpolated the same
+ * way, and its material: constant values, or texture samples as phx_scene_set_textures describes;
+ * a material index outside the table reads albedo 1, alpha 1, roughness 1, metallic 0. A miss
+ * gets zeros and alpha -1. The scene must have TriangleData. `flags` = PHX_MEM_DEVICE |
+ * PHX_ASYNC. */
+phx_status phx_interpret_hits(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                              const PhxRayHit* hits, uint32_t numRays, PhxRayHitInfo* out,
+                              uint32_t flags);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Served form: a request loop over resident scenes with continuous batching. No counterpart in  */

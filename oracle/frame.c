@@ -264,11 +264,46 @@ static inline void set_dummy(OrRay* r)
 
 static inline V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
-/* interpretRayHitKernel, InterpretRayHitKernel.cu:13-121 (constant materials; a material index
- * outside the table reads the default material: albedo 1, roughness 1, metallic 0). */
-static void interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const OrTriData* datas,
-                      const OrMaterial* mats, uint32_t numMats, OrHitInfo* out)
+/* CPURayTracerRenderer::sampleImage (CPUTracerRenderer.cpp:160-180) along one axis: the texel
+ * index floor(fmod(fmod(c * dim, dim) + dim, dim)); NaN and infinite c (for which the reference
+ * converts NaN to an integer) read texel 0. */
+uint32_t oracle_texel_coord(float c, uint32_t dim)
 {
+	const float d = (float)dim;
+	float s = fmodf(c * d, d);
+	s = s + d;
+	s = fmodf(s, d);
+	const float f = floorf(s);
+	if (isnan(f)) return 0u;
+	return (uint32_t)f;
+}
+
+/* linearize (InterpretRayHitKernel.cu:39-45) of the 256 byte values c / 255.0f: powf(x, 2.2f),
+ * here evaluated in long double and rounded to float. */
+void oracle_linear_table(float out[256])
+{
+	for (int c = 0; c < 256; c++) {
+		const float x = (float)c / 255.0f;
+		out[c] = (float)powl((long double)x, (long double)2.2f);
+	}
+}
+
+static const uint8_t* texel_of(const OrTexture* t, float u, float v)
+{
+	const uint32_t x = oracle_texel_coord(u, t->width), y = oracle_texel_coord(v, t->height);
+	return t->data + (size_t)y * t->pitch + (size_t)x * t->bpp;
+}
+
+/* interpretRayHitKernel, InterpretRayHitKernel.cu:13-121; a material index outside the table
+ * reads the default material (albedo 1, roughness 1, metallic 0), a texture index outside
+ * [0, numTex) means no texture. Albedo textures are read only with 4 bytes per texel
+ * (CPUTracerRenderer.cpp:200-206; the CUDA path reads uchar4). */
+void oracle_interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const OrTriData* datas,
+                      const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
+                      uint32_t numTex, OrHitInfo* out)
+{
+	float lin[256];
+	oracle_linear_table(lin);
 	for (uint32_t i = 0; i < n; i++) {
 		OrHitInfo hi;
 		memset(&hi, 0, sizeof(hi));
@@ -294,6 +329,24 @@ static void interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const Or
 				memcpy(alb, m->albedo, sizeof(alb));
 				rough = m->f2[0];
 				metal = m->f2[1];
+				float uv[2];
+				for (int k = 0; k < 2; k++) {
+					float a = (d->uv1[k] - d->uv0[k]) * bu;
+					float b = (d->uv2[k] - d->uv0[k]) * bv;
+					uv[k] = (d->uv0[k] + a) + b;
+				}
+				const int32_t ai = m->i[0], ri = m->i[1], mi = m->i[2];
+				if (ai >= 0 && (uint32_t)ai < numTex && texs[ai].bpp == 4u) {
+					const uint8_t* p = texel_of(&texs[ai], uv[0], uv[1]);
+					alb[0] = lin[p[0]];
+					alb[1] = lin[p[1]];
+					alb[2] = lin[p[2]];
+					alb[3] = (float)p[3] / 255.0f;
+				}
+				if (ri >= 0 && (uint32_t)ri < numTex)
+					rough = (float)texel_of(&texs[ri], uv[0], uv[1])[0] / 255.0f;
+				if (mi >= 0 && (uint32_t)mi < numTex)
+					metal = (float)texel_of(&texs[mi], uv[0], uv[1])[0] / 255.0f;
 			}
 			hi.d1[3] = rough;
 			hi.d2[3] = metal;
@@ -466,7 +519,8 @@ void oracle_bounce_rays(const OrTriData* datas, const OrRay* rays, const OrHit* 
  * shade x throughput + shadow rays, any-hit, shadow logic added at half resolution.
  * Outputs: colour (w*h RGBA, row-major); optional G-buffer (record order) and bounce rays. */
 void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas,
-                  const OrMaterial* mats, uint32_t numMats, const OrLight* lights,
+                  const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
+                  uint32_t numTex, const OrLight* lights,
                   uint32_t numLights, const float cam[12], uint32_t w, uint32_t h, float jx,
                   float jy, uint64_t seed, uint32_t frame, float* color, OrHitInfo* gbufOut,
                   OrRay* bounceOut, uint32_t threads)
@@ -484,7 +538,7 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 
 	oracle_primary_rays_jittered(cam, w, h, jx, jy, 1e-4f, FLT_MAX, prim);
 	oracle_cast_closest(nodes, tris, prim, N, ph, threads);
-	interpret(prim, ph, N, datas, mats, numMats, gb);
+	oracle_interpret(prim, ph, N, datas, mats, numMats, texs, numTex, gb);
 
 	/* gBufferMaterialKernel (MaterialKernel.cu:119-140); a pixel without a hit gets dummy rays
 	 * and no contribution, as materialKernel does for a secondary miss (:151-164). */
@@ -561,7 +615,7 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 			}
 		if (bounceOut) memcpy(bounceOut, sec, sizeof(OrRay) * M);
 		oracle_cast_closest(nodes, tris, sec, M, sh, threads);
-		interpret(sec, sh, M, datas, mats, numMats, sgb);
+		oracle_interpret(sec, sh, M, datas, mats, numMats, texs, numTex, sgb);
 		/* materialKernel (MaterialKernel.cu:145-184) */
 		size_t ms = (size_t)M * numLights;
 		for (uint32_t id = 0; id < M; id++) {

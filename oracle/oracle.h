@@ -115,6 +115,12 @@ typedef struct OrLight {
 	uint8_t staticShadows, dynamicShadows, pad[2];
 } OrLight;
 
+/* RawImage (rendering/RawImage.hpp:21-29): `height` rows of `pitch` bytes, `bpp` bytes a texel. */
+typedef struct OrTexture {
+	const uint8_t* data;
+	uint32_t width, height, bpp, pitch;
+} OrTexture;
+
 typedef struct OrHitInfo {
 	float d1[4]; /* position, roughness */
 	float d2[4]; /* normal, metallic */
@@ -132,8 +138,16 @@ void oracle_primary_rays_jittered(const float cam[12], uint32_t w, uint32_t h, f
                                   float tMin, float tMax, OrRay* rays);
 void oracle_bounce_rays(const OrTriData* datas, const OrRay* rays, const OrHit* hits, uint32_t n,
                         uint64_t seed, uint32_t frame, OrRay* out);
+/* Texel coordinate of a wrapping nearest-texel lookup along one axis of `dim` texels. */
+uint32_t oracle_texel_coord(float c, uint32_t dim);
+/* The 256 values pow(c / 255, 2.2) rounded to float. */
+void oracle_linear_table(float out[256]);
+void oracle_interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const OrTriData* datas,
+                      const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
+                      uint32_t numTex, OrHitInfo* out);
 void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas,
-                  const OrMaterial* mats, uint32_t numMats, const OrLight* lights,
+                  const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
+                  uint32_t numTex, const OrLight* lights,
                   uint32_t numLights, const float cam[12], uint32_t w, uint32_t h, float jx,
                   float jy, uint64_t seed, uint32_t frame, float* color, OrHitInfo* gbufOut,
                   OrRay* bounceOut, uint32_t threads);

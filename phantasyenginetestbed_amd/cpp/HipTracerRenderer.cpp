@@ -38,26 +38,49 @@ void HipTracerRenderer::uploadScene() noexcept
 	                             mStatic.sphereLights.empty() ? nullptr : mStatic.sphereLights.data(),
 	                             uint32_t(mStatic.sphereLights.size()), &s)))
 		return;
+	uploadTextures(s);
+	if (mStatus != PHX_OK) {
+		phx_scene_release(s);
+		return;
+	}
 	if (mScene) phx_scene_release(mScene);
 	mScene = s;
 	mFrame = 0;
 }
 
+void HipTracerRenderer::uploadTextures(PhxScene* scene) noexcept
+{
+	std::vector<PhxTexture> t(mTextures.size());
+	for (size_t i = 0; i < t.size(); i++) {
+		const RawImage& img = mTextures[i];
+		t[i] = PhxTexture{img.imgData.data(), uint32_t(img.dim.x), uint32_t(img.dim.y),
+		                  img.bytesPerPixel, img.pitch};
+		// An image whose rows do not fit its buffer is rejected like an empty one.
+		if (size_t(img.pitch) * uint32_t(img.dim.y) > img.imgData.size()) t[i].data = nullptr;
+	}
+	record(phx_scene_set_textures(mCtx, scene, t.empty() ? nullptr : t.data(),
+	                              uint32_t(t.size())));
+}
+
+// The reference's CUDA backend uploads the images as bindless textures here
+// (CudaTracerRenderer.cpp:192-239); the scene is re-uploaded with them.
 void HipTracerRenderer::setMaterialsAndTextures(const std::vector<PhxMaterial>& materials,
                                                 const std::vector<RawImage>& textures) noexcept
 {
 	mMaterials = materials;
+	mTextures = textures;
 	uploadScene();
-	if (!textures.empty() && mStatus == PHX_OK) {
-		mStatus = PHX_ERR_INVALID_ARGUMENT;
-		mError = "HipTracerRenderer: textures are not sampled (constant materials only)";
-	}
 }
 
-void HipTracerRenderer::addTexture(const RawImage&) noexcept
+// Unimplemented in the reference (CudaTracerRenderer.cpp:241-244); here the texture list grows
+// by one and the current scene takes the new list.
+void HipTracerRenderer::addTexture(const RawImage& texture) noexcept
 {
-	mStatus = PHX_ERR_INVALID_ARGUMENT;
-	mError = "HipTracerRenderer::addTexture: textures are not sampled (constant materials only)";
+	mTextures.push_back(texture);
+	if (mScene) {
+		uploadTextures(mScene);
+		if (mStatus != PHX_OK) mTextures.pop_back();
+	}
 }
 
 void HipTracerRenderer::addMaterial(const PhxMaterial& material) noexcept

@@ -49,7 +49,13 @@ struct StaticScene {
 	std::vector<PhxSphereLight> sphereLights;
 };
 
-struct RawImage {};   // textures are not sampled by this backend
+// rendering/RawImage.hpp:21-29: dim.y rows of `pitch` bytes, bytesPerPixel bytes a texel.
+struct RawImage {
+	std::vector<uint8_t> imgData;
+	vec2i dim;
+	uint32_t bytesPerPixel = 0;
+	uint32_t pitch = 0;
+};
 struct RawMesh {};    // dynamic meshes are not supported (nor by the reference's CUDA backend)
 struct DynObject {};
 
@@ -128,11 +134,13 @@ protected:
 private:
 	bool record(phx_status st) noexcept;
 	void uploadScene() noexcept;
+	void uploadTextures(PhxScene* scene) noexcept;
 
 	PhxContext* mCtx = nullptr;
 	PhxScene* mScene = nullptr;
 	PhxRenderer* mRenderer = nullptr;
 	std::vector<PhxMaterial> mMaterials;
+	std::vector<RawImage> mTextures;
 	StaticScene mStatic;
 	bool mHaveScene = false;
 	bool mTaa = true;

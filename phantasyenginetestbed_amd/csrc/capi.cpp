@@ -84,6 +84,7 @@ void freeScene(PhxScene* s)
 	hipFree(s->datas);
 	hipFree(s->materials);
 	hipFree(s->lights);
+	hipFree(s->texArena);
 	delete s;
 }
 
@@ -529,6 +530,93 @@ size_t phx_scene_device_bytes(const PhxScene* scene)
 	return scene->bytes;
 }
 
+phx_status phx_scene_set_textures(PhxContext* ctx, PhxScene* scene, const PhxTexture* textures,
+                                  uint32_t count)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (count && !textures)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_set_textures: textures is NULL");
+	if (count > (1u << 20))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_set_textures: more than 2^20 textures");
+	// Arena: linearisation table (256 floats), descriptors, then each image's packed rows.
+	const size_t tableBytes = 256 * sizeof(float);
+	const size_t descBytes = size_t(count) * sizeof(TexDesc);
+	std::vector<TexDesc> descs(count);
+	size_t texelBytes = 0;
+	for (uint32_t i = 0; i < count; i++) {
+		const PhxTexture& t = textures[i];
+		const uint32_t bpp = t.bytesPerPixel;
+		if (!t.data || t.width == 0 || t.height == 0)
+			return fail(PHX_ERR_INVALID_ARGUMENT, "phx_scene_set_textures: texture %u is empty", i);
+		if (bpp != 1 && bpp != 2 && bpp != 4)
+			return fail(PHX_ERR_INVALID_ARGUMENT,
+			            "phx_scene_set_textures: texture %u has %u bytes per texel (1, 2 or 4)", i,
+			            bpp);
+		// Texel coordinates are computed in float: dimensions up to 2^16 stay exact with room.
+		if (t.width > 65536u || t.height > 65536u)
+			return fail(PHX_ERR_INVALID_ARGUMENT,
+			            "phx_scene_set_textures: texture %u is larger than 65536 texels a side", i);
+		if (t.pitch < t.width * bpp)
+			return fail(PHX_ERR_INVALID_ARGUMENT,
+			            "phx_scene_set_textures: texture %u pitch %u < width * bytes per texel", i,
+			            t.pitch);
+		const uint32_t rowBytes = t.width * bpp;
+		descs[i] = TexDesc{uint64_t(tableBytes + descBytes + texelBytes), t.width, t.height,
+		                   rowBytes, bpp, {0u, 0u}};
+		texelBytes += (size_t(rowBytes) * t.height + 15u) & ~size_t(15);
+	}
+	void* arena = nullptr;
+	const size_t total = tableBytes + descBytes + texelBytes;
+	if (count) {
+		// Texel offsets are relative to the arena base (the `texels` pointer of the set).
+		std::vector<uint8_t> host(total);
+		float* table = reinterpret_cast<float*>(host.data());
+		for (uint32_t c = 0; c < 256; c++) {
+			// powf(c / 255.0f, 2.2f) (InterpretRayHitKernel.cu:39-45), evaluated in double and
+			// rounded once to float.
+			const float x = float(c) / 255.0f;
+			table[c] = float(std::pow(double(x), double(2.2f)));
+		}
+		std::memcpy(host.data() + tableBytes, descs.data(), descBytes);
+		for (uint32_t i = 0; i < count; i++) {
+			const PhxTexture& t = textures[i];
+			const uint32_t rowBytes = descs[i].pitch;
+			for (uint32_t y = 0; y < t.height; y++)
+				std::memcpy(host.data() + descs[i].offset + size_t(y) * rowBytes,
+				            t.data + size_t(y) * t.pitch, rowBytes);
+		}
+		if (hipMalloc(&arena, total) != hipSuccess)
+			return fail(PHX_ERR_OUT_OF_MEMORY, "phx_scene_set_textures: device allocation of %zu "
+			            "bytes failed", total);
+		hipError_t e = hipMemcpy(arena, host.data(), total, hipMemcpyHostToDevice);
+		if (e != hipSuccess) {
+			hipFree(arena);
+			return hipFail(e, "phx_scene_set_textures");
+		}
+	}
+	// Work queued on the context stream may still read the old arena.
+	hipError_t e = hipStreamSynchronize(ctx->stream);
+	if (e != hipSuccess) {
+		hipFree(arena);
+		return hipFail(e, "phx_scene_set_textures");
+	}
+	hipFree(scene->texArena);
+	scene->bytes -= scene->texBytes;
+	scene->texArena = arena;
+	scene->texBytes = count ? total : 0;
+	scene->bytes += scene->texBytes;
+	const uint8_t* base = static_cast<const uint8_t*>(arena);
+	scene->tex.linear = count ? reinterpret_cast<const float*>(base) : nullptr;
+	scene->tex.descs = count ? reinterpret_cast<const TexDesc*>(base + tableBytes) : nullptr;
+	scene->tex.texels = count ? base : nullptr;
+	scene->tex.count = count;
+	scene->version++;
+	return ok();
+}
+
 phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                             uint32_t numRays, PhxRayHit* hits, uint32_t flags)
 {
@@ -646,6 +734,51 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
 	hipFree(tmp);
 	if (e != hipSuccess) return hipFail(e, "phx_gen_bounce_rays");
 	if (e2 != hipSuccess) return hipFail(e2, "phx_gen_bounce_rays");
+	return ok();
+}
+
+phx_status phx_interpret_hits(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                              const PhxRayHit* hits, uint32_t numRays, PhxRayHitInfo* out,
+                              uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (!scene->datas)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_interpret_hits: scene has no TriangleData");
+	if (numRays == 0) return ok();
+	if (!rays || !hits || !out)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_interpret_hits: null pointer");
+	const bool device = (flags & PHX_MEM_DEVICE) != 0;
+	const bool async = (flags & PHX_ASYNC) != 0;
+	if (async && !device)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_interpret_hits: PHX_ASYNC needs PHX_MEM_DEVICE");
+	if (device) {
+		PHX_HIP(launchInterpret(rays, hits, numRays, scene->datas, scene->materials,
+		                        scene->numMaterials, scene->tex, reinterpret_cast<float4*>(out),
+		                        ctx->stream),
+		        "phx_interpret_hits");
+		if (!async) PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_interpret_hits");
+		return ok();
+	}
+	const size_t rb = size_t(numRays) * sizeof(PhxRayIn), hb = size_t(numRays) * sizeof(PhxRayHit);
+	const size_t ob = size_t(numRays) * sizeof(PhxRayHitInfo);
+	void* tmp = nullptr;
+	PHX_HIP(hipMalloc(&tmp, rb + hb + ob), "phx_interpret_hits");
+	PhxRayIn* dr = static_cast<PhxRayIn*>(tmp);
+	PhxRayHit* dh = reinterpret_cast<PhxRayHit*>(dr + numRays);
+	float4* dout = reinterpret_cast<float4*>(dh + numRays);
+	hipError_t e = hipMemcpyAsync(dr, rays, rb, hipMemcpyHostToDevice, ctx->stream);
+	if (e == hipSuccess) e = hipMemcpyAsync(dh, hits, hb, hipMemcpyHostToDevice, ctx->stream);
+	if (e == hipSuccess)
+		e = launchInterpret(dr, dh, numRays, scene->datas, scene->materials, scene->numMaterials,
+		                    scene->tex, dout, ctx->stream);
+	if (e == hipSuccess) e = hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, ctx->stream);
+	hipError_t e2 = hipStreamSynchronize(ctx->stream);
+	hipFree(tmp);
+	if (e != hipSuccess) return hipFail(e, "phx_interpret_hits");
+	if (e2 != hipSuccess) return hipFail(e2, "phx_interpret_hits");
 	return ok();
 }
 

@@ -50,15 +50,40 @@ __device__ __forceinline__ void store_dummy(float4* rays, size_t i)
 
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
+// Texel coordinate along one axis of a wrapping, nearest-texel lookup
+// (CPURayTracerRenderer::sampleImage, CPUTracerRenderer.cpp:160-180):
+// floor(fmod(fmod(c * dim, dim) + dim, dim)), in [0, dim) for finite c; NaN or infinite c reads 0.
+__device__ __forceinline__ uint32_t texel_coord(float c, uint32_t dim)
+{
+	const float d = float(dim);
+	float s = fmodf(c * d, d);
+	s = fmodf(s + d, d);
+	const float f = floorf(s);
+	return f >= 0.0f ? uint32_t(f) : 0u;
+}
+
+// First byte of the texel of texture `idx` at (u, v); its size in *bpp.
+__device__ __forceinline__ const uint8_t* texel_at(const TexSet& tex, uint32_t idx, float u,
+                                                   float v, uint32_t* bpp)
+{
+	const TexDesc d = tex.descs[idx];
+	const uint32_t x = texel_coord(u, d.width), y = texel_coord(v, d.height);
+	*bpp = d.bpp;
+	return tex.texels + d.offset + size_t(y) * d.pitch + size_t(x) * d.bpp;
+}
+
 // interpretRayHitKernel, InterpretRayHitKernel.cu:58-121: position o + d*t, normal
-// normalize(n0 + (n1-n0)u + (n2-n0)v), constant material (albedo rgba, roughness, metallic; an
-// index outside the table reads albedo 1, roughness 1, metallic 0). Record layout (48 B):
-// (pos, roughness) (normal, metallic) (albedo, alpha), alpha negative for a miss.
+// normalize((n0 + (n1-n0)u) + (n2-n0)v), uv interpolated the same way, material from the table
+// (an index outside it reads albedo 1, roughness 1, metallic 0). A texture index in [0, count)
+// replaces the constant: albedo from a 4-byte texel, rgb through the linearisation table, alpha
+// c / 255 (:86-93; other texel sizes keep the constant, CPUTracerRenderer.cpp:200-206); roughness
+// and metallic from byte 0 as c / 255 (:96-111). Record layout (48 B): (pos, roughness)
+// (normal, metallic) (albedo, alpha), alpha negative for a miss.
 __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict__ rays,
                                                         const uint4* __restrict__ hits, uint32_t n,
                                                         const PhxTriangleData* __restrict__ datas,
                                                         const PhxMaterial* __restrict__ mats,
-                                                        uint32_t numMats,
+                                                        uint32_t numMats, TexSet tex,
                                                         float4* __restrict__ info)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,6 +113,28 @@ __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict
 		alb = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], m.albedo[3]);
 		rough = m.fData2[0];
 		metal = m.fData2[1];
+		if (tex.count != 0u) {
+			float uv[2];
+			for (int k = 0; k < 2; k++) {
+				const float a = (d.uv1[k] - d.uv0[k]) * bu;
+				const float b = (d.uv2[k] - d.uv0[k]) * bv;
+				uv[k] = (d.uv0[k] + a) + b;
+			}
+			uint32_t bpp;
+			const int32_t ai = m.iData[0], ri = m.iData[1], mi = m.iData[2];
+			if (ai >= 0 && uint32_t(ai) < tex.count) {
+				const uint8_t* p = texel_at(tex, uint32_t(ai), uv[0], uv[1], &bpp);
+				if (bpp == 4u) {
+					const uchar4 c = *reinterpret_cast<const uchar4*>(p);
+					alb = make_float4(tex.linear[c.x], tex.linear[c.y], tex.linear[c.z],
+					                  float(c.w) / 255.0f);
+				}
+			}
+			if (ri >= 0 && uint32_t(ri) < tex.count)
+				rough = float(*texel_at(tex, uint32_t(ri), uv[0], uv[1], &bpp)) / 255.0f;
+			if (mi >= 0 && uint32_t(mi) < tex.count)
+				metal = float(*texel_at(tex, uint32_t(mi), uv[0], uv[1], &bpp)) / 255.0f;
+		}
 	}
 	info[3 * size_t(i)] = make_float4(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t, rough);
 	info[3 * size_t(i) + 1] = make_float4(nrm.x, nrm.y, nrm.z, metal);
@@ -323,12 +370,12 @@ dim3 grid2d(uint32_t w, uint32_t h) { return dim3((w + 15u) / 16u, (h + 15u) / 1
 
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
                            const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
-                           float4* info, hipStream_t stream)
+                           const TexSet& tex, float4* info, hipStream_t stream)
 {
 	if (n == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_interpret, dim3(blocks1d(n)), dim3(kBlock1D), 0, stream,
 	                   reinterpret_cast<const float4*>(rays), reinterpret_cast<const uint4*>(hits),
-	                   n, datas, mats, numMats, info);
+	                   n, datas, mats, numMats, tex, info);
 	return hipGetLastError();
 }
 

@@ -20,6 +20,22 @@ struct SceneView {
 	uint32_t topCount;      // nodes [0, topCount) are the top levels in breadth-first order
 };
 
+// Material textures of a scene: one device arena holding the linearisation table, the
+// descriptors and the texels (rows packed to width * bpp bytes, each image 16-byte aligned).
+struct TexDesc {
+	uint64_t offset;  // byte offset of texel (0, 0) from `texels`
+	uint32_t width, height, pitch, bpp;
+	uint32_t pad[2];
+};
+static_assert(sizeof(TexDesc) == 32, "TexDesc is 32 bytes");
+
+struct TexSet {
+	const float* linear = nullptr;  // linear[c] = pow(c / 255, 2.2), rounded once to float
+	const TexDesc* descs = nullptr;
+	const uint8_t* texels = nullptr;
+	uint32_t count = 0;
+};
+
 enum class TraceKernel { Persistent, Simple };
 
 // Launch parameters of the traversal kernels. `work` points at two zeroed uint32 words owned by
@@ -116,7 +132,7 @@ hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h,
 // (position, roughness) (normal, metallic) (albedo, alpha; alpha < 0 = no hit).
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
                            const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
-                           float4* info, hipStream_t stream);
+                           const TexSet& tex, float4* info, hipStream_t stream);
 // The seed and frame index of the random-number streams come from fp.
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
                        const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,

@@ -40,6 +40,7 @@ struct PhxRenderer {
 	// The captured frame (PHX_FRAME_GRAPH) and what it was captured for.
 	struct GraphKey {
 		const PhxScene* scene;
+		uint64_t sceneVersion;  // bumped when the scene's textures are replaced
 		PhxCameraDef cam;
 		uint32_t flags;  // PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP
 		uint32_t numLights;
@@ -125,7 +126,7 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	        what);
 	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, closest, nullptr, s), what);
 	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->materials,
-	                        scene->numMaterials, r->gbuf, s),
+	                        scene->numMaterials, scene->tex, r->gbuf, s),
 	        what);
 	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, r->params, 1u,
 	                    r->contrib, r->shadow, s),
@@ -147,7 +148,7 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
 	if (r->m) {
 		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->materials,
-		                        scene->numMaterials, r->bounceInfo, s),
+		                        scene->numMaterials, scene->tex, r->bounceInfo, s),
 		        what);
 		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
 		                    r->m, r->params, 3u, r->contrib, r->shadow, s),
@@ -316,6 +317,7 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 	if (flags & PHX_FRAME_GRAPH) {
 		PhxRenderer::GraphKey k{};
 		k.scene = scene;
+		k.sceneVersion = scene->version;
 		k.cam = *cam;
 		k.flags = flags & (PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP);
 		k.numLights = scene->numLights;

@@ -105,6 +105,10 @@ struct PhxScene {
 	uint32_t finiteBounds = 0;
 	uint32_t topCount = 0;
 	size_t bytes = 0;
+	void* texArena = nullptr;  // material textures (phx_scene_set_textures)
+	size_t texBytes = 0;
+	phx::TexSet tex;
+	uint64_t version = 0;  // incremented when the textures are replaced
 };
 
 namespace phx {

@@ -73,6 +73,35 @@ class Request(C.Structure):
                 ("pad", C.c_uint32)]
 
 
+class Texture(C.Structure):
+    """PhxTexture: `height` rows of `pitch` bytes, `bytesPerPixel` bytes a texel."""
+    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("bytesPerPixel", C.c_uint32), ("pitch", C.c_uint32)]
+
+
+def texture_arrays(images):
+    """ctypes PhxTexture array for `images`: uint8 arrays of shape (h, w, bpp) or (h, w) (one
+    byte a texel), or (rows, width, bpp) tuples where `rows` is a (h, pitch) uint8 array.
+    Returns (array, keep-alive list)."""
+    arr = (Texture * max(len(images), 1))()
+    keep = []
+    for i, img in enumerate(images):
+        if isinstance(img, tuple):
+            rows, width, bpp = img
+            rows = np.ascontiguousarray(rows, np.uint8)
+            h, pitch = rows.shape
+        else:
+            a = np.ascontiguousarray(img, np.uint8)
+            if a.ndim == 2:
+                a = a[:, :, None]
+            h, width, bpp = a.shape
+            rows = a.reshape(h, width * bpp)
+            pitch = width * bpp
+        keep.append(rows)
+        arr[i] = Texture(rows.ctypes.data, width, h, bpp, pitch)
+    return arr, keep
+
+
 class SceneArrays(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("numNodes", C.c_uint32), ("numTriangles", C.c_uint32),
                 ("triangles", C.c_void_p), ("datas", C.c_void_p), ("materials", C.c_void_p),
@@ -140,6 +169,8 @@ _SIGNATURES = {
                                    C.POINTER(_P)]),
     "phx_scene_release": (None, [_P]),
     "phx_scene_device_bytes": (C.c_size_t, [_P]),
+    "phx_scene_set_textures": (C.c_int, [_P, _P, _P, _U32]),
+    "phx_interpret_hits": (C.c_int, [_P, _P, _P, _P, _U32, _P, _U32]),
     "phx_cast_closest": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_cast_anyhit": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_traversal_stats": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(C.c_uint64)]),
@@ -405,6 +436,12 @@ class Scene:
     def device_bytes(self) -> int:
         return lib().phx_scene_device_bytes(self.handle)
 
+    def set_textures(self, images) -> None:
+        """Material textures (phx_scene_set_textures); `images` as texture_arrays takes them."""
+        arr, keep = texture_arrays(images)
+        check(lib().phx_scene_set_textures(self.ctx.handle, self.handle, arr, len(images)))
+        del keep
+
     def release(self) -> None:
         if self.handle:
             lib().phx_scene_release(self.handle)
@@ -661,6 +698,16 @@ class Context:
         out = np.empty(len(rays), RAY)
         check(lib().phx_gen_bounce_rays(self.handle, scene.handle, _ptr(rays), _ptr(hits),
                                         len(rays), seed, frame, _ptr(out), 0))
+        return out
+
+    def interpret_hits(self, scene: Scene, rays: np.ndarray, hits: np.ndarray) -> np.ndarray:
+        """RayHitInfo records (HITINFO) of closest hits (phx_interpret_hits)."""
+        rays = np.ascontiguousarray(rays, RAY)
+        hits = np.ascontiguousarray(hits, HIT)
+        assert len(rays) == len(hits)
+        out = np.empty(len(rays), HITINFO)
+        check(lib().phx_interpret_hits(self.handle, scene.handle, _ptr(rays), _ptr(hits),
+                                       len(rays), _ptr(out), 0))
         return out
 
     def gen_bounce_rays_device(self, scene: Scene, rays_ptr: int, hits_ptr: int, n: int,

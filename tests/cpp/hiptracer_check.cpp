@@ -1,9 +1,10 @@
 // Engine-side check of the C++ renderer interface (HipTracerRenderer) and the kernel-level launch
 // functions, written the way engine code uses the reference's CudaTracerRenderer: set materials
 // and the static scene, set the target resolution, update the camera, render frames. Every frame
-// must equal the C ABI's phx_render_frame on the same inputs, byte for byte, and the launch
-// functions must equal phx_cast_closest / phx_cast_anyhit. Built as a shared library and called in
-// process (hiptracer_check_run returns the number of failed checks).
+// must equal the C ABI's phx_render_frame on the same inputs (textured materials included), byte
+// for byte, and the launch functions must equal phx_cast_closest / phx_cast_anyhit. Built as a
+// shared library and called in process (hiptracer_check_run returns the number of failed
+// checks).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -108,13 +109,71 @@ extern "C" int hiptracer_check_run()
 		for (size_t i = 0; i < a.size(); i += 4) sum += a[i] + a[i + 1] + a[i + 2];
 		CHECK(sum > 0.0 && std::isfinite(sum), "frame %u is black or not finite", f);
 	}
-	phx_renderer_release(direct);
 
 	// Unsupported engine calls record an error and change nothing.
 	renderer.setDynamicMeshes({RawMesh{}});
 	CHECK(renderer.lastStatus() == PHX_ERR_INVALID_ARGUMENT, "dynamic meshes");
-	renderer.addTexture(RawImage{});
-	CHECK(renderer.lastStatus() == PHX_ERR_INVALID_ARGUMENT, "textures");
+
+	// Material textures: setMaterialsAndTextures + addTexture give the frames of a scene uploaded
+	// and textured through the C ABI, which differ from the untextured scene's frames.
+	std::printf("hiptracer_check: textures\n");
+	RawImage rgba;
+	rgba.dim = vec2i{8, 4};
+	rgba.bytesPerPixel = 4;
+	rgba.pitch = 32;
+	rgba.imgData.resize(128);
+	for (size_t i = 0; i < rgba.imgData.size(); i++) rgba.imgData[i] = uint8_t(i * 37 + 11);
+	RawImage gray;
+	gray.dim = vec2i{3, 3};
+	gray.bytesPerPixel = 1;
+	gray.pitch = 4;  // one padding byte per row
+	gray.imgData.resize(12);
+	for (size_t i = 0; i < gray.imgData.size(); i++) gray.imgData[i] = uint8_t(200 - i * 13);
+	std::vector<PhxMaterial> tmats = mats;
+	for (PhxMaterial& m : tmats) {
+		m.iData[0] = 0;
+		m.iData[1] = 1;
+	}
+	renderer.setMaterialsAndTextures(tmats, {rgba});
+	CHECK(renderer.lastStatus() == PHX_OK, "textures: %s", renderer.lastError().c_str());
+	renderer.addTexture(gray);
+	CHECK(renderer.lastStatus() == PHX_OK, "addTexture: %s", renderer.lastError().c_str());
+	RawImage bad = gray;
+	bad.bytesPerPixel = 3;
+	renderer.addTexture(bad);
+	CHECK(renderer.lastStatus() == PHX_ERR_INVALID_ARGUMENT, "3-byte texture accepted");
+	const BVH& sb = scene.bvh;
+	PhxScene* textured = nullptr;
+	PhxScene* plain = nullptr;
+	CHECK(phx_scene_upload(ctx, sb.nodes.data(), uint32_t(sb.nodes.size()),
+	                       sb.triangleVerts.data(), sb.triangleDatas.data(),
+	                       uint32_t(sb.triangleVerts.size()), tmats.data(), numMats,
+	                       scene.sphereLights.data(), numLights, &textured) == PHX_OK, "upload");
+	CHECK(phx_scene_upload(ctx, sb.nodes.data(), uint32_t(sb.nodes.size()),
+	                       sb.triangleVerts.data(), sb.triangleDatas.data(),
+	                       uint32_t(sb.triangleVerts.size()), tmats.data(), numMats,
+	                       scene.sphereLights.data(), numLights, &plain) == PHX_OK, "upload");
+	const PhxTexture tx[2] = {{rgba.imgData.data(), 8, 4, 4, 32}, {gray.imgData.data(), 3, 3, 1, 4}};
+	CHECK(phx_scene_set_textures(ctx, textured, tx, 2) == PHX_OK, "set textures");
+	std::vector<float> c(a.size());
+	PhxRenderer* other = nullptr;  // its own TAA history
+	CHECK(phx_renderer_create(ctx, w, h, &other) == PHX_OK, "second renderer");
+	for (uint32_t f = 0; f < 2; f++) {
+		RenderResult res = renderer.render({}, {});
+		CHECK(res.colorDevice, "textured render %u: %s", f, renderer.lastError().c_str());
+		CHECK(phx_copy_to_host(ctx, a.data(), res.colorDevice, a.size() * 4) == PHX_OK, "copy");
+		const uint32_t flags = PHX_FRAME_TAA | (f == 0 ? PHX_FRAME_TAA_RESET : 0u);
+		CHECK(phx_render_frame(ctx, direct, textured, &def, f, 11, flags, b.data()) == PHX_OK,
+		      "direct textured frame %u: %s", f, phx_last_error());
+		CHECK(std::memcmp(a.data(), b.data(), a.size() * 4) == 0, "textured frame %u differs", f);
+		CHECK(phx_render_frame(ctx, other, plain, &def, f, 11, flags, c.data()) == PHX_OK,
+		      "direct untextured frame %u: %s", f, phx_last_error());
+		CHECK(std::memcmp(a.data(), c.data(), a.size() * 4) != 0, "textures had no effect");
+	}
+	phx_scene_release(textured);
+	phx_scene_release(plain);
+	phx_renderer_release(other);
+	phx_renderer_release(direct);
 
 	// Kernel-level API: primary rays on the device, the three launch functions.
 	std::printf("hiptracer_check: launch functions\n");
