@@ -154,8 +154,9 @@ def batched_form(ctx, scene, pools, ev, reps: int = 3, plan_launch=None) -> dict
 
     plan = W.plan_for(ctx, scene, bs, track=True)
     if plan_launch:
-        grid, block, top, refill = plan_launch
-        plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill)
+        grid, block, top, refill = plan_launch[:4]
+        plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill,
+                               helper_waves=plan_launch[4] if len(plan_launch) > 4 else 0)
         tuned = {"gridBlocksClosest": grid, "blockThreads": block, "topNodes": top,
                  "refillMin": refill}
     else:
@@ -311,7 +312,7 @@ def main() -> int:
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
+    ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL[,HELPERS]",
                     help="fixed persistent launch shape instead of the autotuner's (profiling)")
     ap.add_argument("--plan-launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
                     help="fixed launch shape of the fused 256-batch plan (profiling)")
@@ -363,9 +364,11 @@ def main() -> int:
 
     launch = {"kernel": "persistent", **ctx.launch_params()}
     if args.launch:
-        grid, block, top, refill = (int(x) for x in args.launch.split(","))
+        vals = [int(x) for x in args.launch.split(",")]
+        grid, block, top, refill = vals[:4]
         ctx.set_launch_params(grid_closest=grid, grid_anyhit=grid, refill_min=refill,
-                              block_threads=block, top_nodes=top)
+                              block_threads=block, top_nodes=top,
+                              helper_waves=vals[4] if len(vals) > 4 else 0)
         launch = {"kernel": "persistent", **ctx.launch_params()}
         log(f"fixed launch: {launch}")
     elif not args.no_autotune:

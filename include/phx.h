@@ -198,13 +198,16 @@ void* phx_context_stream(PhxContext* ctx);
 uint32_t phx_context_num_cus(PhxContext* ctx);
 
 /* Launch shape of the persistent traversal kernels on this context: grid size in blocks for
- * closest-hit and any-hit, block size (threads, a multiple of 64 up to 1024), the number of top
- * tree nodes each block keeps in LDS (0..2048; the upload lays out the top of the tree breadth
- * first), and the refill threshold (a wave refills its idle lanes once at least `refillMin` of its
- * 64 lanes are idle; 1..64). phx_create's defaults: one block of 512 threads per CU, no treelet,
- * refill 8; phx_autotune picks them per scene and ray mix. In phx_set_launch_params a zero field (0xFFFFFFFF for
- * topNodes) keeps the current value; combinations that do not fit a CU are rejected. Results do
- * not depend on these parameters. */
+ * closest-hit and any-hit, block size (lane-mode threads, a multiple of 64 up to 1024), the number
+ * of top tree nodes each block keeps in LDS (0..2048; the upload lays out the top of the tree
+ * breadth first), the refill threshold (a wave refills its idle lanes once at least `refillMin` of
+ * its 64 lanes are idle; 1..64), and helper waves: extra waves per block that trace no ray of
+ * their own and take over, one wave per ray, the last rays the block's lane-mode waves hand out
+ * (blockThreads + 64 * helperWaves <= 1024). phx_create's defaults: one block of 512 threads per
+ * CU, no treelet, refill 8, no helper waves; phx_autotune picks them per scene and ray mix. In
+ * phx_set_launch_params a zero field (0xFFFFFFFF for topNodes and helperWaves) keeps the current
+ * value; combinations that do not fit a CU are rejected. In phx_plan_set_launch_params
+ * 0xFFFFFFFF helper waves means none. Results do not depend on these parameters. */
 typedef struct PhxLaunchParams {
 	uint32_t gridBlocksClosest;
 	uint32_t gridBlocksAnyhit;
@@ -212,6 +215,7 @@ typedef struct PhxLaunchParams {
 	uint32_t blockThreads;
 	uint32_t topNodes;
 	uint32_t residentGridBlocks; /* read only: blocks of this shape the device holds at once */
+	uint32_t helperWaves;
 } PhxLaunchParams;
 
 phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out);

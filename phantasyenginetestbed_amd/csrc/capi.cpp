@@ -247,7 +247,7 @@ phx_status phx_create(int device, PhxContext** outCtx)
 	// fastest or close to it for incoherent closest-hit and any-hit casts on MI355X;
 	// phx_autotune refines it per scene and ray mix.
 	uint32_t bpc = 0;
-	e = persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, &bpc);
+	e = persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, 0, &bpc);
 	if (e == hipSuccess && bpc == 0) e = hipErrorInvalidConfiguration;
 	ctx->gridClosest = ctx->numCUs;
 	ctx->gridAnyhit = ctx->numCUs;
@@ -344,8 +344,9 @@ phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out)
 	out->blockThreads = ctx->blockThreads;
 	out->topNodes = ctx->topNodes;
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, &bpc);
+	persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, ctx->helperWaves, &bpc);
 	out->residentGridBlocks = ctx->numCUs * bpc;
+	out->helperWaves = ctx->helperWaves;
 	return ok();
 }
 
@@ -356,16 +357,21 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
 	if (!p) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: params is NULL");
 	const uint32_t block = p->blockThreads ? p->blockThreads : ctx->blockThreads;
 	const uint32_t top = p->topNodes != 0xFFFFFFFFu ? p->topNodes : ctx->topNodes;
+	const uint32_t helpers = p->helperWaves != 0xFFFFFFFFu ? p->helperWaves : ctx->helperWaves;
 	if (block % 64 != 0 || block > 1024)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: blockThreads %u", block);
 	if (top > kTopCap)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: topNodes above %u", kTopCap);
+	if (block + 64u * helpers > 1024u)
+		return fail(PHX_ERR_INVALID_ARGUMENT,
+		            "phx_set_launch_params: %u threads and %u helper waves exceed 1024 threads",
+		            block, helpers);
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(block, top, &bpc);
+	persistentBlocksPerCU(block, top, helpers, &bpc);
 	if (bpc == 0)
 		return fail(PHX_ERR_INVALID_ARGUMENT,
-		            "phx_set_launch_params: %u threads with a %u-node treelet do not fit a CU",
-		            block, top);
+		            "phx_set_launch_params: %u threads (+%u helper waves) with a %u-node treelet do "
+		            "not fit a CU", block, helpers, top);
 	const uint32_t maxGrid = ctx->spillLanes / block;
 	if (p->gridBlocksClosest > maxGrid || p->gridBlocksAnyhit > maxGrid)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: grid above %u blocks",
@@ -373,10 +379,12 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
 	if (p->refillMin > 64)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: refillMin above 64");
 	// A new block or treelet size without an explicit grid gets the resident grid of that shape.
-	const bool reshaped = block != ctx->blockThreads || top != ctx->topNodes;
+	const bool reshaped =
+		block != ctx->blockThreads || top != ctx->topNodes || helpers != ctx->helperWaves;
 	const uint32_t resident = std::min(ctx->numCUs * bpc, maxGrid);
 	ctx->blockThreads = block;
 	ctx->topNodes = top;
+	ctx->helperWaves = helpers;
 	if (p->gridBlocksClosest)
 		ctx->gridClosest = p->gridBlocksClosest;
 	else if (reshaped)
@@ -824,7 +832,8 @@ phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t nu
 	p->launch = traceLaunch(ctx, 0, false);
 	p->launch.topNodes = 0;
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(p->launch.blockThreads, 0, &bpc);
+	p->launch.helperWaves = 0;
+	persistentBlocksPerCU(p->launch.blockThreads, 0, 0, &bpc);
 	p->launch.gridBlocks = std::max(1u, std::min(ctx->numCUs * bpc,
 	                                             ctx->spillLanes / p->launch.blockThreads));
 	p->hostDescs = descs;
@@ -922,12 +931,16 @@ namespace {
 phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, bool plan)
 {
 	const uint32_t block = p.blockThreads;
+	const uint32_t helpers = p.helperWaves == 0xFFFFFFFFu ? 0u : p.helperWaves;
 	if (block == 0 || block % 64 != 0 || block > 1024)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: blockThreads %u", block);
 	if (p.topNodes > kTopCap)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: topNodes above %u", kTopCap);
+	if (block + 64u * helpers > 1024u)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: %u threads and %u helper waves",
+		            block, helpers);
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(block, p.topNodes, &bpc);
+	persistentBlocksPerCU(block, p.topNodes, helpers, &bpc);
 	if (bpc == 0) return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: shape does not fit a CU");
 	const uint32_t maxGrid = ctx->spillLanes / block;
 	const uint32_t grid = plan ? p.gridBlocksClosest : p.gridBlocksClosest;
@@ -937,6 +950,7 @@ phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, b
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: refillMin %u", p.refillMin);
 	L.blockThreads = block;
 	L.topNodes = p.topNodes;
+	L.helperWaves = helpers;
 	L.gridBlocks = grid;
 	L.refillMin = p.refillMin;
 	L.spillLanes = ctx->spillLanes;
@@ -952,29 +966,33 @@ struct Candidate {
 };
 
 // Candidate shapes: block 256/512, treelet 0/512/1024 nodes, a quarter, half or all of the
-// resident grid, refill threshold 8 or 32 -- plus the one-ray-per-lane kernel for single batches.
+// resident grid, refill threshold 8 or 32, and for single batches 0 or 4 helper waves -- plus the
+// one-ray-per-lane kernel for single batches.
 std::vector<Candidate> candidates(PhxContext* ctx, bool plan)
 {
 	std::vector<Candidate> out;
-	for (uint32_t block : {256u, 512u})
-		for (uint32_t top : {0u, 512u, 1024u}) {
-			uint32_t bpc = 0;
-			persistentBlocksPerCU(block, top, &bpc);
-			if (bpc == 0) continue;
-			const uint32_t resident = std::min(ctx->numCUs * bpc, ctx->spillLanes / block);
-			for (uint32_t div : {4u, 2u, 1u})
-				for (uint32_t refill : {8u, 32u}) {
-					Candidate c;
-					c.params = PhxLaunchParams{std::max(1u, resident / div),
-					                           std::max(1u, resident / div), refill, block, top,
-					                           resident};
-					if (shapeFor(ctx, c.params, c.launch, plan) == PHX_OK) out.push_back(c);
-				}
-		}
+	for (uint32_t helpers : {0u, 4u}) {
+		if (plan && helpers) continue;
+		for (uint32_t block : {256u, 512u})
+			for (uint32_t top : {0u, 512u, 1024u}) {
+				uint32_t bpc = 0;
+				persistentBlocksPerCU(block, top, helpers, &bpc);
+				if (bpc == 0) continue;
+				const uint32_t resident = std::min(ctx->numCUs * bpc, ctx->spillLanes / block);
+				for (uint32_t div : {4u, 2u, 1u})
+					for (uint32_t refill : {8u, 32u}) {
+						Candidate c;
+						c.params = PhxLaunchParams{std::max(1u, resident / div),
+						                           std::max(1u, resident / div), refill, block, top,
+						                           resident, helpers};
+						if (shapeFor(ctx, c.params, c.launch, plan) == PHX_OK) out.push_back(c);
+					}
+			}
+	}
 	if (!plan) {
 		Candidate c;
 		c.launch.kernel = TraceKernel::Simple;
-		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0};
+		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0};
 		out.push_back(c);
 	}
 	return out;
@@ -1050,6 +1068,7 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 		else
 			ctx->gridClosest = best.launch.gridBlocks;
 		ctx->refillMin = best.launch.refillMin;
+		ctx->helperWaves = best.launch.helperWaves;
 	}
 	if (chosen) {
 		*chosen = best.params;

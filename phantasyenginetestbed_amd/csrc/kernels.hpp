@@ -48,6 +48,7 @@ struct TraceLaunch {
 	uint32_t topNodes = 0;       // treelet nodes held in LDS (capped by the scene's topCount)
 	uint32_t refillMin = 8;      // refill a wave once this many of its lanes are idle
 	uint32_t spillLanes = 0;     // lanes the spill area is sized for
+	uint32_t helperWaves = 0;    // extra waves per block that only trace offered rays solo
 	uint32_t* work = nullptr;
 	uint32_t* spill = nullptr;
 };
@@ -57,7 +58,8 @@ uint32_t persistentSpillWords(uint32_t lanes);
 // Dynamic LDS bytes of a persistent block.
 uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes);
 // Resident persistent blocks per CU for a block size and treelet size (0 if it does not fit).
-hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t* blocks);
+hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
+                                 uint32_t* blocks);
 
 // Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
 // counters of the batch are added to counters[0..4] (device memory).

@@ -70,6 +70,7 @@ struct PhxContext {
 	uint32_t blockThreads = 512;
 	uint32_t topNodes = 0;
 	uint32_t refillMin = 8;
+	uint32_t helperWaves = 0;
 	uint32_t spillLanes = 0;
 	// Staging buffers for calls that pass host pointers, grown on demand.
 	void* stageIn = nullptr;
@@ -142,6 +143,7 @@ inline TraceLaunch traceLaunch(const PhxContext* ctx, uint32_t flags, bool anyhi
 	L.topNodes = ctx->topNodes;
 	L.refillMin = ctx->refillMin;
 	L.spillLanes = ctx->spillLanes;
+	L.helperWaves = ctx->helperWaves;
 	L.work = ctx->work;
 	L.spill = ctx->spill;
 	return L;

@@ -355,7 +355,7 @@ phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t num
 	s->shape = traceLaunch(ctx, 0, false);
 	s->shape.topNodes = 0;
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(s->shape.blockThreads, 0, &bpc);
+	persistentBlocksPerCU(s->shape.blockThreads, 0, 0, &bpc);
 	s->shape.gridBlocks = std::max(1u, ctx->numCUs * bpc);
 	s->shape.spillLanes = s->shape.gridBlocks * s->shape.blockThreads;
 	s->slots.resize(numSlots);

@@ -820,8 +820,8 @@ __device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 // ---- handing solo rays to the block's other waves ------------------------------------------------
 //
 // A wave that goes solo keeps one ray and offers the others in LDS slots; waves of the block that
-// have nothing left claim them and trace them solo, so a wave's last rays run side by side
-// instead of one after another. The stack of an offered ray stays where it is (its owner's LDS
+// have nothing left -- and the block's helper waves, which trace nothing else -- claim them and
+// trace them solo, so a wave's last rays run side by side instead of one after another. The stack of an offered ray stays where it is (its owner's LDS
 // column and spill column, which the owner no longer writes). Area layout (32-bit words):
 // [0] slots allocated, [1] slots claimed, [2] waves that may still offer, [3] waves waiting to
 // claim, then
@@ -832,9 +832,10 @@ __device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 constexpr uint32_t kHandoffPerWave = kMaxSoloRays - 1u;  // a wave offers at most this many rays
 constexpr uint32_t kSlotWords = 24;
 
-__host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t blockThreads)
+// `laneThreads`: the block's lane-mode threads (helper waves offer nothing).
+__host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t laneThreads)
 {
-	return 16u + (blockThreads / 64u) * kHandoffPerWave * kSlotWords * 4u;
+	return 16u + (laneThreads / 64u) * kHandoffPerWave * kSlotWords * 4u;
 }
 
 __device__ __forceinline__ uint32_t lds_load(const uint32_t* p)
@@ -851,16 +852,17 @@ __device__ __forceinline__ uint32_t solo_threshold(const uint32_t* H)
 
 // Every thread of the block calls this before the block's first barrier: resets the counters and
 // every slot's ready word (LDS holds whatever an earlier block left there, and a stale ready word
-// would let a claimant read a slot before its owner writes it).
-__device__ __forceinline__ void handoff_init(uint32_t* H)
+// would let a claimant read a slot before its owner writes it). `laneWaves`: the waves that run
+// the lane-mode loop and may offer rays.
+__device__ __forceinline__ void handoff_init(uint32_t* H, uint32_t laneWaves)
 {
 	if (threadIdx.x == 0) {
 		H[0] = 0u;
 		H[1] = 0u;
-		H[2] = blockDim.x / 64u;
+		H[2] = laneWaves;
 		H[3] = 0u;
 	}
-	const uint32_t slots = (blockDim.x / 64u) * kHandoffPerWave;
+	const uint32_t slots = laneWaves * kHandoffPerWave;
 	for (uint32_t i = threadIdx.x; i < slots; i += blockDim.x) H[4u + i * kSlotWords + 22u] = 0u;
 }
 
@@ -958,34 +960,60 @@ __device__ __forceinline__ SoloStart solo_start_slot(const uint32_t* H, uint32_t
 	S.kind = v[19];
 	const uint32_t wave = v[20] >> 6, lane = v[20] & 63u;
 	S.ldsCol = ldsStacks + wave * uint32_t(kLdsRows) * 64u + lane;
-	S.spillCol = spill + (blockIdx.x * blockDim.x + wave * 64u + lane);
+	// Spill columns are numbered over the grid's lane-mode threads (`stride` of them).
+	S.spillCol = spill + (blockIdx.x * (stride / gridDim.x) + wave * 64u + lane);
 	S.stride = stride;
 	batch = v[21];
 	return S;
 }
 
+// The block's last `helpers` waves (helper waves) run no lane-mode loop: they wait for offered rays
+// and trace them solo, at raised priority so that the waves still in lane mode on their SIMD delay
+// them little. A block with helpers has more waves for a solo tail than waves in lane mode.
+__device__ __forceinline__ bool is_helper(uint32_t helpers)
+{
+	return (threadIdx.x >> 6) >= blockDim.x / 64u - helpers;
+}
+
 // Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Without a
 // treelet (kTreelet false) every record comes from global memory and no LDS path is compiled.
+// LDS stacks and spill columns belong to the lane-mode threads only.
 template <bool kAnyHit, bool kCount, bool kTreelet>
 __device__ __forceinline__ void persistent_trace(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ inLight,
 	uint32_t n, uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	bool finiteBounds, uint32_t topCount, unsigned long long* __restrict__ counters)
+	bool finiteBounds, uint32_t topCount, uint32_t helpers,
+	unsigned long long* __restrict__ counters)
 {
 	extern __shared__ float4 ldsRaw[];
+	const uint32_t laneWaves = blockDim.x / 64u - helpers;
+	const uint32_t laneThreads = laneWaves * 64u;
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
-	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
+	float4* treelet = ldsRaw + laneWaves * uint32_t(kLdsRows) * 64u / 4u;
 	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
 	if (kTreelet)
 		for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
-	handoff_init(handoff);
+	handoff_init(handoff, laneWaves);
 	__syncthreads();
+
+	const SoloScene S = {nodes, tris, rays,
+	                     kAnyHit ? static_cast<void*>(inLight) : static_cast<void*>(hits)};
+	const uint32_t stride = gridDim.x * laneThreads;
+	if (!kCount && is_helper(helpers)) {
+		__builtin_amdgcn_s_setprio(3);
+		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
+			uint32_t batch;
+			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, stride, batch));
+		}
+		retire_wave(work);
+		return;
+	}
 
 	LaneStack st;
 	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
-	st.stride = gridDim.x * blockDim.x;
-	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
+	st.stride = stride;
+	st.spill = spill + (blockIdx.x * laneThreads + threadIdx.x);
 	st.lane = __lane_id();
 	st.sp = 0;
 
@@ -1012,8 +1040,6 @@ __device__ __forceinline__ void persistent_trace(
 	if (!kCount) {
 		// Solo: keep the first busy lane's ray, offer the others to the block, then trace offered
 		// rays until none is left (the loads still in flight land in A, which is not read again).
-		const SoloScene S = {nodes, tris, rays,
-		                     kAnyHit ? static_cast<void*>(inLight) : static_cast<void*>(hits)};
 		const uint64_t busy = next == kStepSolo ? __ballot(L.kind != kRecIdle) : 0ull;
 		handoff_offer(handoff, busy & (busy - 1u), L, st, 0u);
 		if (busy != 0) solo_run<kAnyHit>(S, solo_start_lane(L, st, uint32_t(__builtin_ctzll(busy))));
@@ -1030,10 +1056,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
 	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	uint32_t finiteBounds, uint32_t topCount, unsigned long long* __restrict__ counters)
+	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers,
+	unsigned long long* __restrict__ counters)
 {
 	persistent_trace<false, kCount, kTreelet>(nodes, tris, rays, hits, nullptr, n, work, spill,
-	                                          refillMin, finiteBounds != 0, topCount, counters);
+	                                          refillMin, finiteBounds != 0, topCount, helpers,
+	                                          counters);
 }
 
 template <bool kTreelet>
@@ -1041,10 +1069,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_anyhit_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint8_t* __restrict__ inLight, uint32_t n,
 	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	uint32_t finiteBounds, uint32_t topCount)
+	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers)
 {
 	persistent_trace<true, false, kTreelet>(nodes, tris, rays, nullptr, inLight, n, work, spill,
-	                                        refillMin, finiteBounds != 0, topCount, nullptr);
+	                                        refillMin, finiteBounds != 0, topCount, helpers,
+	                                        nullptr);
 }
 
 // ---- fused multi-batch launch ---------------------------------------------------------------------
@@ -1231,14 +1260,17 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
                                               uint32_t* __restrict__ work,
                                               uint32_t* __restrict__ spill, uint32_t refillMin,
                                               const float4* __restrict__ treeletNodes,
-                                              uint32_t topCount, BatchClock* __restrict__ clock)
+                                              uint32_t topCount, uint32_t helpers,
+                                              BatchClock* __restrict__ clock)
 {
 	extern __shared__ float4 ldsRaw[];
+	const uint32_t laneWaves = blockDim.x / 64u - helpers;
+	const uint32_t laneThreads = laneWaves * 64u;
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
-	float4* treelet = ldsRaw + (blockDim.x / 64u) * uint32_t(kLdsRows) * 64u / 4u;
+	float4* treelet = ldsRaw + laneWaves * uint32_t(kLdsRows) * 64u / 4u;
 	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
 	unsigned long long* blockFinish =
-		reinterpret_cast<unsigned long long*>(handoff + handoff_bytes(blockDim.x) / 4u);
+		reinterpret_cast<unsigned long long*>(handoff + handoff_bytes(laneThreads) / 4u);
 	uint32_t* blockDone = reinterpret_cast<uint32_t*>(blockFinish + (kTrack ? numBatches : 0u));
 	uint32_t* offsets = blockDone + (kTrack ? numBatches : 0u);
 	if (kTrack && threadIdx.x == 0) atomicMin(&clock->start, wall_clock64());
@@ -1249,19 +1281,38 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 			blockFinish[i] = 0ull;
 			blockDone[i] = 0u;
 		}
-	handoff_init(handoff);
+	handoff_init(handoff, laneWaves);
 	__syncthreads();
-
-	LaneStack st;
-	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
-	st.stride = gridDim.x * blockDim.x;
-	st.spill = spill + (blockIdx.x * blockDim.x + threadIdx.x);
-	st.lane = __lane_id();
-	st.sp = 0;
 
 	const MultiArgs P = {descs, offsets, numBatches, total, work, refillMin,
 	                     kTreelet ? topCount : 0u, treeletNodes, treelet, blockFinish, blockDone,
 	                     handoff};
+	const uint32_t stride = gridDim.x * laneThreads;
+	// Traces the ray of a claimed slot solo.
+	auto runSlot = [&](uint32_t slot) {
+		uint32_t b;
+		const SoloStart S0 = solo_start_slot(handoff, slot, ldsStacks, spill, stride, b);
+		const BatchDesc d = descs[b];
+		const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
+		if (d.kind != 0u)
+			solo_run<true>(S, S0);
+		else
+			solo_run<false>(S, S0);
+		if (kTrack && __lane_id() == 0u) track_done(P, b);
+	};
+	const bool helper = is_helper(helpers);
+	if (helper) {
+		__builtin_amdgcn_s_setprio(3);
+		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff))
+			runSlot(slot);
+	}
+
+	LaneStack st;
+	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
+	st.stride = stride;
+	st.spill = spill + (blockIdx.x * laneThreads + threadIdx.x);
+	st.lane = __lane_id();
+	st.sp = 0;
 	WaveQueue q;
 	MultiTracer M;
 	Tracer& L = M.t;
@@ -1282,28 +1333,20 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 	M.inTreelet = false;
 	const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 	Rec A = {z, z, z, z, z, z}, B = A;
-	int next = kStepGo;
+	int next = helper ? kStepDone : kStepGo;
 	while (next == kStepGo) {
 		pipe_step_multi<kTreelet, kTrack, false>(P, M, st, q, A, B);
 		next = pipe_step_multi<kTreelet, kTrack, true>(P, M, st, q, B, A);
 	}
-	{
+	if (!helper) {
 		// Solo: keep the first busy lane's ray, offer the others to the block, then trace offered
 		// rays until none is left.
 		const uint64_t busy = next == kStepSolo ? __ballot(L.kind != kRecIdle) : 0ull;
 		handoff_offer(handoff, busy & (busy - 1u), L, st, M.batch);
-		const uint32_t lsel = busy != 0 ? uint32_t(__builtin_ctzll(busy)) : 0u;
-		uint32_t slot = kInvalid;
-		bool have = busy != 0;
-		while (have) {
-			uint32_t b;
-			SoloStart S0;
-			if (slot == kInvalid) {
-				b = lane_u(M.batch, lsel);
-				S0 = solo_start_lane(L, st, lsel);
-			} else {
-				S0 = solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, b);
-			}
+		if (busy != 0) {
+			const uint32_t lsel = uint32_t(__builtin_ctzll(busy));
+			const uint32_t b = lane_u(M.batch, lsel);
+			const SoloStart S0 = solo_start_lane(L, st, lsel);
 			const BatchDesc d = descs[b];
 			const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
 			if (d.kind != 0u)
@@ -1311,21 +1354,9 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 			else
 				solo_run<false>(S, S0);
 			if (kTrack && __lane_id() == 0u) track_done(P, b);
-			slot = handoff_claim(handoff);
-			have = slot != kInvalid;
 		}
-		if (busy == 0)
-			for (slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
-				uint32_t b;
-				const SoloStart S0 = solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, b);
-				const BatchDesc d = descs[b];
-				const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
-				if (d.kind != 0u)
-					solo_run<true>(S, S0);
-				else
-					solo_run<false>(S, S0);
-				if (kTrack && __lane_id() == 0u) track_done(P, b);
-			}
+		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff))
+			runSlot(slot);
 	}
 	if (kTrack) {
 		__syncthreads();
@@ -1357,10 +1388,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	const BatchDesc* __restrict__ descs, const uint32_t* __restrict__ offsetsGlobal,
 	uint32_t numBatches, uint32_t total, uint32_t* __restrict__ work,
 	uint32_t* __restrict__ spill, uint32_t refillMin, const float4* __restrict__ treeletNodes,
-	uint32_t topCount, BatchClock* __restrict__ clock)
+	uint32_t topCount, uint32_t helpers, BatchClock* __restrict__ clock)
 {
 	trace_batches<kTreelet, kTrack>(descs, offsetsGlobal, numBatches, total, work, spill, refillMin,
-	                                treeletNodes, topCount, clock);
+	                                treeletNodes, topCount, helpers, clock);
 }
 
 // ---- device self-check of the fast reciprocal -----------------------------------------------------
@@ -1388,17 +1419,18 @@ __global__ __launch_bounds__(kBlock) void k_check_reciprocal(bool exhaustive, ui
 
 uint32_t gridFor(uint32_t n, uint32_t block) { return n / block + (n % block != 0u); }
 
-// Persistent launch shape: grid capped by the work and by the spill area's lane count.
+// Persistent launch shape: grid capped by the work and by the spill area's lane count (lane-mode
+// threads only); the block is the lane-mode threads plus the helper waves.
 void persistentShape(const TraceLaunch& L, const SceneView& s, uint32_t n, uint32_t& grid,
                      uint32_t& block, uint32_t& top, uint32_t& ldsBytes)
 {
-	block = L.blockThreads;
+	block = L.blockThreads + 64u * L.helperWaves;
 	top = L.topNodes < s.topCount ? L.topNodes : s.topCount;
-	grid = L.gridBlocks < gridFor(n, block) ? L.gridBlocks : gridFor(n, block);
-	const uint32_t maxGrid = L.spillLanes / block;
+	grid = L.gridBlocks < gridFor(n, L.blockThreads) ? L.gridBlocks : gridFor(n, L.blockThreads);
+	const uint32_t maxGrid = L.spillLanes / L.blockThreads;
 	if (grid > maxGrid) grid = maxGrid;
 	if (grid == 0) grid = 1;
-	ldsBytes = persistent_lds_bytes(block, top);
+	ldsBytes = persistent_lds_bytes(L.blockThreads, top);
 }
 
 }  // namespace
@@ -1413,13 +1445,14 @@ uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes)
 	return persistent_lds_bytes(blockThreads, topNodes);
 }
 
-hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t* blocks)
+hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
+                                 uint32_t* blocks)
 {
 	int a = 0;
 	const void* fn = topNodes ? reinterpret_cast<const void*>(k_closest_persistent<false, true>)
 	                          : reinterpret_cast<const void*>(k_closest_persistent<false, false>);
 	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		&a, fn, int(blockThreads), persistent_lds_bytes(blockThreads, topNodes));
+		&a, fn, int(blockThreads + 64u * helperWaves), persistent_lds_bytes(blockThreads, topNodes));
 	*blocks = uint32_t(a > 0 ? a : 0);
 	return e;
 }
@@ -1452,11 +1485,14 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 		return hipGetLastError();
 	}
 	uint32_t grid, block, top, lds;
-	persistentShape(L, s, n, grid, block, top, lds);
+	TraceLaunch shape = L;
+	if (counters) shape.helperWaves = 0;  // the counting variant traces every ray in lane mode
+	persistentShape(shape, s, n, grid, block, top, lds);
 	auto kernel = counters ? (top ? k_closest_persistent<true, true> : k_closest_persistent<true, false>)
 	                       : (top ? k_closest_persistent<false, true> : k_closest_persistent<false, false>);
 	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, nodes, tris, r4, h4, n,
-	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, counters);
+	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, shape.helperWaves,
+	                   counters);
 	return hipGetLastError();
 }
 
@@ -1475,8 +1511,8 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	uint32_t grid, block, top, lds;
 	persistentShape(L, s, n, grid, block, top, lds);
 	hipLaunchKernelGGL(top ? k_anyhit_persistent<true> : k_anyhit_persistent<false>, dim3(grid),
-	                   dim3(block), lds, stream, nodes, tris, r4,
-	                   inLight, n, L.work, L.spill, L.refillMin, s.finiteBounds, top);
+	                   dim3(block), lds, stream, nodes, tris, r4, inLight, n, L.work, L.spill,
+	                   L.refillMin, s.finiteBounds, top, L.helperWaves);
 	return hipGetLastError();
 }
 
@@ -1492,17 +1528,17 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
                          uint32_t treeletCount, BatchClock* clock, hipStream_t stream)
 {
 	if (total == 0) return hipSuccess;
-	const uint32_t block = L.blockThreads;
+	const uint32_t block = L.blockThreads;  // lane-mode threads; helper waves come on top
 	const uint32_t top = L.topNodes < treeletCount ? L.topNodes : treeletCount;
 	uint32_t grid = L.gridBlocks < gridFor(total, block) ? L.gridBlocks : gridFor(total, block);
 	if (grid > L.spillLanes / block) grid = L.spillLanes / block;
 	if (grid == 0) grid = 1;
 	auto kernel = clock ? (top ? k_trace_batches<true, true> : k_trace_batches<false, true>)
 	                    : (top ? k_trace_batches<true, false> : k_trace_batches<false, false>);
-	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block),
+	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block + 64u * L.helperWaves),
 	                   batchLdsBytes(block, top, numBatches, clock != nullptr), stream, descs,
 	                   offsets, numBatches, total, L.work, L.spill, L.refillMin, treeletNodes, top,
-	                   clock);
+	                   L.helperWaves, clock);
 	return hipGetLastError();
 }
 

@@ -360,13 +360,17 @@ def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
     assert tested == 2 * 4 * (1 << 23) and bad == 0
 
 
-@pytest.mark.parametrize("shape", [(512, 256, 512, 8), (300, 256, 1024, 32), (128, 1024, 0, 16)],
-                         ids=["b256-treelet512", "b256-treelet1024", "b1024-notreelet"])
+@pytest.mark.parametrize("shape", [(512, 256, 512, 8, 0), (300, 256, 1024, 32, 0),
+                                   (128, 1024, 0, 16, 0), (256, 512, 0, 32, 4),
+                                   (64, 256, 512, 8, 12)],
+                         ids=["b256-treelet512", "b256-treelet1024", "b1024-notreelet",
+                              "b512-helpers4", "b256-treelet512-helpers12"])
 def test_launch_shapes_match_oracle(ctx, c2, shape):
-    """Other launch shapes -- the LDS-treelet kernels, larger blocks -- give the oracle's hits,
-    single-batch and fused."""
-    grid, block, top, refill = shape
-    say(f"[gpu] launch shape grid {grid} block {block} treelet {top} refill {refill}")
+    """Other launch shapes -- the LDS-treelet kernels, larger blocks, helper waves that trace the
+    rays a block's lane-mode waves hand out -- give the oracle's hits, single-batch and fused."""
+    grid, block, top, refill, helpers = shape
+    say(f"[gpu] launch shape grid {grid} block {block} treelet {top} refill {refill} "
+        f"helpers {helpers}")
     b2, s2 = c2
     rng = np.random.default_rng(13)
     n = 40000
@@ -384,7 +388,8 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
     bufs = []
     try:
         ctx.set_launch_params(grid_closest=grid, grid_anyhit=grid, refill_min=refill,
-                              block_threads=block, top_nodes=top)
+                              block_threads=block, top_nodes=top, helper_waves=helpers)
+        assert ctx.launch_params()["helperWaves"] == helpers
         assert_same_hits(ctx.cast_closest(s2, rays), ref, f"closest, shape {shape}")
         assert (ctx.cast_anyhit(s2, short) == ref_lit).all()
         rb, sb = ctx.alloc(n * 32).upload(rays), ctx.alloc(n * 32).upload(short)
@@ -393,7 +398,8 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
         plan = phx.Plan(ctx, [(s2, rb.ptr, hb.ptr, n, phx.BATCH_CLOSEST),
                               (s2, sb.ptr, lb.ptr, n, phx.BATCH_ANYHIT)])
         try:
-            plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill)
+            plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill,
+                                   helper_waves=helpers)
             plan.launch()
             assert_same_hits(hb.download(phx.HIT, n), ref, f"fused, shape {shape}")
             assert (lb.download(np.uint8, n) == ref_lit).all()
@@ -404,4 +410,5 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
             b.free()
         ctx.set_launch_params(grid_closest=old["gridBlocksClosest"],
                               grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
-                              block_threads=old["blockThreads"], top_nodes=old["topNodes"])
+                              block_threads=old["blockThreads"], top_nodes=old["topNodes"],
+                              helper_waves=old["helperWaves"])
