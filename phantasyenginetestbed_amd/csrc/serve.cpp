@@ -353,7 +353,10 @@ phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t num
 	s->flags = flags;
 	s->budget = budgetBytes;
 	s->shape = traceLaunch(ctx, 0, false);
+	// Continuous batching keeps every block busy: the resident grid of lane-mode blocks, no
+	// treelet, no helper waves (which would cut the blocks a CU holds).
 	s->shape.topNodes = 0;
+	s->shape.helperWaves = 0;
 	uint32_t bpc = 0;
 	persistentBlocksPerCU(s->shape.blockThreads, 0, 0, &bpc);
 	s->shape.gridBlocks = std::max(1u, ctx->numCUs * bpc);
