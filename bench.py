@@ -412,14 +412,16 @@ def main() -> int:
                    "traversal": "reference-order (bit-identical to castRay)",
                    "kernel": kernel,
                    "launch": {k: launch[k] for k in ("gridBlocksClosest", "blockThreads",
-                                                     "topNodes", "refillMin") if k in launch}},
+                                                     "topNodes", "refillMin", "helperWaves")
+                              if k in launch}},
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic() if kernel == "persistent" else None,
-                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 256x512 "
-                                       "no treelet, refill 32 (profiles/closest_persistent_bounce1M.md)",
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 256 "
+                                       "blocks x (512 threads + 4 helper waves), no treelet, "
+                                       "refill 8 (profiles/closest_persistent_bounce1M.md)",
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_batched:
