@@ -362,7 +362,7 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: blockThreads %u", block);
 	if (top > kTopCap)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: topNodes above %u", kTopCap);
-	if (block + 64u * helpers > 1024u)
+	if (helpers > 16u || block + 64u * helpers > 1024u)
 		return fail(PHX_ERR_INVALID_ARGUMENT,
 		            "phx_set_launch_params: %u threads and %u helper waves exceed 1024 threads",
 		            block, helpers);
@@ -936,7 +936,7 @@ phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, b
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: blockThreads %u", block);
 	if (p.topNodes > kTopCap)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: topNodes above %u", kTopCap);
-	if (block + 64u * helpers > 1024u)
+	if (helpers > 16u || block + 64u * helpers > 1024u)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: %u threads and %u helper waves",
 		            block, helpers);
 	uint32_t bpc = 0;

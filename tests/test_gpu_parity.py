@@ -360,6 +360,17 @@ def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
     assert tested == 2 * 4 * (1 << 23) and bad == 0
 
 
+def test_launch_params_reject_oversized_blocks(ctx):
+    """Lane-mode threads plus helper waves may not exceed a 1024-thread block; a rejected call
+    leaves the shape unchanged."""
+    old = ctx.launch_params()
+    for block, helpers in ((1024, 1), (512, 9), (64, 0x7FFFFFFF), (64, 0xFFFFFFFE)):
+        with pytest.raises(phx.PhxError) as e:
+            ctx.set_launch_params(block_threads=block, helper_waves=helpers)
+        assert e.value.status == phx.ERR_INVALID_ARGUMENT
+    assert ctx.launch_params() == old
+
+
 @pytest.mark.parametrize("shape", [(512, 256, 512, 8, 0), (300, 256, 1024, 32, 0),
                                    (128, 1024, 0, 16, 0), (256, 512, 0, 32, 4),
                                    (64, 256, 512, 8, 12)],

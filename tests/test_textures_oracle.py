@@ -184,3 +184,17 @@ def test_interpret_wraps_uv_outside_the_unit_square():
     b = _records(outside, mats, uvs, [img])
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert len(np.unique(a["albedo"], axis=0)) > 20
+
+
+def test_texture_arrays_describe_images():
+    """The ctypes PhxTexture records the binding builds: packed (h, w, bpp) and (h, w) arrays, and
+    (rows, width, bpp) with an explicit pitch."""
+    rgba = np.zeros((3, 5, 4), np.uint8)
+    gray = np.zeros((2, 7), np.uint8)
+    rows = np.zeros((4, 20), np.uint8)
+    arr, keep = phx.texture_arrays([rgba, gray, (rows, 6, 2)])
+    got = [(t.width, t.height, t.bytesPerPixel, t.pitch) for t in arr[:3]]
+    assert got == [(5, 3, 4, 20), (7, 2, 1, 7), (6, 4, 2, 20)]
+    assert all(arr[i].data == keep[i].ctypes.data for i in range(3))
+    empty, _ = phx.texture_arrays([])
+    assert len(empty) == 1  # one zeroed record, never read with a count of 0
