@@ -147,6 +147,11 @@ typedef struct PhxCameraDef {
  * This flag selects one ray per lane (rayCastNoPersistenceKernel, RayCastKernel.cu:416-572).
  * Results are identical either way. */
 #define PHX_KERNEL_ONE_RAY_PER_LANE 0x100u
+/* Packet traversal for coherent rays (camera rays in 8x8-tile order, shadow rays towards one
+ * light): a wave traces 64 consecutive rays over one shared stack, expanding the union of their
+ * nodes; each ray's nodes, triangle tests and hit are still castRay's. No counterpart in the
+ * reference. Not combinable with PHX_KERNEL_ONE_RAY_PER_LANE. */
+#define PHX_KERNEL_PACKET 0x200u
 
 /* Ray order of phx_gen_primary_rays. */
 #define PHX_RAYS_ROW_MAJOR 0u  /* id = y * w + x (CPUTracerRenderer.cpp:84-97)                   */
@@ -541,8 +546,9 @@ void phx_renderer_release(PhxRenderer* renderer);
  * any hit, shadow logic; one bounce ray per 2x2 pixel block (createSecondaryRaysKernel), closest
  * hit, shading x path throughput with shadow rays (materialKernel), any hit, shadow logic added
  * at half resolution; then, with PHX_FRAME_TAA, the TAA resolve (taa.frag:25-71 for a static
- * camera, FP32 history, 3x3 clamp). The primary shadow rays and the bounce rays are traced in one
- * fused persistent launch. `rgba` (width*height*4 floats, row-major; host or device with
+ * camera, FP32 history, 3x3 clamp). Shadow rays (coherent: a pixel tile's rays towards one
+ * light) are traced as packets (PHX_KERNEL_PACKET), camera and bounce rays by the persistent
+ * kernel. `rgba` (width*height*4 floats, row-major; host or device with
  * PHX_MEM_DEVICE; may be NULL) receives the output. Random numbers: PCG32 streams keyed by
  * (seed, frameIndex, pass, record). The frame's kernels read the per-frame values (jitter, frame
  * index, seed, TAA reset and history buffer) from a small device block written before the frame,

@@ -635,6 +635,8 @@ phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRay
 	const uint32_t mode = flags & PHX_TRAVERSAL_MASK;
 	if (mode != PHX_TRAVERSAL_REFERENCE_ORDER)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: unknown traversal mode 0x%x", mode);
+	if ((flags & PHX_KERNEL_ONE_RAY_PER_LANE) && (flags & PHX_KERNEL_PACKET))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: two launch models requested");
 	SceneView sv = view(scene);
 	hipStream_t stream = ctx->stream;
 	const TraceLaunch L = traceLaunch(ctx, flags, false);
@@ -653,6 +655,8 @@ phx_status phx_cast_anyhit(PhxContext* ctx, const PhxScene* scene, const PhxRayI
 	if (st != PHX_OK) return st;
 	st = checkScene(ctx, scene);
 	if (st != PHX_OK) return st;
+	if ((flags & PHX_KERNEL_ONE_RAY_PER_LANE) && (flags & PHX_KERNEL_PACKET))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_anyhit: two launch models requested");
 	SceneView sv = view(scene);
 	hipStream_t stream = ctx->stream;
 	const TraceLaunch L = traceLaunch(ctx, flags, true);

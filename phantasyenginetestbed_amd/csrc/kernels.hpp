@@ -36,7 +36,7 @@ struct TexSet {
 	uint32_t count = 0;
 };
 
-enum class TraceKernel { Persistent, Simple };
+enum class TraceKernel { Persistent, Simple, Packet };
 
 // Launch parameters of the traversal kernels. `work` points at two zeroed uint32 words owned by
 // the launching stream (the persistent kernels leave them zeroed); `spill` holds

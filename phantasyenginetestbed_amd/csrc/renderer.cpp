@@ -30,11 +30,6 @@ struct PhxRenderer {
 	float4* out = nullptr;
 	uint32_t histIndex = 0;
 	bool haveHistory = false;
-	BatchDesc* descs = nullptr;  // 2 descriptors of the fused shadow + bounce launch
-	uint32_t* offsets = nullptr;
-	BatchDesc hostDescs[2];      // what descs / offsets hold
-	uint32_t hostOff[3] = {0u, 0u, 0u};
-	bool descsValid = false;
 	float4* lastOut = nullptr;
 	FrameParams* params = nullptr;  // per-frame values the frame kernels read (device)
 	// The captured frame (PHX_FRAME_GRAPH) and what it was captured for.
@@ -67,8 +62,7 @@ void freeRenderer(PhxRenderer* r)
 	dropGraph(r);
 	void* bufs[] = {r->prim,    r->primHits, r->gbuf,   r->bounce,  r->throughput,
 	                r->bounceHits, r->bounceInfo, r->contrib, r->shadow, r->inLight,
-	                r->color,   r->hist[0],  r->hist[1], r->out,    r->descs,
-	                r->offsets, r->params};
+	                r->color,   r->hist[0],  r->hist[1], r->out,    r->params};
 	for (void* b : bufs) hipFree(b);
 	delete r;
 }
@@ -117,7 +111,10 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	const uint32_t order = (r->w % 8u == 0u && r->h % 8u == 0u) ? PHX_RAYS_TILED_8X8 : PHX_RAYS_ROW_MAJOR;
 	const SceneView sv = view(scene);
 	const TraceLaunch closest = traceLaunch(ctx, 0, false);
-	const TraceLaunch anyhit = traceLaunch(ctx, 0, true);
+	// Shadow rays are coherent -- a pixel tile's rays towards one light, tiles in order -- so they
+	// are traced as packets, each wave expanding the union of 64 rays' nodes.
+	TraceLaunch shadows = traceLaunch(ctx, 0, true);
+	shadows.kernel = TraceKernel::Packet;
 	const char* what = "phx_render_frame";
 
 	// Primary hits -> hit records -> contributions and shadow rays of every light.
@@ -131,18 +128,12 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, r->params, 1u,
 	                    r->contrib, r->shadow, s),
 	        what);
-	// Bounce rays (they need only the hit records), traced together with the primary shadow rays
-	// in one fused persistent launch.
+	// Bounce rays (they need only the hit records): incoherent, traced by the persistent kernel.
 	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, camPos, r->params, r->bounce, r->throughput,
 	                           s),
 	        what);
-	{
-		const uint32_t total = r->n * L + r->m;
-		PHX_HIP(launchBatches(r->descs, r->offsets, 2u, total, closest,
-		                      reinterpret_cast<const float4*>(scene->nodes), scene->topCount,
-		                      nullptr, s),
-		        what);
-	}
+	PHX_HIP(launchAnyHit(sv, r->shadow, r->n * L, r->inLight, shadows, s), what);
+	if (r->m) PHX_HIP(launchClosest(sv, r->bounce, r->m, r->bounceHits, closest, nullptr, s), what);
 	PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 1u, false, r->inLight, r->contrib, L, s), what);
 
 	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
@@ -153,7 +144,7 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
 		                    r->m, r->params, 3u, r->contrib, r->shadow, s),
 		        what);
-		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, anyhit, s), what);
+		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, shadows, s), what);
 		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
 		        what);
 	}
@@ -162,35 +153,6 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 		PHX_HIP(launchTaa(r->color, r->hist[0], r->hist[1], r->gbuf, r->w, r->h,
 		                  !(flags & PHX_FRAME_NO_CLAMP), r->params, r->out, s),
 		        what);
-	return PHX_OK;
-}
-
-// The two batch descriptors of the fused shadow + bounce launch, uploaded when they change (before
-// any capture: the copies read the renderer's host copies, which change only after the stream has
-// drained the frames that used them).
-phx_status uploadDescs(PhxRenderer* r, const PhxScene* scene)
-{
-	hipStream_t s = r->ctx->stream;
-	const char* what = "phx_render_frame";
-	const uint32_t L = scene->numLights;
-	BatchDesc d[2];
-	const float4* nodes = reinterpret_cast<const float4*>(scene->nodes);
-	const float4* tris = reinterpret_cast<const float4*>(scene->tris);
-	d[0] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->shadow), r->inLight, r->n * L,
-	                 1u, scene->finiteBounds, 0u};
-	d[1] = BatchDesc{nodes, tris, reinterpret_cast<const float4*>(r->bounce), r->bounceHits, r->m,
-	                 0u, scene->finiteBounds, 0u};
-	const uint32_t off[3] = {0u, r->n * L, r->n * L + r->m};
-	if (!r->descsValid || std::memcmp(d, r->hostDescs, sizeof(d)) != 0 ||
-	    std::memcmp(off, r->hostOff, sizeof(off)) != 0) {
-		PHX_HIP(hipStreamSynchronize(s), what);
-		std::memcpy(r->hostDescs, d, sizeof(d));
-		std::memcpy(r->hostOff, off, sizeof(off));
-		PHX_HIP(hipMemcpyAsync(r->descs, r->hostDescs, sizeof(d), hipMemcpyHostToDevice, s), what);
-		PHX_HIP(hipMemcpyAsync(r->offsets, r->hostOff, sizeof(off), hipMemcpyHostToDevice, s),
-		        what);
-		r->descsValid = true;
-	}
 	return PHX_OK;
 }
 
@@ -241,8 +203,6 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 	if (st == PHX_OK) st = devAlloc(r->hist[0], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->hist[1], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->out, r->n, what);
-	if (st == PHX_OK) st = devAlloc(r->descs, 2, what);
-	if (st == PHX_OK) st = devAlloc(r->offsets, 3, what);
 	if (st == PHX_OK) st = devAlloc(r->params, 1, what);
 	if (st != PHX_OK) {
 		freeRenderer(r);
@@ -311,8 +271,6 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 	fp.taaReset = (flags & PHX_FRAME_TAA_RESET) || !r->haveHistory ? 1u : 0u;
 	fp.seed = seed;
 	fp.histIn = r->histIndex;
-	st = uploadDescs(r, scene);
-	if (st != PHX_OK) return st;
 	PHX_HIP(launchSetFrameParams(r->params, fp, s), what);
 	if (flags & PHX_FRAME_GRAPH) {
 		PhxRenderer::GraphKey k{};

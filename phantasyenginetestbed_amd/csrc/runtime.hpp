@@ -137,7 +137,9 @@ inline SceneView view(const PhxScene* s)
 inline TraceLaunch traceLaunch(const PhxContext* ctx, uint32_t flags, bool anyhit)
 {
 	TraceLaunch L;
-	L.kernel = (flags & PHX_KERNEL_ONE_RAY_PER_LANE) ? TraceKernel::Simple : TraceKernel::Persistent;
+	L.kernel = (flags & PHX_KERNEL_ONE_RAY_PER_LANE) ? TraceKernel::Simple
+	           : (flags & PHX_KERNEL_PACKET)        ? TraceKernel::Packet
+	                                                : TraceKernel::Persistent;
 	L.gridBlocks = anyhit ? ctx->gridAnyhit : ctx->gridClosest;
 	L.blockThreads = ctx->blockThreads;
 	L.topNodes = ctx->topNodes;

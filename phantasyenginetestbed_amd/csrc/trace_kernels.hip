@@ -348,6 +348,107 @@ __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t block
 	       handoff_bytes(blockThreads);
 }
 
+// ---- packet traversal (coherent rays) ------------------------------------------------------------
+//
+// For coherent rays -- camera rays in 8x8-tile order, shadow rays towards one light -- a wave traces
+// 64 consecutive rays together over one shared stack of (node, lane mask) entries. Each node is
+// loaded once for the wave (every lane reads the same address); each lane decides both child boxes
+// with its own ray and closest hit, and tests a visited leaf's triangles only where it decided to.
+// Per lane, the nodes expanded and the triangles tested, in their order, are exactly castRay's
+// (BVHTraversal.hpp:37-128): both child boxes are decided before either child is used, a visited
+// leaf's triangles are tested at once, left child first, and inner children are pushed left then
+// right, so the right subtree is expanded first; a lane takes part only in the entries it pushed.
+// The wave expands the union of its lanes' nodes, which for coherent rays is close to one ray's,
+// and every step does the same kind of work. Stack entry e lives in lane e % 64 of the register
+// triple e / 64 (node, mask low, mask high); trees are at most 64 levels deep (checked at upload),
+// and the stack holds at most one pending left child per level.
+
+constexpr uint32_t kPacketBlock = 256;
+
+template <bool kAnyHit>
+__global__ __launch_bounds__(kPacketBlock) void k_packet(const float4* __restrict__ nodes,
+                                                         const float4* __restrict__ tris,
+                                                         const float4* __restrict__ rays,
+                                                         void* __restrict__ out, uint32_t n,
+                                                         uint32_t finiteBounds)
+{
+	const uint32_t lane = __lane_id();
+	const uint32_t first = blockIdx.x * kPacketBlock + (threadIdx.x & ~63u);
+	if (first >= n) return;  // whole wave
+	const uint32_t i = first + lane;
+	const bool active = i < n;
+	const uint32_t ri = active ? i : first;  // lanes past the end read a valid ray, never used
+	const dev::Ray r = dev::make_ray(rays[2 * size_t(ri)], rays[2 * size_t(ri) + 1]);
+	// The NaN-free slab path only when every active lane may take it (the decision is uniform).
+	const bool fast = __ballot(active && !(finiteBounds != 0u && r.nanFree)) == 0ull;
+	Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+	Counts none;
+	bool occluded = false;
+	uint32_t sN0 = 0u, sL0 = 0u, sH0 = 0u, sN1 = 0u, sL1 = 0u, sH1 = 0u;
+	int sp = 0;
+	auto push = [&](uint32_t node, uint64_t m) {
+		const uint32_t lo = uint32_t(m), hi = uint32_t(m >> 32);
+		const bool low = sp < 64;
+		const bool mine = int(lane) == (low ? sp : sp - 64);
+		sN0 = low && mine ? node : sN0;
+		sL0 = low && mine ? lo : sL0;
+		sH0 = low && mine ? hi : sH0;
+		sN1 = !low && mine ? node : sN1;
+		sL1 = !low && mine ? lo : sL1;
+		sH1 = !low && mine ? hi : sH1;
+		sp++;
+	};
+	uint32_t cur = 0u;
+	uint64_t mask = __ballot(active);
+	while (true) {
+		const bool in = ((mask >> lane) & 1ull) != 0;
+		NodeWords nd = load_node(nodes, cur);
+		const int32_t lRaw = __builtin_amdgcn_readfirstlane(nd.w3.x);
+		const int32_t rRaw = __builtin_amdgcn_readfirstlane(nd.w3.y);
+		const int32_t lCnt = __builtin_amdgcn_readfirstlane(nd.w3.z);
+		const int32_t rCnt = __builtin_amdgcn_readfirstlane(nd.w3.w);
+		const float tCurrMax = kAnyHit ? r.tMax : dev::min_first(r.tMax, c.t);
+		bool visitL, visitR;
+		decide(r, fast, nd, tCurrMax, visitL, visitR);
+		visitL = visitL && in;
+		visitR = visitR && in;
+		uint64_t pushL = 0ull, pushR = 0ull;
+		if (lCnt == 0)
+			pushL = __ballot(visitL);
+		else if (visitL)
+			kAnyHit ? void(occluded = anyhit_leaf(r, tris, child_index(lRaw), lCnt))
+			        : closest_leaf<false>(r, tris, child_index(lRaw), lCnt, c, none);
+		if (rCnt == 0)
+			pushR = __ballot(visitR && !(kAnyHit && occluded));
+		else if (visitR && !(kAnyHit && occluded))
+			kAnyHit ? void(occluded = anyhit_leaf(r, tris, child_index(rRaw), rCnt))
+			        : closest_leaf<false>(r, tris, child_index(rRaw), rCnt, c, none);
+		const uint64_t done = kAnyHit ? __ballot(occluded) : 0ull;
+		pushL &= ~done;
+		if (pushL != 0ull && sp < 128) push(child_index(lRaw), pushL);
+		if (pushR != 0ull && sp < 128) push(child_index(rRaw), pushR);
+		// Next: the top entry that still has lanes (any hit: lanes without an occluder).
+		mask = 0ull;
+		while (mask == 0ull && sp > 0) {
+			sp--;
+			const bool low = sp < 64;
+			const int l = low ? sp : sp - 64;
+			cur = uint32_t(__builtin_amdgcn_readlane(int(low ? sN0 : sN1), l));
+			const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(low ? sL0 : sL1), l));
+			const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(low ? sH0 : sH1), l));
+			mask = ((uint64_t(hi) << 32) | lo) & ~done;
+		}
+		if (mask == 0ull) break;
+	}
+	if (active) {
+		if (kAnyHit)
+			static_cast<uint8_t*>(out)[i] = occluded ? 0 : 1;
+		else
+			static_cast<uint4*>(out)[i] = make_uint4(c.tri, __float_as_uint(c.t),
+			                                         __float_as_uint(c.u), __float_as_uint(c.v));
+	}
+}
+
 // ---- software-pipelined persistent loop -------------------------------------------------------
 //
 // Each iteration of a lane consumes the record it loaded in the previous iteration and issues the
@@ -1475,7 +1576,12 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	const float4* tris = reinterpret_cast<const float4*>(s.trisPre);
 	const float4* r4 = reinterpret_cast<const float4*>(rays);
 	uint4* h4 = reinterpret_cast<uint4*>(hits);
-	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
+	if (L.kernel == TraceKernel::Packet && !counters) {
+		hipLaunchKernelGGL(k_packet<false>, dim3(gridFor(n, kPacketBlock)), dim3(kPacketBlock), 0,
+		                   stream, nodes, tris, r4, static_cast<void*>(h4), n, s.finiteBounds);
+		return hipGetLastError();
+	}
+	if (L.kernel != TraceKernel::Persistent || n > 0x7FFFFFFFu) {
 		if (counters)
 			hipLaunchKernelGGL(k_closest_simple<true>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0,
 			                   stream, nodes, tris, r4, h4, n, counters);
@@ -1503,6 +1609,11 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	const float4* nodes = reinterpret_cast<const float4*>(s.nodes);
 	const float4* tris = reinterpret_cast<const float4*>(s.trisPre);
 	const float4* r4 = reinterpret_cast<const float4*>(rays);
+	if (L.kernel == TraceKernel::Packet) {
+		hipLaunchKernelGGL(k_packet<true>, dim3(gridFor(n, kPacketBlock)), dim3(kPacketBlock), 0,
+		                   stream, nodes, tris, r4, static_cast<void*>(inLight), n, s.finiteBounds);
+		return hipGetLastError();
+	}
 	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
 		hipLaunchKernelGGL(k_anyhit_simple, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, stream,
 		                   nodes, tris, r4, inLight, n);

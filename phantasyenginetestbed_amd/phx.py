@@ -42,6 +42,7 @@ ASYNC = 0x2
 TRAVERSAL_REFERENCE_ORDER = 0x00
 TRAVERSAL_NEAR_FIRST = 0x10
 KERNEL_ONE_RAY_PER_LANE = 0x100
+KERNEL_PACKET = 0x200
 BATCH_CLOSEST = 0
 BATCH_ANYHIT = 1
 PLAN_TRACK_COMPLETION = 0x10

@@ -51,8 +51,8 @@ def c2(ctx):
     s.release()
 
 
-KERNELS = [0, phx.KERNEL_ONE_RAY_PER_LANE]
-KERNEL_IDS = ["persistent", "one-ray-per-lane"]
+KERNELS = [0, phx.KERNEL_ONE_RAY_PER_LANE, phx.KERNEL_PACKET]
+KERNEL_IDS = ["persistent", "one-ray-per-lane", "packet"]
 
 
 def assert_same_hits(gpu, ref, what):
@@ -140,7 +140,9 @@ def test_c2_primary_bounce_shadow_bit_identical(ctx, c2, golden):
     say("[gpu] C2 1920x1080 primary")
     rays = ctx.gen_primary_rays(cam, 1920, 1080)
     hits = ctx.cast_closest(s, rays)
-    assert_same_hits(hits, O.cast_closest(b.nodes, b.tris, rays), "C2 primary")
+    ref_primary = O.cast_closest(b.nodes, b.tris, rays)
+    assert_same_hits(hits, ref_primary, "C2 primary")
+    assert_same_hits(ctx.cast_closest(s, rays, phx.KERNEL_PACKET), ref_primary, "C2 primary packet")
     hitmask = hits["tri"] != phx.MISS
     assert 0.3 < hitmask.mean() < 0.9
 
@@ -423,3 +425,15 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
                               grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
                               block_threads=old["blockThreads"], top_nodes=old["topNodes"],
                               helper_waves=old["helperWaves"])
+
+
+def test_launch_models_are_exclusive(ctx, c1):
+    _, s = c1
+    rays = np.zeros(4, phx.RAY)
+    rays["d"] = (0.0, 0.0, 1.0)
+    rays["tmax"] = np.float32(phx.FLT_MAX)
+    both = phx.KERNEL_ONE_RAY_PER_LANE | phx.KERNEL_PACKET
+    for cast in (ctx.cast_closest, ctx.cast_anyhit):
+        with pytest.raises(phx.PhxError) as e:
+            cast(s, rays, both)
+        assert e.value.status == phx.ERR_INVALID_ARGUMENT
