@@ -32,6 +32,10 @@ struct PhxRenderer {
 	bool haveHistory = false;
 	float4* lastOut = nullptr;
 	FrameParams* params = nullptr;  // per-frame values the frame kernels read (device)
+	// The bounce rays' closest hits run on a side stream beside the primary shadow rays (fork
+	// after the bounce rays are generated, join before the bounce records).
+	hipStream_t side = nullptr;
+	hipEvent_t fork = nullptr, join = nullptr;
 	// The captured frame (PHX_FRAME_GRAPH) and what it was captured for.
 	struct GraphKey {
 		const PhxScene* scene;
@@ -60,6 +64,9 @@ void dropGraph(PhxRenderer* r)
 void freeRenderer(PhxRenderer* r)
 {
 	dropGraph(r);
+	if (r->fork) hipEventDestroy(r->fork);
+	if (r->join) hipEventDestroy(r->join);
+	if (r->side) hipStreamDestroy(r->side);
 	void* bufs[] = {r->prim,    r->primHits, r->gbuf,   r->bounce,  r->throughput,
 	                r->bounceHits, r->bounceInfo, r->contrib, r->shadow, r->inLight,
 	                r->color,   r->hist[0],  r->hist[1], r->out,    r->params};
@@ -132,12 +139,22 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, camPos, r->params, r->bounce, r->throughput,
 	                           s),
 	        what);
+	// The bounce rays' closest hits (the persistent kernel, whose last rays take long) overlap the
+	// primary shadow rays and their shadow logic; the bounce records need the shading buffers back,
+	// so the streams join before them. No other persistent launch runs meanwhile, so the two
+	// streams never share the context's ray counter or spill area.
+	if (r->m) {
+		PHX_HIP(hipEventRecord(r->fork, s), what);
+		PHX_HIP(hipStreamWaitEvent(r->side, r->fork, 0), what);
+		PHX_HIP(launchClosest(sv, r->bounce, r->m, r->bounceHits, closest, nullptr, r->side), what);
+		PHX_HIP(hipEventRecord(r->join, r->side), what);
+	}
 	PHX_HIP(launchAnyHit(sv, r->shadow, r->n * L, r->inLight, shadows, s), what);
-	if (r->m) PHX_HIP(launchClosest(sv, r->bounce, r->m, r->bounceHits, closest, nullptr, s), what);
 	PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 1u, false, r->inLight, r->contrib, L, s), what);
 
 	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
 	if (r->m) {
+		PHX_HIP(hipStreamWaitEvent(s, r->join, 0), what);
 		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->materials,
 		                        scene->numMaterials, scene->tex, r->bounceInfo, s),
 		        what);
@@ -204,6 +221,10 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 	if (st == PHX_OK) st = devAlloc(r->hist[1], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->out, r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->params, 1, what);
+	if (st == PHX_OK && (hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
+	                     hipEventCreateWithFlags(&r->fork, hipEventDisableTiming) != hipSuccess ||
+	                     hipEventCreateWithFlags(&r->join, hipEventDisableTiming) != hipSuccess))
+		st = fail(PHX_ERR_HIP, "%s: stream or event creation failed", what);
 	if (st != PHX_OK) {
 		freeRenderer(r);
 		return st;
