@@ -316,7 +316,7 @@ def main() -> int:
                     help="fixed persistent launch shape instead of the autotuner's (profiling)")
     ap.add_argument("--plan-launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
                     help="fixed launch shape of the fused 256-batch plan (profiling)")
-    ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane"], default=None,
+    ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane", "packet"], default=None,
                     help="force a launch model (default: the autotuner's choice)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB first (a plain copy kernel "
@@ -375,7 +375,8 @@ def main() -> int:
         launch = ctx.autotune(scene, rays.ptr, n)
         log(f"autotuned launch: {launch}")
     kernel = args.kernel or launch["kernel"]
-    mode = phx.KERNEL_ONE_RAY_PER_LANE if kernel == "one-ray-per-lane" else 0
+    mode = {"one-ray-per-lane": phx.KERNEL_ONE_RAY_PER_LANE,
+            "packet": phx.KERNEL_PACKET}.get(kernel, 0)
 
     ev = HipEvents()
     stream = ctx.stream

@@ -228,11 +228,12 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* params)
 
 /* Autotuner for single-batch casts: times phx_cast_closest (or, with PHX_AUTOTUNE_ANYHIT,
  * phx_cast_anyhit) on the given device rays under candidate launch shapes -- block 256/512,
- * treelet 0/512/1024 nodes, 1/4, 1/2 or all of the resident grid, refill 8/32, and the
- * one-ray-per-lane kernel -- with HIP events, and applies the fastest persistent shape to the
- * context. `chosen` (may be NULL) receives it; residentGridBlocks = 0xFFFFFFFF there means the
- * one-ray-per-lane kernel won (pass PHX_KERNEL_ONE_RAY_PER_LANE to the casts). `rays` must be
- * device memory (PHX_MEM_DEVICE). */
+ * treelet 0/512/1024 nodes, 1/4, 1/2 or all of the resident grid, refill 8/32, 0 or 4 helper
+ * waves, the one-ray-per-lane kernel and the packet kernel -- with HIP events, and applies the
+ * fastest persistent shape to the context. `chosen` (may be NULL) receives it; residentGridBlocks
+ * = 0xFFFFFFFF there means the one-ray-per-lane kernel won (pass PHX_KERNEL_ONE_RAY_PER_LANE to
+ * the casts) and 0xFFFFFFFE the packet kernel (pass PHX_KERNEL_PACKET). `rays` must be device
+ * memory (PHX_MEM_DEVICE). */
 #define PHX_AUTOTUNE_ANYHIT 0x1000u
 phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                         uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen);

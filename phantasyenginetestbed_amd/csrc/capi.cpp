@@ -998,6 +998,9 @@ std::vector<Candidate> candidates(PhxContext* ctx, bool plan)
 		c.launch.kernel = TraceKernel::Simple;
 		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0};
 		out.push_back(c);
+		c.launch.kernel = TraceKernel::Packet;
+		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0};
+		out.push_back(c);
 	}
 	return out;
 }
@@ -1063,7 +1066,8 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 	hipStreamSynchronize(ctx->stream);
 	hipFree(out);
 	if (st != PHX_OK) return st;
-	// Apply: the one-ray-per-lane kernel is selected per call by flag; record the persistent shape.
+	// Apply: the one-ray-per-lane and packet kernels are selected per call by flag; record the
+	// persistent shape.
 	if (best.launch.kernel == TraceKernel::Persistent) {
 		ctx->blockThreads = best.launch.blockThreads;
 		ctx->topNodes = best.launch.topNodes;
@@ -1076,7 +1080,9 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 	}
 	if (chosen) {
 		*chosen = best.params;
-		chosen->residentGridBlocks = best.launch.kernel == TraceKernel::Simple ? 0xFFFFFFFFu : 0u;
+		chosen->residentGridBlocks = best.launch.kernel == TraceKernel::Simple   ? 0xFFFFFFFFu
+		                             : best.launch.kernel == TraceKernel::Packet ? 0xFFFFFFFEu
+		                                                                         : 0u;
 	}
 	return ok();
 }

@@ -642,13 +642,14 @@ class Context:
 
     def autotune(self, scene: Scene, rays_ptr: int, n: int, anyhit: bool = False) -> dict:
         """Times candidate launch shapes on device rays and applies the fastest. Returns it;
-        'kernel' is 'one-ray-per-lane' when that kernel won (pass KERNEL_ONE_RAY_PER_LANE)."""
+        'kernel' is 'one-ray-per-lane' or 'packet' when that kernel won (pass
+        KERNEL_ONE_RAY_PER_LANE or KERNEL_PACKET)."""
         p = LaunchParams()
         flags = MEM_DEVICE | (AUTOTUNE_ANYHIT if anyhit else 0)
         check(lib().phx_autotune(self.handle, scene.handle, rays_ptr, n, flags, C.byref(p)))
         d = p.as_dict()
-        d["kernel"] = "one-ray-per-lane" if d.pop("residentGridBlocks") == 0xFFFFFFFF \
-            else "persistent"
+        d["kernel"] = {0xFFFFFFFF: "one-ray-per-lane", 0xFFFFFFFE: "packet"}.get(
+            d.pop("residentGridBlocks"), "persistent")
         return d
 
     def alloc(self, nbytes: int) -> DeviceBuffer:

@@ -437,3 +437,32 @@ def test_launch_models_are_exclusive(ctx, c1):
         with pytest.raises(phx.PhxError) as e:
             cast(s, rays, both)
         assert e.value.status == phx.ERR_INVALID_ARGUMENT
+
+
+def test_autotune_reports_packets_for_coherent_shadow_rays(ctx, c2):
+    """On coherent shadow rays (camera hits towards one light) the autotuner's fastest candidate
+    is the packet kernel, reported as such, and its results are the oracle's."""
+    b, s = c2
+    cam = phx.camera_look_at(*C2_CAM, math.radians(60), 1920, 1080)
+    rays = ctx.gen_primary_rays(cam, 1920, 1080)
+    hits = ctx.cast_closest(s, rays)
+    m = hits["tri"] != phx.MISS
+    pos = (rays["o"] + rays["d"] * hits["t"][:, None])[m].astype(np.float32)
+    to_l = LIGHT[None, :] - pos
+    dist = np.sqrt((to_l.astype(np.float64) ** 2).sum(1)).astype(np.float32)
+    sh = np.zeros(len(pos), phx.RAY)
+    sh["o"], sh["d"] = pos, (to_l / dist[:, None]).astype(np.float32)
+    sh["tmin"], sh["tmax"] = np.float32(1e-3), dist - np.float32(1e-3)
+    old = ctx.launch_params()
+    buf = ctx.alloc(len(sh) * 32).upload(sh)
+    try:
+        chosen = ctx.autotune(s, buf.ptr, len(sh), anyhit=True)
+        say(f"[gpu] autotune on {len(sh)} shadow rays: {chosen}")
+        assert chosen["kernel"] == "packet"
+        assert (ctx.cast_anyhit(s, sh, phx.KERNEL_PACKET) == O.cast_anyhit(b.nodes, b.tris, sh)).all()
+    finally:
+        buf.free()
+        ctx.set_launch_params(grid_closest=old["gridBlocksClosest"],
+                              grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
+                              block_threads=old["blockThreads"], top_nodes=old["topNodes"],
+                              helper_waves=old["helperWaves"])
