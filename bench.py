@@ -123,6 +123,13 @@ def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
                       f" on {threads} threads ({cpu_model}); hits identical to GPU: {same}"}
 
 
+def set_shape(ctx, p: dict) -> None:
+    ctx.set_launch_params(grid_closest=p["gridBlocksClosest"], grid_anyhit=p["gridBlocksAnyhit"],
+                          refill_min=p["refillMin"], block_threads=p["blockThreads"],
+                          top_nodes=p["topNodes"], helper_waves=p["helperWaves"],
+                          rays_per_fetch=p["raysPerFetch"], stack_lds_entries=p["stackLdsEntries"])
+
+
 def batched_form(ctx, scene, pools, ev, reps: int = 3, plan_launch=None) -> dict:
     from phantasyenginetestbed_amd import workloads as W
     bs = W.batch_set(pools)
@@ -136,29 +143,43 @@ def batched_form(ctx, scene, pools, ev, reps: int = 3, plan_launch=None) -> dict
     W.launch_plain(ctx, scene, bs)
     ctx.synchronize()
     ref = W.download_outputs(bs)
-    plain_best, plain_lat = None, None
-    for _ in range(reps):
-        ev.record(evs[0], stream)
-        for i, (k, r, (rb, ob)) in enumerate(zip(bs.kinds, bs.rays, bs.device)):
-            if k == "shadow":
-                ctx.cast_anyhit_device(scene, rb.ptr, len(r), ob.ptr, asynchronous=True)
-            else:
-                ctx.cast_closest_device(scene, rb.ptr, len(r), ob.ptr, asynchronous=True)
-            ev.record(evs[i + 1], stream)
-        ctx.synchronize()
-        lat = np.array([ev.elapsed_ms(evs[i], evs[i + 1]) for i in range(len(sizes))])
-        tot = ev.elapsed_ms(evs[0], evs[-1])
-        if plain_best is None or tot < plain_best:
-            plain_best, plain_lat = tot, lat
+    plain_best, plain_lat, plain_shape = None, None, None
+    # The plain form gets its best of two shapes: the context's (tuned for one large cast) and
+    # the library defaults (64 rays per fetch suit its small batches better).
+    tuned_shape = ctx.launch_params()
+    default_shape = dict(tuned_shape, gridBlocksClosest=ctx.num_cus, gridBlocksAnyhit=ctx.num_cus,
+                         refillMin=8, blockThreads=512, topNodes=0, helperWaves=0,
+                         raysPerFetch=64, stackLdsEntries=16)
+    for name, shape in (("tuned", tuned_shape), ("default", default_shape)):
+        set_shape(ctx, shape)
+        for _ in range(reps):
+            ev.record(evs[0], stream)
+            for i, (k, r, (rb, ob)) in enumerate(zip(bs.kinds, bs.rays, bs.device)):
+                if k == "shadow":
+                    ctx.cast_anyhit_device(scene, rb.ptr, len(r), ob.ptr, asynchronous=True)
+                else:
+                    ctx.cast_closest_device(scene, rb.ptr, len(r), ob.ptr, asynchronous=True)
+                ev.record(evs[i + 1], stream)
+            ctx.synchronize()
+            lat = np.array([ev.elapsed_ms(evs[i], evs[i + 1]) for i in range(len(sizes))])
+            tot = ev.elapsed_ms(evs[0], evs[-1])
+            if plain_best is None or tot < plain_best:
+                plain_best, plain_lat, plain_shape = tot, lat, name
+        same_plain = all(np.array_equal(a, b) for a, b in zip(ref, W.download_outputs(bs)))
+        assert same_plain, f"plain outputs differ under the {name} shape"
+    set_shape(ctx, tuned_shape)
     ev.destroy(evs)
 
     plan = W.plan_for(ctx, scene, bs, track=True)
     if plan_launch:
         grid, block, top, refill = plan_launch[:4]
+        extra = list(plan_launch[4:]) + [0] * (3 - len(plan_launch[4:]))
         plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill,
-                               helper_waves=plan_launch[4] if len(plan_launch) > 4 else 0)
+                               helper_waves=extra[0], rays_per_fetch=extra[1],
+                               stack_lds_entries=extra[2])
         tuned = {"gridBlocksClosest": grid, "blockThreads": block, "topNodes": top,
-                 "refillMin": refill}
+                 "refillMin": refill, "raysPerFetch": extra[1] or 64,
+                 "stackLdsEntries": extra[2] or 16}
     else:
         tuned = plan.autotune()
     e2 = ev.create(2)
@@ -195,9 +216,10 @@ def batched_form(ctx, scene, pools, ev, reps: int = 3, plan_launch=None) -> dict
         "fused_batch_completion_p50_ms": round(float(np.percentile(fused_lat, 50)), 4),
         "fused_batch_completion_p99_ms": round(float(np.percentile(fused_lat, 99)), 4),
         "fused_vs_plain": round(plain_best / fused_ms, 2),
+        "plain_shape": plain_shape,
         "identical_outputs": bool(same_fused and same_graph),
         "fused_launch": {k: tuned[k] for k in ("gridBlocksClosest", "blockThreads", "topNodes",
-                                               "refillMin")},
+                                               "refillMin", "raysPerFetch", "stackLdsEntries")},
     }
 
 
@@ -312,9 +334,11 @@ def main() -> int:
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--launch", default=None, metavar="GRID,BLOCK,TOP,REFILL[,HELPERS]",
+    ap.add_argument("--launch", default=None,
+                    metavar="GRID,BLOCK,TOP,REFILL[,HELPERS[,RAYS_PER_FETCH[,LDS_STACK]]]",
                     help="fixed persistent launch shape instead of the autotuner's (profiling)")
-    ap.add_argument("--plan-launch", default=None, metavar="GRID,BLOCK,TOP,REFILL",
+    ap.add_argument("--plan-launch", default=None,
+                    metavar="GRID,BLOCK,TOP,REFILL[,HELPERS[,RAYS_PER_FETCH[,LDS_STACK]]]",
                     help="fixed launch shape of the fused 256-batch plan (profiling)")
     ap.add_argument("--kernel", choices=["persistent", "one-ray-per-lane", "packet"], default=None,
                     help="force a launch model (default: the autotuner's choice)")
@@ -366,9 +390,10 @@ def main() -> int:
     if args.launch:
         vals = [int(x) for x in args.launch.split(",")]
         grid, block, top, refill = vals[:4]
+        extra = vals[4:] + [0] * (3 - len(vals[4:]))
         ctx.set_launch_params(grid_closest=grid, grid_anyhit=grid, refill_min=refill,
-                              block_threads=block, top_nodes=top,
-                              helper_waves=vals[4] if len(vals) > 4 else 0)
+                              block_threads=block, top_nodes=top, helper_waves=extra[0],
+                              rays_per_fetch=extra[1], stack_lds_entries=extra[2])
         launch = {"kernel": "persistent", **ctx.launch_params()}
         log(f"fixed launch: {launch}")
     elif not args.no_autotune:
@@ -413,7 +438,8 @@ def main() -> int:
                    "traversal": "reference-order (bit-identical to castRay)",
                    "kernel": kernel,
                    "launch": {k: launch[k] for k in ("gridBlocksClosest", "blockThreads",
-                                                     "topNodes", "refillMin", "helperWaves")
+                                                     "topNodes", "refillMin", "helperWaves",
+                                                     "raysPerFetch", "stackLdsEntries")
                               if k in launch}},
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),

@@ -208,11 +208,14 @@ uint32_t phx_context_num_cus(PhxContext* ctx);
  * breadth first), the refill threshold (a wave refills its idle lanes once at least `refillMin` of
  * its 64 lanes are idle; 1..64), and helper waves: extra waves per block that trace no ray of
  * their own and take over, one wave per ray, the last rays the block's lane-mode waves hand out
- * (blockThreads + 64 * helperWaves <= 1024). phx_create's defaults: one block of 512 threads per
- * CU, no treelet, refill 8, no helper waves; phx_autotune picks them per scene and ray mix. In
- * phx_set_launch_params a zero field (0xFFFFFFFF for topNodes and helperWaves) keeps the current
- * value; combinations that do not fit a CU are rejected. In phx_plan_set_launch_params
- * 0xFFFFFFFF helper waves means none. Results do not depend on these parameters. */
+ * (blockThreads + 64 * helperWaves <= 1024), rays per fetch (the rays a wave claims from the
+ * launch's shared counter with one atomic; 8..4096) and the stack split (the traversal stack
+ * entries per lane kept in LDS, 4..32; deeper entries live in HBM). phx_create's defaults: one
+ * block of 512 threads per CU, no treelet, refill 8, no helper waves, 64 rays per fetch, 16 LDS
+ * stack entries; phx_autotune picks them per scene and ray mix. In phx_set_launch_params a zero
+ * field (0xFFFFFFFF for topNodes and helperWaves) keeps the current value; combinations that do
+ * not fit a CU are rejected. In phx_plan_set_launch_params 0xFFFFFFFF helper waves means none and
+ * a zero raysPerFetch or stackLdsEntries the default. Results do not depend on these parameters. */
 typedef struct PhxLaunchParams {
 	uint32_t gridBlocksClosest;
 	uint32_t gridBlocksAnyhit;
@@ -221,6 +224,8 @@ typedef struct PhxLaunchParams {
 	uint32_t topNodes;
 	uint32_t residentGridBlocks; /* read only: blocks of this shape the device holds at once */
 	uint32_t helperWaves;
+	uint32_t raysPerFetch;
+	uint32_t stackLdsEntries;
 } PhxLaunchParams;
 
 phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out);
@@ -229,8 +234,9 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* params)
 /* Autotuner for single-batch casts: times phx_cast_closest (or, with PHX_AUTOTUNE_ANYHIT,
  * phx_cast_anyhit) on the given device rays under candidate launch shapes -- block 256/512,
  * treelet 0/512/1024 nodes, 1/4, 1/2 or all of the resident grid, refill 8/32, 0 or 4 helper
- * waves, the one-ray-per-lane kernel and the packet kernel -- with HIP events, and applies the
- * fastest persistent shape to the context. `chosen` (may be NULL) receives it; residentGridBlocks
+ * waves, the one-ray-per-lane kernel and the packet kernel -- with HIP events; if a persistent
+ * shape wins it then refines rays per fetch (16..256) and the stack split (8..32 LDS entries) one
+ * at a time around it, and applies the fastest persistent shape to the context. `chosen` (may be NULL) receives it; residentGridBlocks
  * = 0xFFFFFFFF there means the one-ray-per-lane kernel won (pass PHX_KERNEL_ONE_RAY_PER_LANE to
  * the casts) and 0xFFFFFFFE the packet kernel (pass PHX_KERNEL_PACKET). `rays` must be device
  * memory (PHX_MEM_DEVICE). */

@@ -247,7 +247,7 @@ phx_status phx_create(int device, PhxContext** outCtx)
 	// fastest or close to it for incoherent closest-hit and any-hit casts on MI355X;
 	// phx_autotune refines it per scene and ray mix.
 	uint32_t bpc = 0;
-	e = persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, 0, &bpc);
+	e = persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, 0, ctx->ldsDepth, &bpc);
 	if (e == hipSuccess && bpc == 0) e = hipErrorInvalidConfiguration;
 	ctx->gridClosest = ctx->numCUs;
 	ctx->gridAnyhit = ctx->numCUs;
@@ -344,9 +344,11 @@ phx_status phx_get_launch_params(PhxContext* ctx, PhxLaunchParams* out)
 	out->blockThreads = ctx->blockThreads;
 	out->topNodes = ctx->topNodes;
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, ctx->helperWaves, &bpc);
+	persistentBlocksPerCU(ctx->blockThreads, ctx->topNodes, ctx->helperWaves, ctx->ldsDepth, &bpc);
 	out->residentGridBlocks = ctx->numCUs * bpc;
 	out->helperWaves = ctx->helperWaves;
+	out->raysPerFetch = ctx->window;
+	out->stackLdsEntries = ctx->ldsDepth;
 	return ok();
 }
 
@@ -358,6 +360,8 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
 	const uint32_t block = p->blockThreads ? p->blockThreads : ctx->blockThreads;
 	const uint32_t top = p->topNodes != 0xFFFFFFFFu ? p->topNodes : ctx->topNodes;
 	const uint32_t helpers = p->helperWaves != 0xFFFFFFFFu ? p->helperWaves : ctx->helperWaves;
+	const uint32_t depth = p->stackLdsEntries ? p->stackLdsEntries : ctx->ldsDepth;
+	const uint32_t window = p->raysPerFetch ? p->raysPerFetch : ctx->window;
 	if (block % 64 != 0 || block > 1024)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: blockThreads %u", block);
 	if (top > kTopCap)
@@ -366,12 +370,18 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
 		return fail(PHX_ERR_INVALID_ARGUMENT,
 		            "phx_set_launch_params: %u threads and %u helper waves exceed 1024 threads",
 		            block, helpers);
+	if (!validLdsDepth(depth))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: stackLdsEntries %u outside 4..32",
+		            depth);
+	if (window < 8u || window > 4096u)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: raysPerFetch %u outside 8..4096",
+		            window);
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(block, top, helpers, &bpc);
+	persistentBlocksPerCU(block, top, helpers, depth, &bpc);
 	if (bpc == 0)
 		return fail(PHX_ERR_INVALID_ARGUMENT,
-		            "phx_set_launch_params: %u threads (+%u helper waves) with a %u-node treelet do "
-		            "not fit a CU", block, helpers, top);
+		            "phx_set_launch_params: %u threads (+%u helper waves) with a %u-node treelet and "
+		            "%u LDS stack entries do not fit a CU", block, helpers, top, depth);
 	const uint32_t maxGrid = ctx->spillLanes / block;
 	if (p->gridBlocksClosest > maxGrid || p->gridBlocksAnyhit > maxGrid)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: grid above %u blocks",
@@ -379,12 +389,14 @@ phx_status phx_set_launch_params(PhxContext* ctx, const PhxLaunchParams* p)
 	if (p->refillMin > 64)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_set_launch_params: refillMin above 64");
 	// A new block or treelet size without an explicit grid gets the resident grid of that shape.
-	const bool reshaped =
-		block != ctx->blockThreads || top != ctx->topNodes || helpers != ctx->helperWaves;
+	const bool reshaped = block != ctx->blockThreads || top != ctx->topNodes ||
+	                      helpers != ctx->helperWaves || depth != ctx->ldsDepth;
 	const uint32_t resident = std::min(ctx->numCUs * bpc, maxGrid);
 	ctx->blockThreads = block;
 	ctx->topNodes = top;
 	ctx->helperWaves = helpers;
+	ctx->ldsDepth = depth;
+	ctx->window = window;
 	if (p->gridBlocksClosest)
 		ctx->gridClosest = p->gridBlocksClosest;
 	else if (reshaped)
@@ -832,13 +844,20 @@ phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t nu
 	p->ctx = ctx;
 	p->numBatches = numBatches;
 	p->total = uint32_t(total);
-	// A fused launch is throughput bound: default to the resident grid without a treelet.
+	// A fused launch is throughput bound: default to the resident grid of the fused kernel without
+	// a treelet or helper waves.
 	p->launch = traceLaunch(ctx, 0, false);
 	p->launch.topNodes = 0;
 	uint32_t bpc = 0;
 	p->launch.helperWaves = 0;
-	persistentBlocksPerCU(p->launch.blockThreads, 0, 0, &bpc);
-	p->launch.gridBlocks = std::max(1u, std::min(ctx->numCUs * bpc,
+	batchBlocksPerCU(p->launch.blockThreads, 0, 0, p->launch.ldsDepth, numBatches,
+	                 (flags & PHX_PLAN_TRACK_COMPLETION) != 0, &bpc);
+	if (bpc == 0) {  // the context's stack split does not fit this kernel: use the default split
+		p->launch.ldsDepth = 16;
+		batchBlocksPerCU(p->launch.blockThreads, 0, 0, 16, numBatches,
+		                 (flags & PHX_PLAN_TRACK_COMPLETION) != 0, &bpc);
+	}
+	p->launch.gridBlocks = std::max(1u, std::min(ctx->numCUs * std::max(bpc, 1u),
 	                                             ctx->spillLanes / p->launch.blockThreads));
 	p->hostDescs = descs;
 	if (oneScene) {
@@ -932,10 +951,27 @@ phx_status phx_plan_batch_times(PhxContext* ctx, PhxPlan* plan, float* msSinceSt
 namespace {
 
 // Checks a launch shape against the device (block size, LDS, spill area); fills `L`.
-phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, bool plan)
+// Blocks per CU of the kernel a shape launches: the persistent kernel for single casts, the fused
+// kernel -- with the plan's batch count and completion tracking -- for a plan.
+uint32_t blocksPerCU(const PhxPlan* plan, uint32_t block, uint32_t top, uint32_t helpers,
+                     uint32_t depth)
+{
+	uint32_t bpc = 0;
+	if (plan)
+		batchBlocksPerCU(block, std::min(top, plan->treeletCount), helpers, depth, plan->numBatches,
+		                 plan->clock != nullptr, &bpc);
+	else
+		persistentBlocksPerCU(block, top, helpers, depth, &bpc);
+	return bpc;
+}
+
+phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L,
+                    const PhxPlan* plan)
 {
 	const uint32_t block = p.blockThreads;
 	const uint32_t helpers = p.helperWaves == 0xFFFFFFFFu ? 0u : p.helperWaves;
+	const uint32_t depth = p.stackLdsEntries ? p.stackLdsEntries : 16u;
+	const uint32_t window = p.raysPerFetch ? p.raysPerFetch : 64u;
 	if (block == 0 || block % 64 != 0 || block > 1024)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: blockThreads %u", block);
 	if (p.topNodes > kTopCap)
@@ -943,11 +979,14 @@ phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, b
 	if (helpers > 16u || block + 64u * helpers > 1024u)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: %u threads and %u helper waves",
 		            block, helpers);
-	uint32_t bpc = 0;
-	persistentBlocksPerCU(block, p.topNodes, helpers, &bpc);
-	if (bpc == 0) return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: shape does not fit a CU");
+	if (!validLdsDepth(depth))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: stackLdsEntries %u outside 4..32", depth);
+	if (window < 8u || window > 4096u)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: raysPerFetch %u outside 8..4096", window);
+	if (blocksPerCU(plan, block, p.topNodes, helpers, depth) == 0)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: shape does not fit a CU");
 	const uint32_t maxGrid = ctx->spillLanes / block;
-	const uint32_t grid = plan ? p.gridBlocksClosest : p.gridBlocksClosest;
+	const uint32_t grid = p.gridBlocksClosest;
 	if (grid == 0 || grid > maxGrid)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "launch params: grid %u outside 1..%u", grid, maxGrid);
 	if (p.refillMin == 0 || p.refillMin > 64)
@@ -955,6 +994,8 @@ phx_status shapeFor(PhxContext* ctx, const PhxLaunchParams& p, TraceLaunch& L, b
 	L.blockThreads = block;
 	L.topNodes = p.topNodes;
 	L.helperWaves = helpers;
+	L.ldsDepth = depth;
+	L.window = window;
 	L.gridBlocks = grid;
 	L.refillMin = p.refillMin;
 	L.spillLanes = ctx->spillLanes;
@@ -972,15 +1013,14 @@ struct Candidate {
 // Candidate shapes: block 256/512, treelet 0/512/1024 nodes, a quarter, half or all of the
 // resident grid, refill threshold 8 or 32, and for single batches 0 or 4 helper waves -- plus the
 // one-ray-per-lane kernel for single batches.
-std::vector<Candidate> candidates(PhxContext* ctx, bool plan)
+std::vector<Candidate> candidates(PhxContext* ctx, const PhxPlan* plan)
 {
 	std::vector<Candidate> out;
 	for (uint32_t helpers : {0u, 4u}) {
 		if (plan && helpers) continue;
 		for (uint32_t block : {256u, 512u})
 			for (uint32_t top : {0u, 512u, 1024u}) {
-				uint32_t bpc = 0;
-				persistentBlocksPerCU(block, top, helpers, &bpc);
+				const uint32_t bpc = blocksPerCU(plan, block, top, helpers, 16u);
 				if (bpc == 0) continue;
 				const uint32_t resident = std::min(ctx->numCUs * bpc, ctx->spillLanes / block);
 				for (uint32_t div : {4u, 2u, 1u})
@@ -988,7 +1028,7 @@ std::vector<Candidate> candidates(PhxContext* ctx, bool plan)
 						Candidate c;
 						c.params = PhxLaunchParams{std::max(1u, resident / div),
 						                           std::max(1u, resident / div), refill, block, top,
-						                           resident, helpers};
+						                           resident, helpers, 64u, 16u};
 						if (shapeFor(ctx, c.params, c.launch, plan) == PHX_OK) out.push_back(c);
 					}
 			}
@@ -996,10 +1036,10 @@ std::vector<Candidate> candidates(PhxContext* ctx, bool plan)
 	if (!plan) {
 		Candidate c;
 		c.launch.kernel = TraceKernel::Simple;
-		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0};
+		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0, 0, 0};
 		out.push_back(c);
 		c.launch.kernel = TraceKernel::Packet;
-		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0};
+		c.params = PhxLaunchParams{0, 0, 0, 256, 0, 0, 0, 0, 0};
 		out.push_back(c);
 	}
 	return out;
@@ -1039,6 +1079,40 @@ phx_status tune(PhxContext* ctx, std::vector<Candidate>& cands, int reps, Launch
 	return ok();
 }
 
+// Second stage for a persistent winner: rays per fetch, then the stack split, one at a time around
+// `best` (coordinate descent; the current value stays a candidate so `best` only improves). A
+// split that lowers the blocks a CU holds caps the grid at the new resident grid.
+template <class Launch>
+phx_status refine(PhxContext* ctx, const PhxPlan* plan, int reps, Launch launch, Candidate& best)
+{
+	if (best.launch.kernel != TraceKernel::Persistent) return PHX_OK;
+	for (int stage = 0; stage < 2; stage++) {
+		std::vector<Candidate> cands{best};
+		for (uint32_t v : stage == 0 ? std::vector<uint32_t>{16u, 32u, 64u, 128u, 256u}
+		                             : std::vector<uint32_t>{8u, 16u, 24u, 32u}) {
+			Candidate c = best;
+			if (stage == 0) {
+				if (v == best.params.raysPerFetch) continue;
+				c.params.raysPerFetch = v;
+			} else {
+				if (v == best.params.stackLdsEntries) continue;
+				c.params.stackLdsEntries = v;
+				const uint32_t bpc = blocksPerCU(plan, c.params.blockThreads, c.params.topNodes,
+				                                 c.params.helperWaves, v);
+				const uint32_t resident = std::min(ctx->numCUs * bpc, ctx->spillLanes / c.params.blockThreads);
+				if (resident == 0) continue;
+				c.params.gridBlocksClosest = std::min(c.params.gridBlocksClosest, resident);
+				c.params.gridBlocksAnyhit = std::min(c.params.gridBlocksAnyhit, resident);
+				c.params.residentGridBlocks = resident;
+			}
+			if (shapeFor(ctx, c.params, c.launch, plan) == PHX_OK) cands.push_back(c);
+		}
+		phx_status st = tune(ctx, cands, reps, launch, best);
+		if (st != PHX_OK) return st;
+	}
+	return PHX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1055,14 +1129,16 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 	const bool anyhit = (flags & PHX_AUTOTUNE_ANYHIT) != 0;
 	void* out = nullptr;
 	PHX_HIP(hipMalloc(&out, size_t(numRays) * (anyhit ? 1 : sizeof(PhxRayHit))), "phx_autotune");
-	std::vector<Candidate> cands = candidates(ctx, false);
+	std::vector<Candidate> cands = candidates(ctx, nullptr);
 	SceneView sv = view(scene);
 	Candidate best;
-	st = tune(ctx, cands, 3, [&](const TraceLaunch& L) {
+	auto cast = [&](const TraceLaunch& L) {
 		return anyhit ? launchAnyHit(sv, rays, numRays, static_cast<uint8_t*>(out), L, ctx->stream)
 		              : launchClosest(sv, rays, numRays, static_cast<PhxRayHit*>(out), L, nullptr,
 		                              ctx->stream);
-	}, best);
+	};
+	st = tune(ctx, cands, 3, cast, best);
+	if (st == PHX_OK) st = refine(ctx, nullptr, 3, cast, best);
 	hipStreamSynchronize(ctx->stream);
 	hipFree(out);
 	if (st != PHX_OK) return st;
@@ -1077,6 +1153,8 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 			ctx->gridClosest = best.launch.gridBlocks;
 		ctx->refillMin = best.launch.refillMin;
 		ctx->helperWaves = best.launch.helperWaves;
+		ctx->ldsDepth = best.launch.ldsDepth;
+		ctx->window = best.launch.window;
 	}
 	if (chosen) {
 		*chosen = best.params;
@@ -1095,7 +1173,7 @@ phx_status phx_plan_set_launch_params(PhxContext* ctx, PhxPlan* plan, const PhxL
 		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
 	if (!p) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_plan_set_launch_params: NULL params");
 	TraceLaunch L;
-	st = shapeFor(ctx, *p, L, true);
+	st = shapeFor(ctx, *p, L, plan);
 	if (st != PHX_OK) return st;
 	plan->launch = L;
 	if (plan->graphExec) {
@@ -1113,12 +1191,15 @@ phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* ch
 	if (st != PHX_OK) return st;
 	if (!plan || !registry().has(plan) || plan->ctx != ctx)
 		return fail(PHX_ERR_INVALID_HANDLE, "invalid plan handle");
-	std::vector<Candidate> cands = candidates(ctx, true);
+	std::vector<Candidate> cands = candidates(ctx, plan);
 	Candidate best;
-	st = tune(ctx, cands, 2, [&](const TraceLaunch& L) {
+	// Timed as the plan launches: with its completion tracking, whose LDS can change residency.
+	auto run = [&](const TraceLaunch& L) {
 		return launchBatches(plan->descs, plan->offsets, plan->numBatches, plan->total, L,
-		                     plan->treeletNodes, plan->treeletCount, nullptr, ctx->stream);
-	}, best);
+		                     plan->treeletNodes, plan->treeletCount, plan->clock, ctx->stream);
+	};
+	st = tune(ctx, cands, 2, run, best);
+	if (st == PHX_OK) st = refine(ctx, plan, 2, run, best);
 	if (st != PHX_OK) return st;
 	st = phx_plan_set_launch_params(ctx, plan, &best.params);
 	if (st != PHX_OK) return st;

@@ -49,6 +49,8 @@ struct TraceLaunch {
 	uint32_t refillMin = 8;      // refill a wave once this many of its lanes are idle
 	uint32_t spillLanes = 0;     // lanes the spill area is sized for
 	uint32_t helperWaves = 0;    // extra waves per block that only trace offered rays solo
+	uint32_t ldsDepth = 16;      // stack entries per lane held in LDS (the rest spill to HBM)
+	uint32_t window = 64;        // rays a wave claims per fetch from the shared counter
 	uint32_t* work = nullptr;
 	uint32_t* spill = nullptr;
 };
@@ -56,10 +58,13 @@ struct TraceLaunch {
 // Words of the stack spill area for `lanes` resident lanes.
 uint32_t persistentSpillWords(uint32_t lanes);
 // Dynamic LDS bytes of a persistent block.
-uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes);
-// Resident persistent blocks per CU for a block size and treelet size (0 if it does not fit).
+uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t ldsDepth);
+// True if `depth` LDS stack entries per lane is a supported split (4..32).
+bool validLdsDepth(uint32_t depth);
+// Resident persistent blocks per CU for a block size, treelet size, helper waves and LDS stack
+// depth (0 if it does not fit).
 hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
-                                 uint32_t* blocks);
+                                 uint32_t ldsDepth, uint32_t* blocks);
 
 // Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
 // counters of the batch are added to counters[0..4] (device memory).
@@ -101,7 +106,12 @@ struct BatchClock {
 constexpr uint32_t kMaxFusedBatches = 1024;
 
 uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches,
-                       bool tracked);
+                       bool tracked, uint32_t ldsDepth);
+// Resident blocks per CU of the fused kernel for a shape and batch count, with or without
+// completion tracking (0 if it does not fit).
+hipError_t batchBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
+                            uint32_t ldsDepth, uint32_t numBatches, bool tracked,
+                            uint32_t* blocks);
 
 // One persistent launch over `numBatches` batches; offsets[b] = first ray of batch b in the fused
 // index space, offsets[numBatches] = total. `clock` may be null (no completion tracking).

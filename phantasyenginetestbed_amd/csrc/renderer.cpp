@@ -44,7 +44,7 @@ struct PhxRenderer {
 		uint32_t flags;  // PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP
 		uint32_t numLights;
 		const void* lightBuffers;
-		uint32_t launch[5];  // the context's launch shape
+		uint32_t launch[8];  // the context's launch shape
 	};
 	GraphKey key{};
 	hipGraph_t graph = nullptr;
@@ -301,8 +301,9 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 		k.flags = flags & (PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP);
 		k.numLights = scene->numLights;
 		k.lightBuffers = r->contrib;
-		const uint32_t launch[5] = {ctx->gridClosest, ctx->gridAnyhit, ctx->blockThreads,
-		                            ctx->topNodes, ctx->refillMin};
+		const uint32_t launch[8] = {ctx->gridClosest, ctx->gridAnyhit, ctx->blockThreads,
+		                            ctx->topNodes,      ctx->refillMin, ctx->helperWaves,
+		                            ctx->ldsDepth,      ctx->window};
 		std::memcpy(k.launch, launch, sizeof(launch));
 		if (!r->graphExec || std::memcmp(&k, &r->key, sizeof(k)) != 0) {
 			dropGraph(r);

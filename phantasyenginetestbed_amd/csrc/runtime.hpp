@@ -71,6 +71,8 @@ struct PhxContext {
 	uint32_t topNodes = 0;
 	uint32_t refillMin = 8;
 	uint32_t helperWaves = 0;
+	uint32_t ldsDepth = 16;   // stack entries per lane kept in LDS
+	uint32_t window = 64;     // rays per fetch
 	uint32_t spillLanes = 0;
 	// Staging buffers for calls that pass host pointers, grown on demand.
 	void* stageIn = nullptr;
@@ -146,6 +148,8 @@ inline TraceLaunch traceLaunch(const PhxContext* ctx, uint32_t flags, bool anyhi
 	L.refillMin = ctx->refillMin;
 	L.spillLanes = ctx->spillLanes;
 	L.helperWaves = ctx->helperWaves;
+	L.ldsDepth = ctx->ldsDepth;
+	L.window = ctx->window;
 	L.work = ctx->work;
 	L.spill = ctx->spill;
 	return L;

@@ -353,12 +353,16 @@ phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t num
 	s->flags = flags;
 	s->budget = budgetBytes;
 	s->shape = traceLaunch(ctx, 0, false);
-	// Continuous batching keeps every block busy: the resident grid of lane-mode blocks, no
-	// treelet, no helper waves (which would cut the blocks a CU holds).
+	// Continuous batching keeps every block busy: the resident grid of lane-mode blocks of the
+	// fused kernel at its largest batch count, no treelet, no helper waves (which would cut the
+	// blocks a CU holds), the default stack split and rays per fetch (a context's tuned values
+	// are for its single casts).
 	s->shape.topNodes = 0;
 	s->shape.helperWaves = 0;
+	s->shape.ldsDepth = 16;
+	s->shape.window = 64;
 	uint32_t bpc = 0;
-	persistentBlocksPerCU(s->shape.blockThreads, 0, 0, &bpc);
+	batchBlocksPerCU(s->shape.blockThreads, 0, 0, 16, kMaxFusedBatches, false, &bpc);
 	s->shape.gridBlocks = std::max(1u, ctx->numCUs * bpc);
 	s->shape.spillLanes = s->shape.gridBlocks * s->shape.blockThreads;
 	s->slots.resize(numSlots);

@@ -202,23 +202,24 @@ __global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restri
 // ---- persistent wavefronts ----------------------------------------------------------------------
 //
 // A fixed grid of waves pulls rays from a shared counter, one atomic per wavefront for a window
-// of kWindow rays. Every iteration, lanes without a ray are refilled from the window when at least
-// `refillMin` of the wave's 64 lanes are idle (always when all are). The last wave to finish resets
-// the counter, so every launch starts from zero without a memset.
+// of `window` rays (rays per fetch, a launch parameter). Every iteration, lanes without a ray are
+// refilled from the window when at least `refillMin` of the wave's 64 lanes are idle (always when
+// all are). The last wave to finish resets the counter, so every launch starts from zero without a
+// memset.
 //
 // LDS (dynamic) holds, per block:
 //   - castRay's pending stack, [wave][entry][lane] (one 4-B column per lane, so each push/pop is a
-//     conflict-free 256-B row), kLdsDepth entries deep; deeper entries spill to a global array laid
-//     out [entry][lane] (coalesced). Trees are at most 64 levels deep (checked at upload), so the
+//     conflict-free 256-B row), `depth` entries deep (the stack split, a launch parameter) plus
+//     one row that lanes without a push write to; deeper entries spill to a global array laid out
+//     [entry][lane] (coalesced). Trees are at most 64 levels deep (checked at upload), so the
 //     pending stack never exceeds kPersistentStack entries;
 //   - the top treelet: nodes [0, topCount) of the device node array, which the upload lays out in
 //     breadth-first order so they are the top levels of the tree -- where most node visits of
 //     incoherent rays happen. They are copied once per block and read from LDS afterwards.
 
-constexpr int kLdsDepth = 16;
-constexpr int kLdsRows = kLdsDepth + 1;  // + one row that lanes without a push write to
+constexpr int kMinLdsDepth = 4;    // smallest stack split; the spill area is sized for it
+constexpr int kMaxLdsDepth = 32;
 constexpr int kPersistentStack = 72;
-constexpr uint32_t kWindow = 64;
 constexpr int kMaxBlock = 1024;
 
 struct WaveQueue {
@@ -230,7 +231,7 @@ struct WaveQueue {
 // Assigns ray indices to the idle lanes of the wave (`idle` = ballot of idle lanes). Returns the
 // lane's new ray index, or kInvalid. Wave-uniform state lives in `q`.
 __device__ __forceinline__ uint32_t refill(WaveQueue& q, uint64_t idle, uint32_t* work,
-                                           uint32_t n, bool isIdle)
+                                           uint32_t n, bool isIdle, uint32_t window)
 {
 	const uint32_t need = uint32_t(__popcll(idle));
 	const uint32_t take1 = need < q.left ? need : q.left;
@@ -241,13 +242,13 @@ __device__ __forceinline__ uint32_t refill(WaveQueue& q, uint64_t idle, uint32_t
 	if (need > take1) {
 		const int leader = __ffsll((unsigned long long)idle) - 1;
 		uint32_t b = 0;
-		if (int(__lane_id()) == leader) b = atomicAdd(work, kWindow);
+		if (int(__lane_id()) == leader) b = atomicAdd(work, window);
 		b = __shfl(b, leader);
 		if (b >= n) {
 			q.exhausted = true;
 			q.left = 0;
 		} else {
-			const uint32_t avail = (n - b) < kWindow ? (n - b) : kWindow;
+			const uint32_t avail = (n - b) < window ? (n - b) : window;
 			take2 = (need - take1) < avail ? (need - take1) : avail;
 			base2 = b;
 			q.base = b + take2;
@@ -275,20 +276,21 @@ __device__ __forceinline__ void retire_wave(uint32_t* work)
 }
 
 struct LaneStack {
-	uint32_t* lds;    // this wave's [entry][lane] block
+	uint32_t* lds;    // this wave's [entry][lane] block: depth + 1 rows
 	uint32_t* spill;  // this lane's spill column
 	uint32_t stride;  // spill row stride (lanes in the grid)
 	uint32_t lane;
+	int depth;        // entries kept in LDS
 	int sp;
 
 	// Pushes v if doPush. Lanes that do not push (or whose entry spills) store to the extra row, so
 	// the LDS store needs no branch; only a spilling entry takes one.
 	__device__ __forceinline__ void push_if(bool doPush, uint32_t v)
 	{
-		const bool toLds = doPush && sp < kLdsDepth;
-		lds[(toLds ? sp : kLdsDepth) * 64 + lane] = v;
-		if (doPush && sp >= kLdsDepth && sp < kPersistentStack)
-			spill[size_t(sp - kLdsDepth) * stride] = v;
+		const bool toLds = doPush && sp < depth;
+		lds[(toLds ? sp : depth) * 64 + int(lane)] = v;
+		if (doPush && sp >= depth && sp < kPersistentStack)
+			spill[size_t(sp - depth) * stride] = v;
 		sp += doPush ? 1 : 0;
 	}
 	// Pops if doPop (the value is meaningless otherwise): every lane reads one LDS word -- lanes
@@ -297,9 +299,9 @@ struct LaneStack {
 	__device__ __forceinline__ uint32_t pop_if(bool doPop)
 	{
 		sp -= doPop ? 1 : 0;
-		uint32_t v = lds[(doPop && sp < kLdsDepth ? sp : kLdsDepth) * 64 + lane];
-		if (doPop && sp >= kLdsDepth)
-			v = sp < kPersistentStack ? spill[size_t(sp - kLdsDepth) * stride] : 0u;
+		uint32_t v = lds[(doPop && sp < depth ? sp : depth) * 64 + int(lane)];
+		if (doPop && sp >= depth)
+			v = sp < kPersistentStack ? spill[size_t(sp - depth) * stride] : 0u;
 		return v;
 	}
 };
@@ -342,9 +344,10 @@ __host__ __device__ __forceinline__ uint32_t handoff_bytes(uint32_t blockThreads
 
 // LDS bytes of a persistent launch: the stacks of every wave, the treelet, the handoff area.
 __host__ __device__ __forceinline__ uint32_t persistent_lds_bytes(uint32_t blockThreads,
-                                                                  uint32_t topCount)
+                                                                  uint32_t topCount,
+                                                                  uint32_t depth)
 {
-	return (blockThreads / 64u) * uint32_t(kLdsRows) * 64u * 4u + topCount * 64u +
+	return (blockThreads / 64u) * (depth + 1u) * 64u * 4u + topCount * 64u +
 	       handoff_bytes(blockThreads);
 }
 
@@ -484,6 +487,7 @@ struct PipeArgs {
 	bool finiteBounds;
 	const float4* treelet;  // LDS copy of nodes [0, topCount)
 	const uint32_t* handoff;  // the block's handoff area (LDS)
+	uint32_t window;          // rays per fetch
 };
 
 // Loop-carried state of one lane.
@@ -661,7 +665,7 @@ __device__ __forceinline__ int pipe_step(const PipeArgs& P, Tracer& L, LaneStack
 	if (!q.exhausted) {
 		const uint64_t freeLanes = __ballot(!more);
 		if (freeLanes != 0 && (uint32_t(__popcll(freeLanes)) >= P.refillMin || freeLanes == ~0ull))
-			fresh = refill(q, freeLanes, P.work, P.n, !more);
+			fresh = refill(q, freeLanes, P.work, P.n, !more, P.window);
 	}
 	const bool tri = more && L.s.triCur < L.s.triEnd;
 	const bool node = more && !tri;
@@ -802,8 +806,9 @@ struct SoloStart {
 	uint32_t rayIdx;
 	uint32_t kind;                // kRecRay: the ray is not set up yet (load it from rays[rayIdx])
 	const uint32_t* ldsCol;       // the owner lane's LDS stack column (entry e at ldsCol[64 e])
-	const uint32_t* spillCol;     // its spill column (entry e >= kLdsDepth at [(e - kLdsDepth) stride])
+	const uint32_t* spillCol;     // its spill column (entry e >= depth at [(e - depth) stride])
 	uint32_t stride;
+	int depth;
 };
 
 // The start state of lane lsel of this wave.
@@ -827,6 +832,7 @@ __device__ __forceinline__ SoloStart solo_start_lane(const Tracer& L, const Lane
 	S.ldsCol = st.lds + lsel;
 	S.spillCol = st.spill - __lane_id() + lsel;
 	S.stride = st.stride;
+	S.depth = st.depth;
 	return S;
 }
 
@@ -863,7 +869,7 @@ __device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 	for (int e = int(lane), h = 0; h < 2; h++, e += 64) {
 		uint32_t v = 0u;
 		if (e < sp)
-			v = e < kLdsDepth ? S.ldsCol[e * 64] : S.spillCol[size_t(e - kLdsDepth) * S.stride];
+			v = e < S.depth ? S.ldsCol[e * 64] : S.spillCol[size_t(e - S.depth) * S.stride];
 		if (h == 0)
 			stk0 = v;
 		else
@@ -1038,7 +1044,7 @@ __device__ __forceinline__ uint32_t handoff_claim(uint32_t* H)
 __device__ __forceinline__ SoloStart solo_start_slot(const uint32_t* H, uint32_t slot,
                                                      const uint32_t* ldsStacks,
                                                      const uint32_t* spill, uint32_t stride,
-                                                     uint32_t& batch)
+                                                     int depth, uint32_t& batch)
 {
 	const uint32_t* w = H + 4u + slot * kSlotWords;
 	while (lane_u(lds_load(&w[22]), 0u) == 0u) __builtin_amdgcn_s_sleep(1);
@@ -1060,10 +1066,11 @@ __device__ __forceinline__ SoloStart solo_start_slot(const uint32_t* H, uint32_t
 	S.rayIdx = v[18];
 	S.kind = v[19];
 	const uint32_t wave = v[20] >> 6, lane = v[20] & 63u;
-	S.ldsCol = ldsStacks + wave * uint32_t(kLdsRows) * 64u + lane;
+	S.ldsCol = ldsStacks + wave * (uint32_t(depth) + 1u) * 64u + lane;
 	// Spill columns are numbered over the grid's lane-mode threads (`stride` of them).
 	S.spillCol = spill + (blockIdx.x * (stride / gridDim.x) + wave * 64u + lane);
 	S.stride = stride;
+	S.depth = depth;
 	batch = v[21];
 	return S;
 }
@@ -1084,14 +1091,14 @@ __device__ __forceinline__ void persistent_trace(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ inLight,
 	uint32_t n, uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	bool finiteBounds, uint32_t topCount, uint32_t helpers,
+	bool finiteBounds, uint32_t topCount, uint32_t helpers, int depth, uint32_t window,
 	unsigned long long* __restrict__ counters)
 {
 	extern __shared__ float4 ldsRaw[];
 	const uint32_t laneWaves = blockDim.x / 64u - helpers;
 	const uint32_t laneThreads = laneWaves * 64u;
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
-	float4* treelet = ldsRaw + laneWaves * uint32_t(kLdsRows) * 64u / 4u;
+	float4* treelet = ldsRaw + laneWaves * (uint32_t(depth) + 1u) * 64u / 4u;
 	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
 	if (kTreelet)
 		for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
@@ -1105,21 +1112,23 @@ __device__ __forceinline__ void persistent_trace(
 		__builtin_amdgcn_s_setprio(3);
 		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
 			uint32_t batch;
-			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, stride, batch));
+			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, stride, depth,
+			                                     batch));
 		}
 		retire_wave(work);
 		return;
 	}
 
 	LaneStack st;
-	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
+	st.lds = ldsStacks + (threadIdx.x >> 6) * (uint32_t(depth) + 1u) * 64u;
+	st.depth = depth;
 	st.stride = stride;
 	st.spill = spill + (blockIdx.x * laneThreads + threadIdx.x);
 	st.lane = __lane_id();
 	st.sp = 0;
 
 	const PipeArgs P = {nodes, tris, rays, hits, inLight, n, work, refillMin,
-	                    kTreelet ? topCount : 0u, finiteBounds, treelet, handoff};
+	                    kTreelet ? topCount : 0u, finiteBounds, treelet, handoff, window};
 	WaveQueue q;
 	Tracer L;
 	L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
@@ -1146,7 +1155,8 @@ __device__ __forceinline__ void persistent_trace(
 		if (busy != 0) solo_run<kAnyHit>(S, solo_start_lane(L, st, uint32_t(__builtin_ctzll(busy))));
 		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
 			uint32_t batch;
-			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, batch));
+			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, depth,
+			                                     batch));
 		}
 	}
 	if (kCount) add_counts(counters, k);
@@ -1157,12 +1167,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
 	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers,
+	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers, uint32_t depth, uint32_t window,
 	unsigned long long* __restrict__ counters)
 {
 	persistent_trace<false, kCount, kTreelet>(nodes, tris, rays, hits, nullptr, n, work, spill,
 	                                          refillMin, finiteBounds != 0, topCount, helpers,
-	                                          counters);
+	                                          int(depth), window, counters);
 }
 
 template <bool kTreelet>
@@ -1170,11 +1180,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_anyhit_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint8_t* __restrict__ inLight, uint32_t n,
 	uint32_t* __restrict__ work, uint32_t* __restrict__ spill, uint32_t refillMin,
-	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers)
+	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers, uint32_t depth, uint32_t window)
 {
 	persistent_trace<true, false, kTreelet>(nodes, tris, rays, nullptr, inLight, n, work, spill,
 	                                        refillMin, finiteBounds != 0, topCount, helpers,
-	                                        nullptr);
+	                                        int(depth), window, nullptr);
 }
 
 // ---- fused multi-batch launch ---------------------------------------------------------------------
@@ -1220,6 +1230,7 @@ struct MultiArgs {
 	unsigned long long* blockFinish;  // LDS, per batch (completion tracking)
 	uint32_t* blockDone;              // LDS, per batch
 	const uint32_t* handoff;          // the block's handoff area (LDS)
+	uint32_t window;                  // rays per fetch
 };
 
 // Per-lane state of the fused kernel: the pipelined loop's, plus the scene, output and kind of
@@ -1290,7 +1301,7 @@ __device__ __forceinline__ int pipe_step_multi(const MultiArgs& P, MultiTracer& 
 	if (!q.exhausted) {
 		const uint64_t freeLanes = __ballot(!more);
 		if (freeLanes != 0 && (uint32_t(__popcll(freeLanes)) >= P.refillMin || freeLanes == ~0ull))
-			fresh = refill(q, freeLanes, P.work, P.total, !more);
+			fresh = refill(q, freeLanes, P.work, P.total, !more, P.window);
 	}
 	const bool tri = more && L.s.triCur < L.s.triEnd;
 	const bool node = more && !tri;
@@ -1361,14 +1372,14 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
                                               uint32_t* __restrict__ work,
                                               uint32_t* __restrict__ spill, uint32_t refillMin,
                                               const float4* __restrict__ treeletNodes,
-                                              uint32_t topCount, uint32_t helpers,
-                                              BatchClock* __restrict__ clock)
+                                              uint32_t topCount, uint32_t helpers, int depth,
+                                              uint32_t window, BatchClock* __restrict__ clock)
 {
 	extern __shared__ float4 ldsRaw[];
 	const uint32_t laneWaves = blockDim.x / 64u - helpers;
 	const uint32_t laneThreads = laneWaves * 64u;
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
-	float4* treelet = ldsRaw + laneWaves * uint32_t(kLdsRows) * 64u / 4u;
+	float4* treelet = ldsRaw + laneWaves * (uint32_t(depth) + 1u) * 64u / 4u;
 	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
 	unsigned long long* blockFinish =
 		reinterpret_cast<unsigned long long*>(handoff + handoff_bytes(laneThreads) / 4u);
@@ -1387,12 +1398,12 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 
 	const MultiArgs P = {descs, offsets, numBatches, total, work, refillMin,
 	                     kTreelet ? topCount : 0u, treeletNodes, treelet, blockFinish, blockDone,
-	                     handoff};
+	                     handoff, window};
 	const uint32_t stride = gridDim.x * laneThreads;
 	// Traces the ray of a claimed slot solo.
 	auto runSlot = [&](uint32_t slot) {
 		uint32_t b;
-		const SoloStart S0 = solo_start_slot(handoff, slot, ldsStacks, spill, stride, b);
+		const SoloStart S0 = solo_start_slot(handoff, slot, ldsStacks, spill, stride, depth, b);
 		const BatchDesc d = descs[b];
 		const SoloScene S = {d.nodes, d.tris, d.rays, d.out};
 		if (d.kind != 0u)
@@ -1409,7 +1420,8 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 	}
 
 	LaneStack st;
-	st.lds = ldsStacks + (threadIdx.x >> 6) * uint32_t(kLdsRows) * 64u;
+	st.lds = ldsStacks + (threadIdx.x >> 6) * (uint32_t(depth) + 1u) * 64u;
+	st.depth = depth;
 	st.stride = stride;
 	st.spill = spill + (blockIdx.x * laneThreads + threadIdx.x);
 	st.lane = __lane_id();
@@ -1489,10 +1501,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace_batches(
 	const BatchDesc* __restrict__ descs, const uint32_t* __restrict__ offsetsGlobal,
 	uint32_t numBatches, uint32_t total, uint32_t* __restrict__ work,
 	uint32_t* __restrict__ spill, uint32_t refillMin, const float4* __restrict__ treeletNodes,
-	uint32_t topCount, uint32_t helpers, BatchClock* __restrict__ clock)
+	uint32_t topCount, uint32_t helpers, uint32_t depth, uint32_t window,
+	BatchClock* __restrict__ clock)
 {
 	trace_batches<kTreelet, kTrack>(descs, offsetsGlobal, numBatches, total, work, spill, refillMin,
-	                                treeletNodes, topCount, helpers, clock);
+	                                treeletNodes, topCount, helpers, int(depth), window, clock);
 }
 
 // ---- device self-check of the fast reciprocal -----------------------------------------------------
@@ -1531,29 +1544,35 @@ void persistentShape(const TraceLaunch& L, const SceneView& s, uint32_t n, uint3
 	const uint32_t maxGrid = L.spillLanes / L.blockThreads;
 	if (grid > maxGrid) grid = maxGrid;
 	if (grid == 0) grid = 1;
-	ldsBytes = persistent_lds_bytes(L.blockThreads, top);
+	ldsBytes = persistent_lds_bytes(L.blockThreads, top, L.ldsDepth);
 }
 
 }  // namespace
 
 uint32_t persistentSpillWords(uint32_t lanes)
 {
-	return uint32_t(kPersistentStack - kLdsDepth) * lanes;
+	return uint32_t(kPersistentStack - kMinLdsDepth) * lanes;
 }
 
-uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes)
+bool validLdsDepth(uint32_t depth)
 {
-	return persistent_lds_bytes(blockThreads, topNodes);
+	return depth >= uint32_t(kMinLdsDepth) && depth <= uint32_t(kMaxLdsDepth);
+}
+
+uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t ldsDepth)
+{
+	return persistent_lds_bytes(blockThreads, topNodes, ldsDepth);
 }
 
 hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
-                                 uint32_t* blocks)
+                                 uint32_t ldsDepth, uint32_t* blocks)
 {
 	int a = 0;
 	const void* fn = topNodes ? reinterpret_cast<const void*>(k_closest_persistent<false, true>)
 	                          : reinterpret_cast<const void*>(k_closest_persistent<false, false>);
 	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-		&a, fn, int(blockThreads + 64u * helperWaves), persistent_lds_bytes(blockThreads, topNodes));
+		&a, fn, int(blockThreads + 64u * helperWaves),
+		persistent_lds_bytes(blockThreads, topNodes, ldsDepth));
 	*blocks = uint32_t(a > 0 ? a : 0);
 	return e;
 }
@@ -1598,7 +1617,7 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	                       : (top ? k_closest_persistent<false, true> : k_closest_persistent<false, false>);
 	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, nodes, tris, r4, h4, n,
 	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, shape.helperWaves,
-	                   counters);
+	                   L.ldsDepth, L.window, counters);
 	return hipGetLastError();
 }
 
@@ -1623,15 +1642,32 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 	persistentShape(L, s, n, grid, block, top, lds);
 	hipLaunchKernelGGL(top ? k_anyhit_persistent<true> : k_anyhit_persistent<false>, dim3(grid),
 	                   dim3(block), lds, stream, nodes, tris, r4, inLight, n, L.work, L.spill,
-	                   L.refillMin, s.finiteBounds, top, L.helperWaves);
+	                   L.refillMin, s.finiteBounds, top, L.helperWaves, L.ldsDepth, L.window);
 	return hipGetLastError();
 }
 
 uint32_t batchLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t numBatches,
-                       bool tracked)
+                       bool tracked, uint32_t ldsDepth)
 {
-	return persistent_lds_bytes(blockThreads, topNodes) + (numBatches + 1u) * 4u +
+	return persistent_lds_bytes(blockThreads, topNodes, ldsDepth) + (numBatches + 1u) * 4u +
 	       (tracked ? numBatches * 12u : 0u);
+}
+
+hipError_t batchBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
+                            uint32_t ldsDepth, uint32_t numBatches, bool tracked,
+                            uint32_t* blocks)
+{
+	int a = 0;
+	const void* fn =
+		tracked ? (topNodes ? reinterpret_cast<const void*>(k_trace_batches<true, true>)
+		                    : reinterpret_cast<const void*>(k_trace_batches<false, true>))
+		        : (topNodes ? reinterpret_cast<const void*>(k_trace_batches<true, false>)
+		                    : reinterpret_cast<const void*>(k_trace_batches<false, false>));
+	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+		&a, fn, int(blockThreads + 64u * helperWaves),
+		batchLdsBytes(blockThreads, topNodes, numBatches, tracked, ldsDepth));
+	*blocks = uint32_t(a > 0 ? a : 0);
+	return e;
 }
 
 hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32_t numBatches,
@@ -1647,9 +1683,9 @@ hipError_t launchBatches(const BatchDesc* descs, const uint32_t* offsets, uint32
 	auto kernel = clock ? (top ? k_trace_batches<true, true> : k_trace_batches<false, true>)
 	                    : (top ? k_trace_batches<true, false> : k_trace_batches<false, false>);
 	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block + 64u * L.helperWaves),
-	                   batchLdsBytes(block, top, numBatches, clock != nullptr), stream, descs,
-	                   offsets, numBatches, total, L.work, L.spill, L.refillMin, treeletNodes, top,
-	                   L.helperWaves, clock);
+	                   batchLdsBytes(block, top, numBatches, clock != nullptr, L.ldsDepth), stream,
+	                   descs, offsets, numBatches, total, L.work, L.spill, L.refillMin, treeletNodes,
+	                   top, L.helperWaves, L.ldsDepth, L.window, clock);
 	return hipGetLastError();
 }
 

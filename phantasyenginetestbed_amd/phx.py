@@ -124,7 +124,8 @@ class LaunchParams(C.Structure):
     _fields_ = [("gridBlocksClosest", C.c_uint32), ("gridBlocksAnyhit", C.c_uint32),
                 ("refillMin", C.c_uint32), ("blockThreads", C.c_uint32),
                 ("topNodes", C.c_uint32), ("residentGridBlocks", C.c_uint32),
-                ("helperWaves", C.c_uint32)]
+                ("helperWaves", C.c_uint32), ("raysPerFetch", C.c_uint32),
+                ("stackLdsEntries", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -474,8 +475,10 @@ class Plan:
         return p.as_dict()
 
     def set_launch_params(self, grid: int, block_threads: int = 256, top_nodes: int = 0,
-                          refill_min: int = 8, helper_waves: int = 0) -> None:
-        p = LaunchParams(grid, 0, refill_min, block_threads, top_nodes, 0, helper_waves)
+                          refill_min: int = 8, helper_waves: int = 0, rays_per_fetch: int = 0,
+                          stack_lds_entries: int = 0) -> None:
+        p = LaunchParams(grid, 0, refill_min, block_threads, top_nodes, 0, helper_waves,
+                         rays_per_fetch, stack_lds_entries)
         check(lib().phx_plan_set_launch_params(self.ctx.handle, self.handle, C.byref(p)))
 
     def batch_times_ms(self) -> np.ndarray:
@@ -628,10 +631,12 @@ class Context:
 
     def set_launch_params(self, grid_closest: int = 0, grid_anyhit: int = 0,
                           refill_min: int = 0, block_threads: int = 0,
-                          top_nodes: int | None = None, helper_waves: int | None = None) -> None:
+                          top_nodes: int | None = None, helper_waves: int | None = None,
+                          rays_per_fetch: int = 0, stack_lds_entries: int = 0) -> None:
         p = LaunchParams(grid_closest, grid_anyhit, refill_min, block_threads,
                          0xFFFFFFFF if top_nodes is None else top_nodes, 0,
-                         0xFFFFFFFF if helper_waves is None else helper_waves)
+                         0xFFFFFFFF if helper_waves is None else helper_waves,
+                         rays_per_fetch, stack_lds_entries)
         check(lib().phx_set_launch_params(self.handle, C.byref(p)))
 
     def check_reciprocal(self, exhaustive: bool = False) -> tuple[int, int]:

@@ -362,28 +362,46 @@ def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
     assert tested == 2 * 4 * (1 << 23) and bad == 0
 
 
+def restore_launch(ctx, old):
+    ctx.set_launch_params(grid_closest=old["gridBlocksClosest"],
+                          grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
+                          block_threads=old["blockThreads"], top_nodes=old["topNodes"],
+                          helper_waves=old["helperWaves"], rays_per_fetch=old["raysPerFetch"],
+                          stack_lds_entries=old["stackLdsEntries"])
+
+
 def test_launch_params_reject_oversized_blocks(ctx):
-    """Lane-mode threads plus helper waves may not exceed a 1024-thread block; a rejected call
-    leaves the shape unchanged."""
+    """Lane-mode threads plus helper waves may not exceed a 1024-thread block, rays per fetch
+    stay in 8..4096 and the LDS stack split in 4..32; a rejected call leaves the shape
+    unchanged."""
     old = ctx.launch_params()
+    assert old["raysPerFetch"] == 64 and old["stackLdsEntries"] == 16
     for block, helpers in ((1024, 1), (512, 9), (64, 0x7FFFFFFF), (64, 0xFFFFFFFE)):
         with pytest.raises(phx.PhxError) as e:
             ctx.set_launch_params(block_threads=block, helper_waves=helpers)
         assert e.value.status == phx.ERR_INVALID_ARGUMENT
+    for fetch, depth in ((4, 0), (4097, 0), (0, 3), (0, 33), (0, 0xFFFFFFFF)):
+        with pytest.raises(phx.PhxError) as e:
+            ctx.set_launch_params(rays_per_fetch=fetch, stack_lds_entries=depth)
+        assert e.value.status == phx.ERR_INVALID_ARGUMENT
     assert ctx.launch_params() == old
 
 
-@pytest.mark.parametrize("shape", [(512, 256, 512, 8, 0), (300, 256, 1024, 32, 0),
-                                   (128, 1024, 0, 16, 0), (256, 512, 0, 32, 4),
-                                   (64, 256, 512, 8, 12)],
+@pytest.mark.parametrize("shape", [(512, 256, 512, 8, 0, 64, 16), (300, 256, 1024, 32, 0, 64, 16),
+                                   (128, 1024, 0, 16, 0, 64, 16), (256, 512, 0, 32, 4, 64, 16),
+                                   (64, 256, 512, 8, 12, 64, 16), (256, 512, 0, 8, 4, 8, 4),
+                                   (256, 512, 0, 8, 0, 256, 32), (200, 256, 512, 8, 4, 4096, 24)],
                          ids=["b256-treelet512", "b256-treelet1024", "b1024-notreelet",
-                              "b512-helpers4", "b256-treelet512-helpers12"])
+                              "b512-helpers4", "b256-treelet512-helpers12",
+                              "fetch8-lds4-helpers4", "fetch256-lds32", "fetch4096-lds24-treelet"])
 def test_launch_shapes_match_oracle(ctx, c2, shape):
     """Other launch shapes -- the LDS-treelet kernels, larger blocks, helper waves that trace the
-    rays a block's lane-mode waves hand out -- give the oracle's hits, single-batch and fused."""
-    grid, block, top, refill, helpers = shape
+    rays a block's lane-mode waves hand out, rays per fetch from 8 to 4096 and LDS stack splits
+    from 4 entries (most of every deep stack in HBM) to 32 -- give the oracle's hits,
+    single-batch and fused."""
+    grid, block, top, refill, helpers, fetch, depth = shape
     say(f"[gpu] launch shape grid {grid} block {block} treelet {top} refill {refill} "
-        f"helpers {helpers}")
+        f"helpers {helpers} fetch {fetch} lds stack {depth}")
     b2, s2 = c2
     rng = np.random.default_rng(13)
     n = 40000
@@ -401,8 +419,11 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
     bufs = []
     try:
         ctx.set_launch_params(grid_closest=grid, grid_anyhit=grid, refill_min=refill,
-                              block_threads=block, top_nodes=top, helper_waves=helpers)
-        assert ctx.launch_params()["helperWaves"] == helpers
+                              block_threads=block, top_nodes=top, helper_waves=helpers,
+                              rays_per_fetch=fetch, stack_lds_entries=depth)
+        got = ctx.launch_params()
+        assert (got["helperWaves"], got["raysPerFetch"], got["stackLdsEntries"]) == \
+            (helpers, fetch, depth)
         assert_same_hits(ctx.cast_closest(s2, rays), ref, f"closest, shape {shape}")
         assert (ctx.cast_anyhit(s2, short) == ref_lit).all()
         rb, sb = ctx.alloc(n * 32).upload(rays), ctx.alloc(n * 32).upload(short)
@@ -412,7 +433,8 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
                               (s2, sb.ptr, lb.ptr, n, phx.BATCH_ANYHIT)])
         try:
             plan.set_launch_params(grid, block_threads=block, top_nodes=top, refill_min=refill,
-                                   helper_waves=helpers)
+                                   helper_waves=helpers, rays_per_fetch=fetch,
+                                   stack_lds_entries=depth)
             plan.launch()
             assert_same_hits(hb.download(phx.HIT, n), ref, f"fused, shape {shape}")
             assert (lb.download(np.uint8, n) == ref_lit).all()
@@ -421,10 +443,7 @@ def test_launch_shapes_match_oracle(ctx, c2, shape):
     finally:
         for b in bufs:
             b.free()
-        ctx.set_launch_params(grid_closest=old["gridBlocksClosest"],
-                              grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
-                              block_threads=old["blockThreads"], top_nodes=old["topNodes"],
-                              helper_waves=old["helperWaves"])
+        restore_launch(ctx, old)
 
 
 def test_launch_models_are_exclusive(ctx, c1):
@@ -462,7 +481,48 @@ def test_autotune_reports_packets_for_coherent_shadow_rays(ctx, c2):
         assert (ctx.cast_anyhit(s, sh, phx.KERNEL_PACKET) == O.cast_anyhit(b.nodes, b.tris, sh)).all()
     finally:
         buf.free()
-        ctx.set_launch_params(grid_closest=old["gridBlocksClosest"],
-                              grid_anyhit=old["gridBlocksAnyhit"], refill_min=old["refillMin"],
-                              block_threads=old["blockThreads"], top_nodes=old["topNodes"],
-                              helper_waves=old["helperWaves"])
+        restore_launch(ctx, old)
+
+
+def test_autotuners_apply_their_shapes(ctx, c2):
+    """The single-cast autotuner applies the shape it reports (rays per fetch and stack split
+    included) and the plan autotuner tunes the plan as it launches (with completion tracking);
+    the hits stay the oracle's under both."""
+    b, s = c2
+    rng = np.random.default_rng(29)
+    n = 60000
+    rays = np.zeros(n, phx.RAY)
+    rays["o"] = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmin"] = np.float32(1e-4)
+    rays["tmax"] = np.float32(phx.FLT_MAX)
+    ref = O.cast_closest(b.nodes, b.tris, rays)
+    old = ctx.launch_params()
+    rb = ctx.alloc(n * 32).upload(rays)
+    hb = ctx.alloc(n * 16)
+    try:
+        chosen = ctx.autotune(s, rb.ptr, n)
+        say(f"[gpu] autotune on {n} incoherent rays: {chosen}")
+        assert 8 <= chosen["raysPerFetch"] <= 4096 and 4 <= chosen["stackLdsEntries"] <= 32
+        if chosen["kernel"] == "persistent":
+            now = ctx.launch_params()
+            for k in ("blockThreads", "topNodes", "refillMin", "helperWaves", "raysPerFetch",
+                      "stackLdsEntries", "gridBlocksClosest"):
+                assert now[k] == chosen[k], k
+        flags = {"persistent": 0, "packet": phx.KERNEL_PACKET,
+                 "one-ray-per-lane": phx.KERNEL_ONE_RAY_PER_LANE}[chosen["kernel"]]
+        assert_same_hits(ctx.cast_closest(s, rays, flags), ref, "after autotune")
+        plan = phx.Plan(ctx, [(s, rb.ptr, hb.ptr, n, phx.BATCH_CLOSEST)], track=True)
+        try:
+            tuned = plan.autotune()
+            say(f"[gpu] plan autotune: {tuned}")
+            assert 8 <= tuned["raysPerFetch"] <= 4096 and 4 <= tuned["stackLdsEntries"] <= 32
+            plan.launch()
+            assert_same_hits(hb.download(phx.HIT, n), ref, "plan after autotune")
+        finally:
+            plan.release()
+    finally:
+        rb.free()
+        hb.free()
+        restore_launch(ctx, old)
