@@ -1045,8 +1045,11 @@ std::vector<Candidate> candidates(PhxContext* ctx, const PhxPlan* plan)
 	return out;
 }
 
-// Times `launch(L)` for every candidate (best of `reps` after one warm-up) with HIP events on the
-// context stream; returns the fastest.
+// Times `launch(L)` for every candidate with HIP events on the context stream and returns the
+// fastest. After one warm-up round, `reps` rounds each time every candidate once; a candidate's
+// time is its median round (single launches vary by a few percent, and a minimum would reward
+// the candidate with the luckiest draw). Interleaving the rounds spreads clock and cache drift
+// over all candidates instead of favouring those timed last.
 template <class Launch>
 phx_status tune(PhxContext* ctx, std::vector<Candidate>& cands, int reps, Launch launch,
                 Candidate& best)
@@ -1054,28 +1057,33 @@ phx_status tune(PhxContext* ctx, std::vector<Candidate>& cands, int reps, Launch
 	hipEvent_t a = nullptr, b = nullptr;
 	PHX_HIP(hipEventCreate(&a), "autotune");
 	hipError_t e = hipEventCreate(&b);
-	float bestMs = 3.0e38f;
-	for (Candidate& c : cands) {
+	for (Candidate& c : cands)
 		if (e == hipSuccess) e = launch(c.launch);
-		float m = 3.0e38f;
-		for (int r = 0; r < reps && e == hipSuccess; r++) {
+	std::vector<std::vector<float>> times(cands.size());
+	for (int r = 0; r < reps && e == hipSuccess; r++)
+		for (size_t i = 0; i < cands.size() && e == hipSuccess; i++) {
 			e = hipEventRecord(a, ctx->stream);
-			if (e == hipSuccess) e = launch(c.launch);
+			if (e == hipSuccess) e = launch(cands[i].launch);
 			if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
 			if (e == hipSuccess) e = hipEventSynchronize(b);
 			float ms = 0.0f;
 			if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-			m = std::min(m, ms);
+			if (e == hipSuccess) times[i].push_back(ms);
 		}
-		c.ms = m;
-		if (e == hipSuccess && m < bestMs) {
-			bestMs = m;
-			best = c;
-		}
-	}
 	hipEventDestroy(a);
 	if (b) hipEventDestroy(b);
 	if (e != hipSuccess) return hipFail(e, "autotune");
+	float bestMs = 3.0e38f;
+	for (size_t i = 0; i < cands.size(); i++) {
+		std::vector<float>& t = times[i];
+		std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+		cands[i].ms = t.empty() ? 3.0e38f : t[t.size() / 2];
+	}
+	for (const Candidate& c : cands)
+		if (c.ms < bestMs) {
+			bestMs = c.ms;
+			best = c;
+		}
 	return ok();
 }
 
@@ -1137,8 +1145,8 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 		              : launchClosest(sv, rays, numRays, static_cast<PhxRayHit*>(out), L, nullptr,
 		                              ctx->stream);
 	};
-	st = tune(ctx, cands, 3, cast, best);
-	if (st == PHX_OK) st = refine(ctx, nullptr, 3, cast, best);
+	st = tune(ctx, cands, 7, cast, best);
+	if (st == PHX_OK) st = refine(ctx, nullptr, 9, cast, best);
 	hipStreamSynchronize(ctx->stream);
 	hipFree(out);
 	if (st != PHX_OK) return st;
@@ -1198,8 +1206,8 @@ phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* ch
 		return launchBatches(plan->descs, plan->offsets, plan->numBatches, plan->total, L,
 		                     plan->treeletNodes, plan->treeletCount, plan->clock, ctx->stream);
 	};
-	st = tune(ctx, cands, 2, run, best);
-	if (st == PHX_OK) st = refine(ctx, plan, 2, run, best);
+	st = tune(ctx, cands, 3, run, best);
+	if (st == PHX_OK) st = refine(ctx, plan, 5, run, best);
 	if (st != PHX_OK) return st;
 	st = phx_plan_set_launch_params(ctx, plan, &best.params);
 	if (st != PHX_OK) return st;
