@@ -448,7 +448,7 @@ def main() -> int:
                      "traffic": profiled_traffic() if kernel == "persistent" else None,
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 256 "
                                        "blocks x (512 threads + 4 helper waves), no treelet, "
-                                       "refill 8, 128 rays per fetch, 8 LDS stack entries "
+                                       "refill 8, 128 rays per fetch, 16 LDS stack entries "
                                        "(profiles/closest_persistent_bounce1M.md)",
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
