@@ -86,9 +86,14 @@ def profiled_traffic():
     return None
 
 
-def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
+def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 24.0) -> dict:
     """The reference's own castRay (compiled from its sources into oracle/_ref) if present,
-    else the oracle's C restatement, on all host cores over a bounded sample of the rays."""
+    else the oracle's C restatement, over a bounded, evenly strided sample of the rays:
+    - value: the parity build (-O3 -ffp-contract=off) on all host cores pulling 1024-ray chunks,
+      1 warm-up + median of 5;
+    - with the reference build also: its own 10-thread fixed-band split
+      (cpu_tracer_renderer/CPUTracerRenderer.cpp:77-87) and the as-shipped -ffast-math build
+      (CMakeLists.txt:42) on all cores, median of 3 each, with the fast-math hits compared."""
     from oracle import oracle as O
     threads = os.cpu_count() or 1
     use_ref = O.have_ref()
@@ -98,15 +103,21 @@ def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
     t0 = time.perf_counter()
     cast(probe)
     rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
-    n = int(min(len(rays), max(len(probe), rate * budget_s / 2)))
+    # the parity runs take about half the budget; the band split and fast-math runs the rest
+    n = int(min(len(rays), max(len(probe), rate * budget_s / 12)))
     idx = np.linspace(0, len(rays) - 1, n).astype(np.int64)
     sample = np.ascontiguousarray(rays[idx])
-    times = []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        hits = cast(sample)
-        times.append(time.perf_counter() - t0)
-    t = min(times)
+
+    def timed(f, reps):
+        times, out = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = f(sample)
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), out
+
+    cast(sample)   # warm-up
+    t, hits = timed(cast, 5)
     same = hits.tobytes() == np.ascontiguousarray(gpu_hits[idx]).tobytes()
     cpu_model = ""
     try:
@@ -116,11 +127,30 @@ def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 12.0) -> dict:
                 break
     except OSError:
         pass
-    return {"value": round(n / t / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-            "kind": "reference" if use_ref else "port",
-            "sample": f"{n} of the {len(rays)} bounce rays (evenly strided), best of 2 runs, "
-                      f"{'castRay<144> compiled from the reference sources' if use_ref else 'oracle castRay restatement'}"
-                      f" on {threads} threads ({cpu_model}); hits identical to GPU: {same}"}
+    out = {"value": round(n / t / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+           "kind": "reference" if use_ref else "port",
+           "sample": f"{n} of the {len(rays)} bounce rays (evenly strided), 1 warm-up + median of "
+                     f"5 runs, "
+                     f"{'castRay<144> compiled from the reference sources' if use_ref else 'oracle castRay restatement'}"
+                     f" (-O3 -ffp-contract=off) on {threads} threads pulling 1024-ray chunks "
+                     f"({cpu_model}); hits identical to GPU: {same}"}
+    if use_ref:
+        tb, hb = timed(lambda r: O.ref_cast_bands(nodes, tris, r, 10), 3)
+        out["reference_10_thread_bands"] = {
+            "value": round(n / tb / 1e6, 3), "cores": 10,
+            "note": "the reference CPU renderer's split: 10 threads over fixed contiguous bands, "
+                    f"median of 3; hits identical to GPU: "
+                    f"{hb.tobytes() == np.ascontiguousarray(gpu_hits[idx]).tobytes()}"}
+        if O.have_ref(fastmath=True):
+            O.ref_cast(nodes, tris, sample[: min(n, 20000)], threads, fastmath=True)
+            tf, hf = timed(lambda r: O.ref_cast(nodes, tris, r, threads, fastmath=True), 3)
+            differ = int(np.count_nonzero(
+                (hf["tri"] != hits["tri"]) | (hf["t"].view(np.uint32) != hits["t"].view(np.uint32))))
+            out["reference_fastmath"] = {
+                "value": round(n / tf / 1e6, 3), "cores": threads,
+                "note": f"as-shipped -O3 -ffast-math build, median of 3; {differ} of {n} hits "
+                        "differ from the parity build in triangle or t"}
+    return out
 
 
 def set_shape(ctx, p: dict) -> None:

@@ -93,6 +93,7 @@ def rlib(fastmath: bool = False) -> C.CDLL:
         L.phxref_get.argtypes = [_P, _P, _P, _P]
         L.phxref_free.argtypes = [_P]
         L.phxref_cast.argtypes = [_P, _P, _P, _U32, _P, _U32]
+        L.phxref_cast_bands.argtypes = [_P, _P, _P, _U32, _P, _U32]
         L.phxref_cast_debug.argtypes = [_P, _P, _P, _U32, _P, _P]
         L.phxref_primary_rays.argtypes = [_P, _P, _P, C.c_float, _U32, _U32, C.c_float,
                                           C.c_float, _P, _P]
@@ -293,6 +294,15 @@ def ref_cast(nodes, tris, rays, threads: int = 0, fastmath: bool = False):
     from phantasyenginetestbed_amd.phx import HIT
     hits = np.empty(len(rays), HIT)
     rlib(fastmath).phxref_cast(_p(nodes), _p(tris), _p(rays), len(rays), _p(hits), threads)
+    return hits
+
+
+def ref_cast_bands(nodes, tris, rays, threads: int = 10, fastmath: bool = False):
+    """castRay over `threads` fixed contiguous bands, the reference CPU renderer's split
+    (cpu_tracer_renderer/CPUTracerRenderer.cpp:77-87)."""
+    from phantasyenginetestbed_amd.phx import HIT
+    hits = np.empty(len(rays), HIT)
+    rlib(fastmath).phxref_cast_bands(_p(nodes), _p(tris), _p(rays), len(rays), _p(hits), threads)
     return hits
 
 

@@ -105,6 +105,31 @@ void phxref_cast(const void* nodes, const void* tris, const RefRay* rays, uint32
 	for (auto& t : th) t.join();
 }
 
+// castRay<144> per ray split the way the reference's CPU renderer splits a frame
+// (cpu_tracer_renderer/CPUTracerRenderer.cpp:77-87): `numThreads` threads, thread i takes the
+// contiguous band [i * (n / numThreads), (i + 1) * (n / numThreads)), the last one also the
+// remainder; no work stealing.
+void phxref_cast_bands(const void* nodes, const void* tris, const RefRay* rays, uint32_t n,
+                       RefHit* hits, uint32_t numThreads)
+{
+	const BVHNode* nd = static_cast<const BVHNode*>(nodes);
+	const TriangleVertices* tv = static_cast<const TriangleVertices*>(tris);
+	if (numThreads == 0) numThreads = 1;
+	const uint32_t per = n / numThreads;
+	auto band = [&](uint32_t t) {
+		const uint32_t b = t * per, e = t + 1 == numThreads ? n : b + per;
+		for (uint32_t i = b; i < e; i++) {
+			const RefRay& r = rays[i];
+			Ray ray(vec3(r.o[0], r.o[1], r.o[2]), vec3(r.d[0], r.d[1], r.d[2]));
+			RayCastResult res = castRay(nd, tv, ray, r.tMin, r.tMax);
+			hits[i] = {res.triangleIndex, res.t, res.u, res.v};
+		}
+	};
+	std::vector<std::thread> th;
+	for (uint32_t t = 0; t < numThreads; t++) th.emplace_back(band, t);
+	for (auto& t : th) t.join();
+}
+
 // castDebugRay<144>: hits plus per-ray counters (nodesVisited, trianglesIntersected,
 // triangleIntersectionTests, aabbIntersectionTests, aabbIntersections).
 void phxref_cast_debug(const void* nodes, const void* tris, const RefRay* rays, uint32_t n,

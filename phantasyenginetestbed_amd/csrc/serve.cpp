@@ -2,7 +2,8 @@
 //
 // Requests (ray batches against resident scenes) are admitted in arrival order while their
 // buffers fit the HBM budget (a first-fit arena over one device allocation, so admission never
-// calls hipMalloc), and every admitted request is coalesced into the next fused persistent launch.
+// calls hipMalloc), and admitted requests are coalesced into the next fused persistent launch, up
+// to kLaunchRays rays per launch (at least one request).
 // Launches run on `slots` streams, each with its own work counter and stack spill area, so a
 // launch formed from requests that arrived while another one runs starts on the next stream and
 // fills the running launch's tail. A dispatcher thread forms launches and retires them (outputs
@@ -31,6 +32,13 @@ using Clock = std::chrono::steady_clock;
 double msSince(Clock::time_point t0) { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 
 constexpr size_t kAlign = 256;
+
+// Rays per fused launch before further admitted requests wait for the next launch. A backlog
+// coalesced into one or two launches runs its host-to-device copies, traversal and copies back
+// one after the other (the 143 M-ray saturated request mix: ~340 ms); in launches of a few million
+// rays the slot streams overlap one launch's copies with another's traversal (~175 ms). Each
+// launch of this size still fills the device many times over (4 M rays on ~1 k blocks).
+constexpr uint64_t kLaunchRays = 4u << 20;
 size_t alignUp(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 
 // First-fit allocator over [0, size) with coalescing of neighbouring free ranges.
@@ -182,6 +190,7 @@ hipError_t launchSlot(PhxServer* s, Slot& slot)
 	while (!s->admitted.empty() && slot.reqs.size() < kMaxFusedBatches) {
 		Request* r = s->admitted.front();
 		if (total + r->req.numRays > 0x7FFFFFFFull) break;
+		if (!slot.reqs.empty() && total + r->req.numRays > kLaunchRays) break;
 		total += r->req.numRays;
 		slot.reqs.push_back(r);
 		s->admitted.pop_front();
