@@ -119,16 +119,21 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	const SceneView sv = view(scene);
 	const TraceLaunch closest = traceLaunch(ctx, 0, false);
 	// Shadow rays are coherent -- a pixel tile's rays towards one light, tiles in order -- so they
-	// are traced as packets, each wave expanding the union of 64 rays' nodes.
+	// are traced as packets, each wave expanding the union of 64 rays' nodes. So are camera rays in
+	// 8x8-tile order (a wave's 64 rays are one tile): 0.50 ms instead of 0.77 ms for a 1080p
+	// frame's closest hits; in row-major order a wave's rays span a 64-pixel row segment and
+	// packets gain nothing, so those go to the persistent kernel.
 	TraceLaunch shadows = traceLaunch(ctx, 0, true);
 	shadows.kernel = TraceKernel::Packet;
+	TraceLaunch primary = closest;
+	if (order == PHX_RAYS_TILED_8X8) primary.kernel = TraceKernel::Packet;
 	const char* what = "phx_render_frame";
 
 	// Primary hits -> hit records -> contributions and shadow rays of every light.
 	PHX_HIP(launchGenPrimaryRays(*cam, r->w, r->h, 0.0f, 0.0f, 1e-4f, FLT_MAX, order, r->prim, s,
 	                             r->params),
 	        what);
-	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, closest, nullptr, s), what);
+	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, primary, nullptr, s), what);
 	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->materials,
 	                        scene->numMaterials, scene->tex, r->gbuf, s),
 	        what);
