@@ -202,3 +202,41 @@ def test_admission_budget(ctx, scenes):
         for rb, ob in bufs:
             rb.free()
             ob.free()
+
+
+def test_backlog_above_the_launch_cap(ctx, scenes):
+    """A burst larger than one launch's ray cap (4M rays) is split over several launches on the
+    slot streams; every result is still the one the request gets alone."""
+    rng = np.random.default_rng(5)
+    specs, total, i = [], 0, 0
+    while total < 5 << 20:   # request sizes are bounded by the pools: add until the cap is passed
+        si = i % len(scenes)
+        kind = ["primary", "diffuse", "shadow"][i % 3]
+        pool = scenes[si][2][kind]
+        m = min(len(pool), int(rng.integers(48000, 64001)))
+        specs.append((si, kind, np.ascontiguousarray(pool[:m])))
+        total += m
+        i += 1
+    assert len(specs) <= 1000
+    say(f"[gpu-serve] backlog of {total} rays in {len(specs)} requests")
+    bufs = host_buffers(specs)
+    want = alone(ctx, scenes, specs)
+    srv = phx.Server(ctx, 1 << 31, slots=3)
+    try:
+        for _, s, _ in scenes:
+            srv.add_scene(s)
+        reqs = [phx.Server.request(scenes[si][1], rb.ptr, ob.ptr, len(rays),
+                                   phx.BATCH_ANYHIT if kind == "shadow" else phx.BATCH_CLOSEST)
+                for (si, kind, rays), (rb, ob) in zip(specs, bufs)]
+        lat, mk = srv.replay(reqs, np.zeros(len(reqs)))
+        assert (lat > 0).all() and mk > 0
+        for i, ((si, kind, rays), (rb, ob)) in enumerate(zip(specs, bufs)):
+            assert ob.array[:len(rays)].tobytes() == want[i].tobytes(), f"request {i}"
+        st = srv.stats()
+        assert st["requestsDone"] == len(reqs)
+        assert st["launches"] >= -(-total // (4 << 20))
+    finally:
+        srv.destroy()
+        for rb, ob in bufs:
+            rb.free()
+            ob.free()
