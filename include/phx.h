@@ -502,9 +502,13 @@ typedef struct PhxServerStats {
 #define PHX_SERVE_PLAIN 0x1u /* one request per launch, one launch in flight (per-batch baseline) */
 
 /* A server on ctx's device with an HBM budget (resident scenes + request buffers) and `slots`
- * (1..4) launch streams. Requests are admitted in arrival order while their buffers fit, and all
- * admitted requests are coalesced into the next fused persistent launch on an idle slot; a launch
- * formed while another runs fills that launch's tail. A dispatcher thread owns the device work. */
+ * (1..4) launch streams. Requests are admitted in arrival order while their buffers fit, and
+ * admitted requests are coalesced, in order, into the next fused persistent launch on an idle
+ * slot, up to 4M rays per launch (a larger request gets a launch of its own), so a backlog spreads
+ * over the slots and one launch's copies overlap another's traversal; a launch formed while
+ * another runs fills that launch's tail. A dispatcher thread owns the device work. The server
+ * launches with the default stack split and rays per fetch and no treelet or helper waves; it
+ * takes the block size and refill threshold from the context. */
 phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t slots,
                              uint32_t flags, PhxServer** outServer);
 /* Waits for every submitted request, then frees the server. */
