@@ -557,10 +557,11 @@ void phx_renderer_release(PhxRenderer* renderer);
  * any hit, shadow logic; one bounce ray per 2x2 pixel block (createSecondaryRaysKernel), closest
  * hit, shading x path throughput with shadow rays (materialKernel), any hit, shadow logic added
  * at half resolution; then, with PHX_FRAME_TAA, the TAA resolve (taa.frag:25-71 for a static
- * camera, FP32 history, 3x3 clamp). Shadow rays (coherent: a pixel tile's rays towards one
- * light) and, when width and height are multiples of 8 (camera rays in 8x8-tile order), camera
- * rays are traced as packets (PHX_KERNEL_PACKET); bounce rays, and camera rays in row-major
- * order, by the persistent kernel. `rgba` (width*height*4 floats, row-major; host or device with
+ * camera, FP32 history, 3x3 clamp). The primary hits' shadow rays (coherent: a pixel tile's rays
+ * towards one light) and, when width and height are multiples of 8 (camera rays in 8x8-tile
+ * order), camera rays are traced as packets (PHX_KERNEL_PACKET); bounce rays, and camera rays in
+ * row-major order, by the persistent kernel; the bounce hits' shadow rays (scattered origins) one
+ * ray per lane (PHX_KERNEL_ONE_RAY_PER_LANE). `rgba` (width*height*4 floats, row-major; host or device with
  * PHX_MEM_DEVICE; may be NULL) receives the output. Random numbers: PCG32 streams keyed by
  * (seed, frameIndex, pass, record). The frame's kernels read the per-frame values (jitter, frame
  * index, seed, TAA reset and history buffer) from a small device block written before the frame,

@@ -125,6 +125,11 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	// packets gain nothing, so those go to the persistent kernel.
 	TraceLaunch shadows = traceLaunch(ctx, 0, true);
 	shadows.kernel = TraceKernel::Packet;
+	// The bounce hits' shadow rays start at scattered points (neighbouring half-resolution pixels'
+	// bounce rays go anywhere), so a packet's union of nodes is large; one ray per lane traces
+	// them fastest (798,504 such rays: 0.18 ms, against 0.25 persistent and 0.35 as packets).
+	TraceLaunch bounceShadows = shadows;
+	bounceShadows.kernel = TraceKernel::Simple;
 	TraceLaunch primary = closest;
 	if (order == PHX_RAYS_TILED_8X8) primary.kernel = TraceKernel::Packet;
 	const char* what = "phx_render_frame";
@@ -166,7 +171,7 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
 		                    r->m, r->params, 3u, r->contrib, r->shadow, s),
 		        what);
-		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, shadows, s), what);
+		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, bounceShadows, s), what);
 		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
 		        what);
 	}
