@@ -1145,8 +1145,8 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 		              : launchClosest(sv, rays, numRays, static_cast<PhxRayHit*>(out), L, nullptr,
 		                              ctx->stream);
 	};
-	st = tune(ctx, cands, 7, cast, best);
-	if (st == PHX_OK) st = refine(ctx, nullptr, 9, cast, best);
+	st = tune(ctx, cands, 13, cast, best);
+	if (st == PHX_OK) st = refine(ctx, nullptr, 15, cast, best);
 	hipStreamSynchronize(ctx->stream);
 	hipFree(out);
 	if (st != PHX_OK) return st;
