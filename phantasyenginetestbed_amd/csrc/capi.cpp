@@ -1107,7 +1107,8 @@ phx_status refine(PhxContext* ctx, const PhxPlan* plan, int reps, Launch launch,
 				c.params.stackLdsEntries = v;
 				const uint32_t bpc = blocksPerCU(plan, c.params.blockThreads, c.params.topNodes,
 				                                 c.params.helperWaves, v);
-				const uint32_t resident = std::min(ctx->numCUs * bpc, ctx->spillLanes / c.params.blockThreads);
+				const uint32_t resident =
+					std::min(ctx->numCUs * bpc, ctx->spillLanes / c.params.blockThreads);
 				if (resident == 0) continue;
 				c.params.gridBlocksClosest = std::min(c.params.gridBlocksClosest, resident);
 				c.params.gridBlocksAnyhit = std::min(c.params.gridBlocksAnyhit, resident);
