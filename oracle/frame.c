@@ -8,13 +8,18 @@
  * as products, cos/sin(2*pi*r) as quadrant reduction plus Taylor polynomials (turn_sincos), and
  * min/max/clamp as explicit compares with fmaxf's NaN rule.
  *
- * Parity pinning: the closed forms below (tests/test_frame_oracle.py) -- shading of a light along
+ * Parity pinning: the closed forms of tests/test_frame_oracle.py -- shading of a light along
  * the normal against the textbook Cook-Torrance value computed in double, turn_sincos against
  * math.cos/sin, the cosine-weighted sample's mean and first moments, the TAA running mean of a
  * static image, the shadow logic's sum over lights -- and a second evaluation of ggx / Fresnel /
- * falloff in numpy. The reference's own random stream (curand XORWOW) is not reproduced: the
- * frame draws from PCG32, so sampled directions are "parity unpinned" against the reference and
- * pinned only by those distribution checks.
+ * falloff in numpy. The bounce stage (secondary rays with the 1-of-4 pixel pick, the Lambert/GGX
+ * roulette and the glass ray, throughput, shading of the bounce hit, the half-resolution add) is
+ * pinned by tests/test_frame_bounce_oracle.py: plane scenes whose colour has a closed form per
+ * pixel, with the random draws recomputed from PCG32's published definition (itself checked
+ * against the published pcg32 demo output); that file also plants one-constant mistakes in a copy
+ * of this file and shows each is caught. The reference's own random stream (curand XORWOW) is not
+ * reproduced: the frame draws from PCG32, so sampled directions are "parity unpinned" against the
+ * reference's stream and pinned to its formulas only.
  */
 #include "oracle.h"
 
@@ -239,6 +244,21 @@ void oracle_sample_lambert(float r1, float r2, const float n[3], float out[3])
 	out[0] = d.x;
 	out[1] = d.y;
 	out[2] = d.z;
+}
+
+void oracle_sample_ggx(float r1, float r2, float a, const float refl[3], const float n[3],
+                       float out[3])
+{
+	V3 d = sample_ggx(r1, r2, a, v3(refl[0], refl[1], refl[2]), v3(n[0], n[1], n[2]));
+	out[0] = d.x;
+	out[1] = d.y;
+	out[2] = d.z;
+}
+
+void oracle_pcg32(uint64_t init, uint64_t seq, uint32_t n, uint32_t* out)
+{
+	Pcg g = pcg_seeded(init, seq);
+	for (uint32_t i = 0; i < n; i++) out[i] = pcg_next(&g);
 }
 
 /* Pixel -> ray/record index: 8x8 tiles (getRayIdx, CudaDeviceHelpers.cuh:49-62) when both sizes

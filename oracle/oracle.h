@@ -16,7 +16,10 @@
  *   oracle_brute_closest  - closed-form rays against hand-built triangles.
  *   oracle_primary_rays   - the reference's CPU renderer formula compiled from its sources
  *                           (golden fixture) and closed form at the image centre/corners.
- *   frame.c functions     - closed forms and distribution checks listed in frame.c's header.
+ *   frame.c functions     - closed forms and distribution checks listed in frame.c's header;
+ *                           the bounce stage by plane scenes with closed-form colours and
+ *                           planted-mistake builds (tests/test_frame_bounce_oracle.py).
+ *   oracle_pcg32          - the published pcg32 demo output (seed 42, sequence 54).
  * The structs are the byte layouts of include/phx.h, restated so the oracle stays standalone.
  */
 #ifndef PHX_ORACLE_H
@@ -134,6 +137,11 @@ void oracle_shade(const float n[3], const float v[3], const float albedo[3], flo
                   float metal, const float l[3], float dist, const float strength[3], float range,
                   float out[3]);
 void oracle_sample_lambert(float r1, float r2, const float n[3], float out[3]);
+/* sampleGgx + clampToHemisphere (MaterialKernel.cu:220-236) around `refl` with a = roughness^2. */
+void oracle_sample_ggx(float r1, float r2, float a, const float refl[3], const float n[3],
+                       float out[3]);
+/* The first n outputs of PCG32 seeded as pcg32_srandom_r(init, seq) (O'Neill 2014). */
+void oracle_pcg32(uint64_t init, uint64_t seq, uint32_t n, uint32_t* out);
 void oracle_primary_rays_jittered(const float cam[12], uint32_t w, uint32_t h, float jx, float jy,
                                   float tMin, float tMax, OrRay* rays);
 void oracle_bounce_rays(const OrTriData* datas, const OrRay* rays, const OrHit* hits, uint32_t n,

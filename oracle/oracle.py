@@ -6,6 +6,7 @@ Arrays use the numpy dtypes of phantasyenginetestbed_amd.phx (same byte layouts)
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import subprocess
 from pathlib import Path
@@ -46,36 +47,54 @@ def olib() -> C.CDLL:
     if _olib is None:
         if not ORACLE_LIB.exists():
             build(reference=False)
-        L = C.CDLL(str(ORACLE_LIB))
-        L.oracle_cast_closest.argtypes = [_P, _P, _P, _U32, _P, _U32]
-        L.oracle_cast_debug.argtypes = [_P, _P, _P, _U32, _P, C.POINTER(Counters), _P, _U32]
-        L.oracle_cast_anyhit.argtypes = [_P, _P, _P, _U32, _P, _U32]
-        L.oracle_brute_closest.argtypes = [_P, _U32, _P, _U32, _P]
-        L.oracle_primary_rays.argtypes = [_P, _U32, _U32, C.c_float, C.c_float, _U32, _P]
-        L.oracle_triangle.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
-        L.oracle_triangle.restype = C.c_int
-        L.oracle_aabb.argtypes = [_P, _P, _P, _P, _P, _P]
-        L.oracle_aabb.restype = C.c_int
-        F = C.c_float
-        L.oracle_turn_sincos.argtypes = [F, C.POINTER(F), C.POINTER(F)]
-        L.oracle_ggx.argtypes = [F, F]
-        L.oracle_ggx.restype = F
-        L.oracle_falloff.argtypes = [F, F]
-        L.oracle_falloff.restype = F
-        L.oracle_shade.argtypes = [_P, _P, _P, F, F, _P, F, _P, F, _P]
-        L.oracle_sample_lambert.argtypes = [F, F, _P, _P]
-        L.oracle_primary_rays_jittered.argtypes = [_P, _U32, _U32, F, F, F, F, _P]
-        L.oracle_bounce_rays.argtypes = [_P, _P, _P, _U32, C.c_uint64, _U32, _P]
-        L.oracle_frame.argtypes = [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, _P, _U32, _U32,
-                                   F, F, C.c_uint64, _U32, _P, _P, _P, _U32]
-        L.oracle_interpret.argtypes = [_P, _P, _U32, _P, _P, _U32, _P, _U32, _P]
-        L.oracle_texel_coord.argtypes = [F, _U32]
-        L.oracle_texel_coord.restype = _U32
-        L.oracle_linear_table.argtypes = [_P]
-        L.oracle_taa.argtypes = [_P, _P, _P, _U32, _U32, C.c_int, C.c_int, _P, _P]
-        L.oracle_halton23.argtypes = [_U32, C.POINTER(F), C.POINTER(F)]
-        _olib = L
+        _olib = _configure(C.CDLL(str(ORACLE_LIB)))
     return _olib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Routes every oracle call of this module through another build of oracle.c + frame.c (a
+    copy with a deliberate mistake planted, in the tests that show a check catches it)."""
+    global _olib
+    saved = olib()
+    _olib = _configure(C.CDLL(str(path)))
+    try:
+        yield _olib
+    finally:
+        _olib = saved
+
+
+def _configure(L: C.CDLL) -> C.CDLL:
+    L.oracle_cast_closest.argtypes = [_P, _P, _P, _U32, _P, _U32]
+    L.oracle_cast_debug.argtypes = [_P, _P, _P, _U32, _P, C.POINTER(Counters), _P, _U32]
+    L.oracle_cast_anyhit.argtypes = [_P, _P, _P, _U32, _P, _U32]
+    L.oracle_brute_closest.argtypes = [_P, _U32, _P, _U32, _P]
+    L.oracle_primary_rays.argtypes = [_P, _U32, _U32, C.c_float, C.c_float, _U32, _P]
+    L.oracle_triangle.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
+    L.oracle_triangle.restype = C.c_int
+    L.oracle_aabb.argtypes = [_P, _P, _P, _P, _P, _P]
+    L.oracle_aabb.restype = C.c_int
+    F = C.c_float
+    L.oracle_turn_sincos.argtypes = [F, C.POINTER(F), C.POINTER(F)]
+    L.oracle_ggx.argtypes = [F, F]
+    L.oracle_ggx.restype = F
+    L.oracle_falloff.argtypes = [F, F]
+    L.oracle_falloff.restype = F
+    L.oracle_shade.argtypes = [_P, _P, _P, F, F, _P, F, _P, F, _P]
+    L.oracle_sample_lambert.argtypes = [F, F, _P, _P]
+    L.oracle_primary_rays_jittered.argtypes = [_P, _U32, _U32, F, F, F, F, _P]
+    L.oracle_bounce_rays.argtypes = [_P, _P, _P, _U32, C.c_uint64, _U32, _P]
+    L.oracle_frame.argtypes = [_P, _P, _P, _P, _U32, _P, _U32, _P, _U32, _P, _U32, _U32,
+                               F, F, C.c_uint64, _U32, _P, _P, _P, _U32]
+    L.oracle_interpret.argtypes = [_P, _P, _U32, _P, _P, _U32, _P, _U32, _P]
+    L.oracle_texel_coord.argtypes = [F, _U32]
+    L.oracle_texel_coord.restype = _U32
+    L.oracle_linear_table.argtypes = [_P]
+    L.oracle_taa.argtypes = [_P, _P, _P, _U32, _U32, C.c_int, C.c_int, _P, _P]
+    L.oracle_halton23.argtypes = [_U32, C.POINTER(F), C.POINTER(F)]
+    L.oracle_sample_ggx.argtypes = [F, F, F, _P, _P, _P]
+    L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, _U32, _P]
+    return L
 
 
 def have_ref(fastmath: bool = False) -> bool:
@@ -189,6 +208,20 @@ def sample_lambert(r1: float, r2: float, n):
     nn = np.ascontiguousarray(n, np.float32)
     out = np.zeros(3, np.float32)
     olib().oracle_sample_lambert(r1, r2, _p(nn), _p(out))
+    return out
+
+
+def sample_ggx(r1: float, r2: float, a: float, refl, n):
+    rr = np.ascontiguousarray(refl, np.float32)
+    nn = np.ascontiguousarray(n, np.float32)
+    out = np.zeros(3, np.float32)
+    olib().oracle_sample_ggx(r1, r2, a, _p(rr), _p(nn), _p(out))
+    return out
+
+
+def pcg32(init: int, seq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    olib().oracle_pcg32(init, seq, n, _p(out))
     return out
 
 

@@ -234,3 +234,29 @@ def test_frame_graph_replays_match_direct_frames(ctx, small):
     finally:
         direct.release()
         graph.release()
+
+
+@pytest.mark.parametrize("kind,w,h", [("lambert", 32, 24), ("lambert", 31, 21), ("ggx", 40, 32),
+                                      ("glass", 32, 24)])
+def test_plane_scene_frames_bit_identical(ctx, kind, w, h):
+    """The closed-form plane scenes of test_frame_bounce_oracle (diffuse bounce, GGX roulette,
+    glass rays) render bit-identically on the GPU: the oracle those tests pin is the GPU's."""
+    from test_frame_bounce_oracle import CAM_POS, make_scene as plane_scene
+    sc = plane_scene(kind)
+    say(f"[gpu-frame] plane scene {kind} {w}x{h}")
+    scene = ctx.upload_scene(sc.bvh.nodes, sc.bvh.tris, sc.bvh.datas, sc.materials, sc.lights)
+    cam = phx.camera_look_at(CAM_POS, (0.05, 0.0, 0.25), (0.0, 1.0, 0.0), math.radians(60), w, h)
+    r = ctx.renderer(w, h)
+    try:
+        out = r.render(scene, cam, 3, 0xB0C4, 0)
+        gbuf = r.download(phx.BUFFER_GBUFFER, phx.HITINFO)
+        bounce = r.download(phx.BUFFER_BOUNCE_RAYS, phx.RAY)
+    finally:
+        r.release()
+        scene.release()
+    ref_color, ref_gbuf, ref_bounce = O.frame(sc.bvh.nodes, sc.bvh.tris, sc.bvh.datas,
+                                              sc.materials, sc.lights, cam.as_array(), w, h, 0.0,
+                                              0.0, 0xB0C4, 3)
+    bits_equal(gbuf.view(np.float32), ref_gbuf.view(np.float32), "hit records")
+    bits_equal(bounce.view(np.float32), ref_bounce.view(np.float32), "bounce rays")
+    bits_equal(out, ref_color, "frame colour")
