@@ -137,9 +137,10 @@ typedef struct PhxCameraDef {
 #define PHX_MEM_DEVICE 0x1u /* ray/hit/image pointers are device pointers                    */
 #define PHX_ASYNC 0x2u      /* return after enqueueing on the context stream (device ptrs only)*/
 
-/* Traversal algorithm of phx_cast_closest (the any-hit result does not depend on it). */
-#define PHX_TRAVERSAL_REFERENCE_ORDER 0x00u /* the CPU castRay visiting order: bit-identical hits */
-#define PHX_TRAVERSAL_NEAR_FIRST 0x10u      /* nearer child first; ties broken by castRay order  */
+/* Traversal order of phx_cast_closest and phx_traversal_stats: always the CPU castRay visiting
+ * order (left child first, BVHTraversal.hpp:37-128), so hits are bit-identical to it. The bits of
+ * PHX_TRAVERSAL_MASK are reserved and must be zero. */
+#define PHX_TRAVERSAL_REFERENCE_ORDER 0x00u
 #define PHX_TRAVERSAL_MASK 0xF0u
 
 /* Launch model of phx_cast_closest / phx_cast_anyhit / phx_traversal_stats. Default: persistent
@@ -274,6 +275,9 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
                             uint32_t numTriangles, const PhxMaterial* materials,
                             uint32_t numMaterials, const PhxSphereLight* lights,
                             uint32_t numLights, PhxScene** outScene);
+/* The handle is invalid after this call. A plan (phx_plan_create) or server (phx_server_add_scene)
+ * that uses the scene keeps its device arrays alive until the plan is released or the server
+ * destroyed, so their launches stay valid. */
 void phx_scene_release(PhxScene* scene);
 
 /* Device bytes held by a scene. */
@@ -438,8 +442,8 @@ phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32
 /* One cosine-weighted diffuse bounce per ray (createSecondaryRaysKernel's Lambertian branch,
  * kernels/MaterialKernel.cu:239-312): origin = o + t*d, direction sampleLambertian around the
  * interpolated TriangleData normal turned to face the incoming ray, minDist 0.001, maxDist
- * FLT_MAX. Rays that missed (triangleIndex 0xFFFFFFFF) get the reference's dummy ray
- * (MaterialKernel.cu:41-47). Random numbers: PCG32 stream `i` seeded from (seed, frame). The
+ * FLT_MAX. Rays that missed (triangleIndex 0xFFFFFFFF), and hits whose triangle index is not a
+ * triangle of `scene`, get the reference's dummy ray (MaterialKernel.cu:41-47). Random numbers: PCG32 stream `i` seeded from (seed, frame). The
  * scene must have TriangleData. */
 phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                                const PhxRayHit* hits, uint32_t numRays, uint64_t seed,
@@ -456,8 +460,8 @@ typedef struct PhxRayHitInfo {
 /* Replaces launchInterpretRayHitKernel (InterpretRayHitKernel.hpp:76-78, .cu:58-136). A hit
  * gets position o + d*t, normal normalize((n0 + (n1-n0)u) + (n2-n0)v), uv interpolated the same
  * way, and its material: constant values, or texture samples as phx_scene_set_textures describes;
- * a material index outside the table reads albedo 1, alpha 1, roughness 1, metallic 0. A miss
- * gets zeros and alpha -1. The scene must have TriangleData. `flags` = PHX_MEM_DEVICE |
+ * a material index outside the table reads albedo 1, alpha 1, roughness 1, metallic 0. A miss,
+ * or a hit whose triangle index is not a triangle of `scene`, gets zeros and alpha -1. The scene must have TriangleData. `flags` = PHX_MEM_DEVICE |
  * PHX_ASYNC. */
 phx_status phx_interpret_hits(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                               const PhxRayHit* hits, uint32_t numRays, PhxRayHitInfo* out,
@@ -496,7 +500,7 @@ typedef struct PhxServerStats {
 	uint64_t requestBudget; /* budget minus resident scene bytes */
 	uint64_t sceneBytes;
 	uint32_t queued;        /* submitted requests not yet launched */
-	uint32_t pad;
+	uint32_t cancelled;     /* requests cancelled before their launch */
 } PhxServerStats;
 
 #define PHX_SERVE_PLAIN 0x1u /* one request per launch, one launch in flight (per-batch baseline) */
@@ -517,12 +521,24 @@ void phx_server_destroy(PhxServer* server);
  * (PHX_ERR_BUDGET if they do not fit); call before submitting requests. */
 phx_status phx_server_add_scene(PhxServer* server, const PhxScene* scene);
 /* Queues a request; never blocks. Fails with PHX_ERR_BUDGET if the request alone exceeds the
- * request budget, PHX_ERR_INVALID_HANDLE if its scene is not resident. */
+ * request budget, PHX_ERR_INVALID_HANDLE if its scene is not resident or was released, and
+ * PHX_ERR_INVALID_ARGUMENT for more than 2^31 - 1 rays. */
 phx_status phx_server_submit(PhxServer* server, const PhxRequest* request, uint64_t* ticket);
-/* done = 1 once the request's outputs are in place; latency = submit to completion (ms). */
+/* done = 1 once the request's outputs are in place; latency = submit to completion (ms). A
+ * cancelled request reports done = 1 and returns PHX_ERR_CANCELLED. */
 phx_status phx_server_poll(PhxServer* server, uint64_t ticket, int* done, double* latencyMs);
-/* Blocks until the request completes, then forgets the ticket. */
+/* Blocks until the request completes, then forgets the ticket. Returns PHX_ERR_CANCELLED (and
+ * forgets the ticket) for a cancelled request. */
 phx_status phx_server_wait(PhxServer* server, uint64_t ticket, double* latencyMs);
+/* Cancels a request that has not been launched yet: it leaves the queue, its arena space is
+ * returned, its outputs are never written, and poll / wait report PHX_ERR_CANCELLED.
+ * *cancelled (may be NULL) = 1 if the request is cancelled, 0 if it was already on the device or
+ * complete (it then completes normally). No counterpart in the reference. */
+phx_status phx_server_cancel(PhxServer* server, uint64_t ticket, int* cancelled);
+/* pause != 0: the dispatcher launches nothing new until called again with 0 (requests are still
+ * admitted while their buffers fit); launches already on the device complete. For draining, and
+ * for holding requests before their launch. */
+phx_status phx_server_pause(PhxServer* server, int pause);
 phx_status phx_server_stats(PhxServer* server, PhxServerStats* stats);
 /* Load driver: submits reqs[i] at arrivalMs[i] after the call starts (open loop), waits for all,
  * and returns each request's latency and the time from the call's start to the last completion. */

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -88,6 +89,12 @@ void freeScene(PhxScene* s)
 	delete s;
 }
 
+std::mutex& sceneHoldMutex()
+{
+	static std::mutex mu;
+	return mu;
+}
+
 constexpr uint32_t kMaxSceneElements = 1u << 28;
 
 // Runs `launch(raysDev, outDev)` with host/device handling of the ray and output arrays.
@@ -168,6 +175,34 @@ std::vector<PhxTriangleVertices> pretransform(const PhxTriangleVertices* t, uint
 }
 
 }  // namespace
+
+namespace phx {
+
+uint64_t nextSceneStamp()
+{
+	static std::atomic<uint64_t> counter{0};
+	return ++counter;
+}
+
+void holdScene(const PhxScene* s)
+{
+	std::lock_guard<std::mutex> g(sceneHoldMutex());
+	const_cast<PhxScene*>(s)->holders++;
+}
+
+void dropScene(const PhxScene* cs)
+{
+	PhxScene* s = const_cast<PhxScene*>(cs);
+	{
+		std::lock_guard<std::mutex> g(sceneHoldMutex());
+		if (--s->holders != 0 || !s->released) return;
+	}
+	hipSetDevice(s->ctx->device);
+	hipStreamSynchronize(s->ctx->stream);
+	freeScene(s);
+}
+
+}  // namespace phx
 
 extern "C" {
 
@@ -531,6 +566,7 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 		freeScene(s);
 		return st;
 	}
+	s->version = nextSceneStamp();
 	registry().add(s);
 	*outScene = s;
 	return ok();
@@ -539,10 +575,18 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 void phx_scene_release(PhxScene* scene)
 {
 	if (!scene || !registry().remove(scene)) return;
+	{
+		std::lock_guard<std::mutex> g(sceneHoldMutex());
+		if (scene->holders) {  // freed by the last plan or server that holds it
+			scene->released = true;
+			return;
+		}
+	}
 	hipSetDevice(scene->ctx->device);
 	hipStreamSynchronize(scene->ctx->stream);
 	freeScene(scene);
 }
+
 
 size_t phx_scene_device_bytes(const PhxScene* scene)
 {
@@ -633,7 +677,7 @@ phx_status phx_scene_set_textures(PhxContext* ctx, PhxScene* scene, const PhxTex
 	scene->tex.descs = count ? reinterpret_cast<const TexDesc*>(base + tableBytes) : nullptr;
 	scene->tex.texels = count ? base : nullptr;
 	scene->tex.count = count;
-	scene->version++;
+	scene->version = nextSceneStamp();
 	return ok();
 }
 
@@ -644,9 +688,9 @@ phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRay
 	if (st != PHX_OK) return st;
 	st = checkScene(ctx, scene);
 	if (st != PHX_OK) return st;
-	const uint32_t mode = flags & PHX_TRAVERSAL_MASK;
-	if (mode != PHX_TRAVERSAL_REFERENCE_ORDER)
-		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: unknown traversal mode 0x%x", mode);
+	if (flags & PHX_TRAVERSAL_MASK)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: reserved flag bits 0x%x",
+		            flags & PHX_TRAVERSAL_MASK);
 	if ((flags & PHX_KERNEL_ONE_RAY_PER_LANE) && (flags & PHX_KERNEL_PACKET))
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest: two launch models requested");
 	SceneView sv = view(scene);
@@ -688,8 +732,8 @@ phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const Phx
 	st = checkScene(ctx, scene);
 	if (st != PHX_OK) return st;
 	if (!counters) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: counters is NULL");
-	if ((flags & PHX_TRAVERSAL_MASK) != PHX_TRAVERSAL_REFERENCE_ORDER)
-		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: unknown traversal mode");
+	if (flags & PHX_TRAVERSAL_MASK)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: reserved flag bits");
 	if (flags & PHX_ASYNC) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: no PHX_ASYNC");
 	for (int k = 0; k < 5; k++) counters[k] = 0;
 	if (numRays == 0) return ok();
@@ -737,8 +781,8 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
 	if (async && !device)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_gen_bounce_rays: PHX_ASYNC needs PHX_MEM_DEVICE");
 	if (device) {
-		PHX_HIP(launchGenBounceRays(rays, hits, scene->datas, numRays, seed, frame, outRays,
-		                            ctx->stream),
+		PHX_HIP(launchGenBounceRays(rays, hits, scene->datas, scene->numTris, numRays, seed, frame,
+		                            outRays, ctx->stream),
 		        "phx_gen_bounce_rays");
 		if (!async) PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_gen_bounce_rays");
 		return ok();
@@ -752,7 +796,8 @@ phx_status phx_gen_bounce_rays(PhxContext* ctx, const PhxScene* scene, const Phx
 	hipError_t e = hipMemcpyAsync(dr, rays, rb, hipMemcpyHostToDevice, ctx->stream);
 	if (e == hipSuccess) e = hipMemcpyAsync(dh, hits, hb, hipMemcpyHostToDevice, ctx->stream);
 	if (e == hipSuccess)
-		e = launchGenBounceRays(dr, dh, scene->datas, numRays, seed, frame, dout, ctx->stream);
+		e = launchGenBounceRays(dr, dh, scene->datas, scene->numTris, numRays, seed, frame, dout,
+		                        ctx->stream);
 	if (e == hipSuccess) e = hipMemcpyAsync(outRays, dout, rb, hipMemcpyDeviceToHost, ctx->stream);
 	hipError_t e2 = hipStreamSynchronize(ctx->stream);
 	hipFree(tmp);
@@ -779,9 +824,9 @@ phx_status phx_interpret_hits(PhxContext* ctx, const PhxScene* scene, const PhxR
 	if (async && !device)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_interpret_hits: PHX_ASYNC needs PHX_MEM_DEVICE");
 	if (device) {
-		PHX_HIP(launchInterpret(rays, hits, numRays, scene->datas, scene->materials,
-		                        scene->numMaterials, scene->tex, reinterpret_cast<float4*>(out),
-		                        ctx->stream),
+		PHX_HIP(launchInterpret(rays, hits, numRays, scene->datas, scene->numTris,
+		                        scene->materials, scene->numMaterials, scene->tex,
+		                        reinterpret_cast<float4*>(out), ctx->stream),
 		        "phx_interpret_hits");
 		if (!async) PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_interpret_hits");
 		return ok();
@@ -796,8 +841,8 @@ phx_status phx_interpret_hits(PhxContext* ctx, const PhxScene* scene, const PhxR
 	hipError_t e = hipMemcpyAsync(dr, rays, rb, hipMemcpyHostToDevice, ctx->stream);
 	if (e == hipSuccess) e = hipMemcpyAsync(dh, hits, hb, hipMemcpyHostToDevice, ctx->stream);
 	if (e == hipSuccess)
-		e = launchInterpret(dr, dh, numRays, scene->datas, scene->materials, scene->numMaterials,
-		                    scene->tex, dout, ctx->stream);
+		e = launchInterpret(dr, dh, numRays, scene->datas, scene->numTris, scene->materials,
+		                    scene->numMaterials, scene->tex, dout, ctx->stream);
 	if (e == hipSuccess) e = hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, ctx->stream);
 	hipError_t e2 = hipStreamSynchronize(ctx->stream);
 	hipFree(tmp);
@@ -882,6 +927,11 @@ phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t nu
 		delete p;
 		return st;
 	}
+	for (uint32_t b = 0; b < numBatches; b++)
+		if (std::find(p->scenes.begin(), p->scenes.end(), batches[b].scene) == p->scenes.end()) {
+			p->scenes.push_back(batches[b].scene);
+			holdScene(batches[b].scene);
+		}
 	registry().add(p);
 	*outPlan = p;
 	return ok();
@@ -1226,6 +1276,7 @@ void phx_plan_release(PhxPlan* plan)
 	hipFree(plan->descs);
 	hipFree(plan->offsets);
 	hipFree(plan->clock);
+	for (const PhxScene* sc : plan->scenes) dropScene(sc);
 	delete plan;
 }
 

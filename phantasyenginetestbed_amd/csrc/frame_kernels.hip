@@ -82,6 +82,7 @@ __device__ __forceinline__ const uint8_t* texel_at(const TexSet& tex, uint32_t i
 __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict__ rays,
                                                         const uint4* __restrict__ hits, uint32_t n,
                                                         const PhxTriangleData* __restrict__ datas,
+                                                        uint32_t numTris,
                                                         const PhxMaterial* __restrict__ mats,
                                                         uint32_t numMats, TexSet tex,
                                                         float4* __restrict__ info)
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const uint4 h = hits[i];
-	if (h.x == 0xFFFFFFFFu) {
+	if (h.x >= numTris) {  // a miss (0xFFFFFFFF), or an index of another scene
 		info[3 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		info[3 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		info[3 * size_t(i) + 2] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
@@ -369,13 +370,13 @@ dim3 grid2d(uint32_t w, uint32_t h) { return dim3((w + 15u) / 16u, (h + 15u) / 1
 }  // namespace
 
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
-                           const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
-                           const TexSet& tex, float4* info, hipStream_t stream)
+                           const PhxTriangleData* datas, uint32_t numTris, const PhxMaterial* mats,
+                           uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream)
 {
 	if (n == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_interpret, dim3(blocks1d(n)), dim3(kBlock1D), 0, stream,
 	                   reinterpret_cast<const float4*>(rays), reinterpret_cast<const uint4*>(hits),
-	                   n, datas, mats, numMats, tex, info);
+	                   n, datas, numTris, mats, numMats, tex, info);
 	return hipGetLastError();
 }
 

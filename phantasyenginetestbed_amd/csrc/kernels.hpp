@@ -142,9 +142,10 @@ hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h,
 
 // Wavefront frame kernels (frame_kernels.hip). Hit records are 3 float4 each:
 // (position, roughness) (normal, metallic) (albedo, alpha; alpha < 0 = no hit).
+// A hit whose triangle index is not below numTris gets the miss record.
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
-                           const PhxTriangleData* datas, const PhxMaterial* mats, uint32_t numMats,
-                           const TexSet& tex, float4* info, hipStream_t stream);
+                           const PhxTriangleData* datas, uint32_t numTris, const PhxMaterial* mats,
+                           uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream);
 // The seed and frame index of the random-number streams come from fp.
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
                        const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
@@ -166,9 +167,10 @@ hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, const floa
 namespace phx {
 
 // One cosine-weighted diffuse bounce per primary hit (createSecondaryRaysKernel's Lambertian
-// branch, MaterialKernel.cu:239-312); misses get the reference's dummy ray (:41-47).
+// branch, MaterialKernel.cu:239-312); misses, and hits whose triangle index is not below
+// numTris, get the reference's dummy ray (:41-47).
 hipError_t launchGenBounceRays(const PhxRayIn* rays, const PhxRayHit* hits,
-                               const PhxTriangleData* datas, uint32_t n, uint64_t seed,
-                               uint32_t frame, PhxRayIn* out, hipStream_t stream);
+                               const PhxTriangleData* datas, uint32_t numTris, uint32_t n,
+                               uint64_t seed, uint32_t frame, PhxRayIn* out, hipStream_t stream);
 
 }  // namespace phx

@@ -73,13 +73,13 @@ __device__ __forceinline__ void normalize3(float v[3])
 __global__ __launch_bounds__(256) void k_gen_bounce(const float4* __restrict__ rays,
                                                     const uint4* __restrict__ hits,
                                                     const PhxTriangleData* __restrict__ datas,
-                                                    uint32_t n, uint64_t seed, uint32_t frame,
-                                                    float4* __restrict__ out)
+                                                    uint32_t numTris, uint32_t n, uint64_t seed,
+                                                    uint32_t frame, float4* __restrict__ out)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const uint4 h = hits[i];
-	if (h.x == 0xFFFFFFFFu) {
+	if (h.x >= numTris) {  // a miss (0xFFFFFFFF), or an index of another scene
 		out[2 * size_t(i)] = make_float4(1000000.0f, 1000000.0f, 1000000.0f, 0.001f);
 		out[2 * size_t(i) + 1] = make_float4(0.0f, 1.0f, 0.0f, 0.001f);
 		return;
@@ -116,14 +116,14 @@ __global__ __launch_bounds__(256) void k_gen_bounce(const float4* __restrict__ r
 }  // namespace
 
 hipError_t launchGenBounceRays(const PhxRayIn* rays, const PhxRayHit* hits,
-                               const PhxTriangleData* datas, uint32_t n, uint64_t seed,
-                               uint32_t frame, PhxRayIn* out, hipStream_t stream)
+                               const PhxTriangleData* datas, uint32_t numTris, uint32_t n,
+                               uint64_t seed, uint32_t frame, PhxRayIn* out, hipStream_t stream)
 {
 	if (n == 0) return hipSuccess;
 	const uint32_t grid = n / 256u + (n % 256u != 0u);
 	hipLaunchKernelGGL(k_gen_bounce, dim3(grid), dim3(256), 0, stream,
 	                   reinterpret_cast<const float4*>(rays), reinterpret_cast<const uint4*>(hits),
-	                   datas, n, seed, frame, reinterpret_cast<float4*>(out));
+	                   datas, numTris, n, seed, frame, reinterpret_cast<float4*>(out));
 	return hipGetLastError();
 }
 

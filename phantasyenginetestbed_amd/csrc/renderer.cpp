@@ -139,8 +139,8 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	                             r->params),
 	        what);
 	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, primary, nullptr, s), what);
-	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->materials,
-	                        scene->numMaterials, scene->tex, r->gbuf, s),
+	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->numTris,
+	                        scene->materials, scene->numMaterials, scene->tex, r->gbuf, s),
 	        what);
 	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, r->params, 1u,
 	                    r->contrib, r->shadow, s),
@@ -165,8 +165,9 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
 	if (r->m) {
 		PHX_HIP(hipStreamWaitEvent(s, r->join, 0), what);
-		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->materials,
-		                        scene->numMaterials, scene->tex, r->bounceInfo, s),
+		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->numTris,
+		                        scene->materials, scene->numMaterials, scene->tex, r->bounceInfo,
+		                        s),
 		        what);
 		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
 		                    r->m, r->params, 3u, r->contrib, r->shadow, s),

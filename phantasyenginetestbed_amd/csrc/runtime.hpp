@@ -94,6 +94,7 @@ struct PhxPlan {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t graphExec = nullptr;
 	std::vector<phx::BatchDesc> hostDescs;
+	std::vector<const PhxScene*> scenes;  // distinct scenes of the batches, held by the plan
 };
 
 struct PhxScene {
@@ -111,7 +112,15 @@ struct PhxScene {
 	void* texArena = nullptr;  // material textures (phx_scene_set_textures)
 	size_t texBytes = 0;
 	phx::TexSet tex;
-	uint64_t version = 0;  // incremented when the textures are replaced
+	// Process-wide unique stamp of the scene's contents, renewed when the textures are replaced:
+	// a captured frame graph is replayed only for the stamp it was captured with, so a scene
+	// uploaded later at a released scene's address is never mistaken for it.
+	uint64_t version = 0;
+	// Plans and servers that launch on this scene (guarded by sceneHoldMutex()). A scene released
+	// while held leaves the registry at once but keeps its device arrays until the last holder
+	// lets go.
+	uint32_t holders = 0;
+	bool released = false;
 };
 
 namespace phx {
@@ -130,6 +139,13 @@ inline phx_status checkScene(PhxContext* ctx, const PhxScene* scene)
 	if (scene->ctx != ctx) return fail(PHX_ERR_INVALID_HANDLE, "scene belongs to another context");
 	return PHX_OK;
 }
+
+// A new value of the process-wide stamp counter (PhxScene::version).
+uint64_t nextSceneStamp();
+// Takes / drops a plan's or server's hold on a scene; dropping the last hold of a released scene
+// frees it.
+void holdScene(const PhxScene* s);
+void dropScene(const PhxScene* s);
 
 inline SceneView view(const PhxScene* s)
 {

@@ -92,7 +92,7 @@ private:
 	size_t used_ = 0;
 };
 
-enum class ReqState { Queued, Admitted, InFlight, Done };
+enum class ReqState { Queued, Admitted, InFlight, Done, Cancelled };
 
 struct Request {
 	uint64_t ticket = 0;
@@ -142,6 +142,7 @@ struct PhxServer {
 	std::unordered_map<uint64_t, Request*> byTicket;
 	uint64_t nextTicket = 1;
 	bool stop = false;
+	bool paused = false;  // phx_server_pause: nothing new is admitted or launched
 	std::string error;  // first dispatcher failure
 	std::thread dispatcher;
 	PhxServerStats stats{};
@@ -297,7 +298,7 @@ void dispatchLoop(PhxServer* s)
 		// One launch per idle slot while admitted work is waiting; the plain mode keeps a
 		// single launch in flight.
 		for (Slot& slot : s->slots) {
-			if (s->admitted.empty()) break;
+			if (s->admitted.empty() || s->paused) break;
 			if (slot.busy) continue;
 			if ((s->flags & PHX_SERVE_PLAIN) && anyBusy) break;
 			const hipError_t e = launchSlot(s, slot);
@@ -314,6 +315,8 @@ void dispatchLoop(PhxServer* s)
 		if (anyBusy) {
 			// poll the running launches; new submissions also wake the loop
 			s->cvWork.wait_for(lk, std::chrono::microseconds(20));
+		} else if (s->paused) {
+			s->cvWork.wait(lk);  // submit, cancel, pause and destroy notify
 		} else {
 			s->cvWork.wait(lk, [&] { return s->stop || !s->queue.empty() || !s->admitted.empty(); });
 		}
@@ -412,11 +415,14 @@ void phx_server_destroy(PhxServer* s)
 	{
 		std::lock_guard<std::mutex> g(s->mu);
 		s->stop = true;
+		s->paused = false;
 	}
 	s->cvWork.notify_all();
 	if (s->dispatcher.joinable()) s->dispatcher.join();
 	hipSetDevice(s->ctx->device);
+	const std::vector<const PhxScene*> held(s->scenes.begin(), s->scenes.end());
 	destroyServer(s);
+	for (const PhxScene* sc : held) dropScene(sc);
 }
 
 // Makes `scene` resident for the server: its bytes count against the budget, and the request
@@ -445,6 +451,7 @@ phx_status phx_server_add_scene(PhxServer* s, const PhxScene* scene)
 	s->arena.reset(poolBytes);
 	s->sceneBytes += scene->bytes;
 	s->scenes.insert(scene);
+	holdScene(scene);
 	return ok();
 }
 
@@ -456,6 +463,11 @@ phx_status phx_server_submit(PhxServer* s, const PhxRequest* req, uint64_t* tick
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_server_submit: kind %u", req->kind);
 	if (req->numRays && (!req->rays || !req->out))
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_server_submit: NULL rays/out");
+	if (req->numRays > 0x7FFFFFFFu)  // a fused launch indexes its rays with 31 bits
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_server_submit: %u rays (at most 2^31 - 1)",
+		            req->numRays);
+	if (!registry().has(req->scene))
+		return fail(PHX_ERR_INVALID_HANDLE, "phx_server_submit: released or unknown scene");
 	Request* r = new (std::nothrow) Request();
 	if (!r) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_server_submit: out of host memory");
 	r->req = *req;
@@ -495,9 +507,53 @@ phx_status phx_server_poll(PhxServer* s, uint64_t ticket, int* done, double* lat
 	std::lock_guard<std::mutex> g(s->mu);
 	auto it = s->byTicket.find(ticket);
 	if (it == s->byTicket.end()) return fail(PHX_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+	if (it->second->state == ReqState::Cancelled) {
+		*done = 1;
+		return fail(PHX_ERR_CANCELLED, "request %llu was cancelled", (unsigned long long)ticket);
+	}
 	*done = it->second->state == ReqState::Done;
 	if (*done && latencyMs) *latencyMs = it->second->latencyMs;
 	if (!s->error.empty()) return fail(PHX_ERR_HIP, "server failed: %s", s->error.c_str());
+	return ok();
+}
+
+phx_status phx_server_cancel(PhxServer* s, uint64_t ticket, int* cancelled)
+{
+	if (!s || !registry().has(s)) return fail(PHX_ERR_INVALID_HANDLE, "invalid server handle");
+	std::lock_guard<std::mutex> g(s->mu);
+	auto it = s->byTicket.find(ticket);
+	if (it == s->byTicket.end())
+		return fail(PHX_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+	Request* r = it->second;
+	int done = 0;
+	if (r->state == ReqState::Queued || r->state == ReqState::Admitted) {
+		std::deque<Request*>& from = r->state == ReqState::Queued ? s->queue : s->admitted;
+		from.erase(std::find(from.begin(), from.end(), r));
+		if (r->state == ReqState::Admitted && !(r->req.flags & PHX_MEM_DEVICE)) {
+			s->arena.release(r->rayOff, r->rayBytes);
+			s->arena.release(r->outOff, r->outBytes);
+		}
+		r->state = ReqState::Cancelled;
+		r->latencyMs = msSince(r->submitted);
+		s->stats.cancelled++;
+		done = 1;
+		s->cvDone.notify_all();
+		s->cvWork.notify_one();  // freed arena space may admit the next request
+	} else if (r->state == ReqState::Cancelled) {
+		done = 1;
+	}
+	if (cancelled) *cancelled = done;
+	return ok();
+}
+
+phx_status phx_server_pause(PhxServer* s, int pause)
+{
+	if (!s || !registry().has(s)) return fail(PHX_ERR_INVALID_HANDLE, "invalid server handle");
+	{
+		std::lock_guard<std::mutex> g(s->mu);
+		s->paused = pause != 0;
+	}
+	s->cvWork.notify_all();
 	return ok();
 }
 
@@ -508,7 +564,15 @@ phx_status phx_server_wait(PhxServer* s, uint64_t ticket, double* latencyMs)
 	auto it = s->byTicket.find(ticket);
 	if (it == s->byTicket.end()) return fail(PHX_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
 	Request* r = it->second;
-	s->cvDone.wait(lk, [&] { return r->state == ReqState::Done || !s->error.empty(); });
+	s->cvDone.wait(lk, [&] {
+		return r->state == ReqState::Done || r->state == ReqState::Cancelled || !s->error.empty();
+	});
+	if (r->state == ReqState::Cancelled) {
+		if (latencyMs) *latencyMs = r->latencyMs;
+		s->byTicket.erase(it);
+		delete r;
+		return fail(PHX_ERR_CANCELLED, "request %llu was cancelled", (unsigned long long)ticket);
+	}
 	if (r->state != ReqState::Done) return fail(PHX_ERR_HIP, "server failed: %s", s->error.c_str());
 	if (latencyMs) *latencyMs = r->latencyMs;
 	s->byTicket.erase(it);

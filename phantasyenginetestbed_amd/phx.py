@@ -40,7 +40,6 @@ OK, ERR_INVALID_ARGUMENT, ERR_INVALID_HANDLE, ERR_OUT_OF_MEMORY, ERR_MALFORMED_B
 MEM_DEVICE = 0x1
 ASYNC = 0x2
 TRAVERSAL_REFERENCE_ORDER = 0x00
-TRAVERSAL_NEAR_FIRST = 0x10
 KERNEL_ONE_RAY_PER_LANE = 0x100
 KERNEL_PACKET = 0x200
 BATCH_CLOSEST = 0
@@ -114,7 +113,7 @@ class ServerStats(C.Structure):
                 ("raysDone", C.c_uint64), ("launches", C.c_uint64), ("rejected", C.c_uint64),
                 ("bytesInUse", C.c_uint64), ("peakBytesInUse", C.c_uint64),
                 ("requestBudget", C.c_uint64), ("sceneBytes", C.c_uint64),
-                ("queued", C.c_uint32), ("pad", C.c_uint32)]
+                ("queued", C.c_uint32), ("cancelled", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
@@ -207,6 +206,8 @@ _SIGNATURES = {
     "phx_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]),
     "phx_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
+    "phx_server_cancel": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
+    "phx_server_pause": (C.c_int, [_P, C.c_int]),
     "phx_server_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "phx_server_replay": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_double), _U32,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -544,6 +545,15 @@ class Server:
         lat = C.c_double()
         check(lib().phx_server_wait(self.handle, ticket, C.byref(lat)))
         return lat.value
+
+    def cancel(self, ticket: int) -> bool:
+        """True if the request was cancelled before its launch."""
+        c = C.c_int()
+        check(lib().phx_server_cancel(self.handle, ticket, C.byref(c)))
+        return bool(c.value)
+
+    def pause(self, paused: bool = True) -> None:
+        check(lib().phx_server_pause(self.handle, int(paused)))
 
     def stats(self) -> dict:
         st = ServerStats()

@@ -153,6 +153,7 @@ class ServedScene:
     triangles: int
     scene: object            # phx.Scene
     pools: dict              # kind -> phx.HostBuffer of RAY (POOL_RAYS records)
+    bvh: object = None       # phx.BVH (host arrays), kept on request
 
 
 def _scene_geometry(name: str):
@@ -176,7 +177,8 @@ def _tile(rays: np.ndarray, n: int, dst: np.ndarray) -> None:
     dst[:] = np.tile(rays, reps)[:n]
 
 
-def served_scene(ctx: phx.Context, name: str, pool_rays: int = POOL_RAYS) -> ServedScene:
+def served_scene(ctx: phx.Context, name: str, pool_rays: int = POOL_RAYS,
+                 keep_bvh: bool = False) -> ServedScene:
     t, d, cam_spec, light, (w, h) = _scene_geometry(name)
     bvh = phx.BVH(t, d)
     scene = ctx.upload_scene(bvh.nodes, bvh.tris, bvh.datas)
@@ -191,7 +193,7 @@ def served_scene(ctx: phx.Context, name: str, pool_rays: int = POOL_RAYS) -> Ser
         hb = phx.HostBuffer(phx.RAY, pool_rays)
         _tile(rays, pool_rays, hb.array)
         pools[kind] = hb
-    return ServedScene(name, len(t), scene, pools)
+    return ServedScene(name, len(t), scene, pools, bvh if keep_bvh else None)
 
 
 def free_served_scene(s: ServedScene) -> None:
