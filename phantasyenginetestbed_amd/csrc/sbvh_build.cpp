@@ -459,71 +459,85 @@ private:
 		return split;
 	}
 
+	// SAH of two children: area * count of one plus area * count of the other, in that order.
+	static float sahPair(float areaA, float countA, float areaB, float countB)
+	{
+		const float a = areaA * countA;
+		return a + areaB * countB;
+	}
+
+	// Spatial split of the node's references (the last spec.numRef entries of mStack) at
+	// split.pos on axis split.dim. First the references are sorted in place into three runs:
+	// [base, lo) entirely below the plane, [lo, hi) straddling it, [hi, end) entirely above (a
+	// reference moved to the upper run is examined again at the same index). Then each straddler,
+	// in stack order, either stays whole on one side or is clipped into both halves (the upper
+	// half appended to the stack), whichever gives the children the smallest SAH -- Stich et al.
+	// 2009's reference unsplitting; ties keep it below, then above. Every swap and append happens
+	// in a fixed order, so the output is deterministic.
 	void performSpatialSplit(NodeSpec& left, NodeSpec& right, const NodeSpec& spec,
 	                         const SpatialSplit& split)
 	{
-		std::vector<Ref>& refs = mStack;
-		int leftStart = int(refs.size()) - spec.numRef;
-		int leftEnd = leftStart;
-		int rightStart = int(refs.size());
-		left.bounds = Box();
-		right.bounds = Box();
-		const int dim = split.dim;
-		const float pos = split.pos;
+		std::vector<Ref>& stack = mStack;
+		const int axis = split.dim;
+		const float plane = split.pos;
+		const int base = int(stack.size()) - spec.numRef;
+		int lo = base;
+		int hi = int(stack.size());
+		Box below, above;
 
-		for (int i = leftEnd; i < rightStart; i++) {
-			if (refs[i].bounds.mx[dim] <= pos) {
-				left.bounds.growBox(refs[i].bounds);
-				std::swap(refs[i], refs[leftEnd]);
-				leftEnd++;
-			} else if (refs[i].bounds.mn[dim] >= pos) {
-				right.bounds.growBox(refs[i].bounds);
-				rightStart--;
-				std::swap(refs[i], refs[rightStart]);
-				i--;
-			}
-		}
-
-		while (leftEnd < rightStart) {
-			Ref lref, rref;
-			splitReference(lref, rref, refs[leftEnd], dim, pos);
-			Box lub = left.bounds, rub = right.bounds, ldb = left.bounds, rdb = right.bounds;
-			lub.growBox(refs[leftEnd].bounds);
-			rub.growBox(refs[leftEnd].bounds);
-			ldb.growBox(lref.bounds);
-			rdb.growBox(rref.bounds);
-
-			int size = int(refs.size());
-			float lac = ntf(leftEnd - leftStart);
-			float rac = ntf(size - rightStart);
-			float lbc = ntf(leftEnd - leftStart + 1);
-			float rbc = ntf(size - rightStart + 1);
-
-			float unsplitLeft = lub.area() * lbc;
-			unsplitLeft = unsplitLeft + right.bounds.area() * rac;
-			float unsplitRight = left.bounds.area() * lac;
-			unsplitRight = unsplitRight + rub.area() * rbc;
-			float duplicate = ldb.area() * lbc;
-			duplicate = duplicate + rdb.area() * rbc;
-			float minSAH = fmin_first(fmin_first(unsplitLeft, unsplitRight), duplicate);
-
-			if (minSAH == unsplitLeft) {
-				left.bounds = lub;
-				leftEnd++;
-			} else if (minSAH == unsplitRight) {
-				right.bounds = rub;
-				rightStart--;
-				std::swap(refs[leftEnd], refs[rightStart]);
+		int i = lo;
+		while (i < hi) {
+			const Box& b = stack[i].bounds;
+			if (b.mx[axis] <= plane) {
+				below.growBox(b);
+				std::swap(stack[i], stack[lo]);
+				lo++;
+				i++;
+			} else if (b.mn[axis] >= plane) {
+				above.growBox(b);
+				hi--;
+				std::swap(stack[i], stack[hi]);
 			} else {
-				left.bounds = ldb;
-				right.bounds = rdb;
-				refs[leftEnd] = lref;
-				leftEnd++;
-				refs.push_back(rref);
+				i++;
 			}
 		}
-		left.numRef = leftEnd - leftStart;
-		right.numRef = int(refs.size()) - rightStart;
+
+		while (lo < hi) {
+			Ref half[2];
+			splitReference(half[0], half[1], stack[lo], axis, plane);
+			Box belowWhole = below, aboveWhole = above, belowHalf = below, aboveHalf = above;
+			belowWhole.growBox(stack[lo].bounds);
+			aboveWhole.growBox(stack[lo].bounds);
+			belowHalf.growBox(half[0].bounds);
+			aboveHalf.growBox(half[1].bounds);
+			const int nBelow = lo - base;
+			const int nAbove = int(stack.size()) - hi;
+			const float keepBelow =
+				sahPair(belowWhole.area(), ntf(nBelow + 1), above.area(), ntf(nAbove));
+			const float keepAbove =
+				sahPair(below.area(), ntf(nBelow), aboveWhole.area(), ntf(nAbove + 1));
+			const float clip =
+				sahPair(belowHalf.area(), ntf(nBelow + 1), aboveHalf.area(), ntf(nAbove + 1));
+			const float best = fmin_first(fmin_first(keepBelow, keepAbove), clip);
+			if (best == keepBelow) {
+				below = belowWhole;
+				lo++;
+			} else if (best == keepAbove) {
+				above = aboveWhole;
+				hi--;
+				std::swap(stack[lo], stack[hi]);
+			} else {
+				below = belowHalf;
+				above = aboveHalf;
+				stack[lo] = half[0];
+				lo++;
+				stack.push_back(half[1]);
+			}
+		}
+		left.bounds = below;
+		right.bounds = above;
+		left.numRef = lo - base;
+		right.numRef = int(stack.size()) - hi;
 	}
 
 	TNode* buildNode(const NodeSpec& spec, int level)
