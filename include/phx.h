@@ -572,18 +572,23 @@ void phx_renderer_release(PhxRenderer* renderer);
  * (interpretRayHitKernel), shading of every light with a shadow ray each (gBufferMaterialKernel),
  * any hit, shadow logic; one bounce ray per 2x2 pixel block (createSecondaryRaysKernel), closest
  * hit, shading x path throughput with shadow rays (materialKernel), any hit, shadow logic added
- * at half resolution; then, with PHX_FRAME_TAA, the TAA resolve (taa.frag:25-71 for a static
- * camera, FP32 history, 3x3 clamp). The primary hits' shadow rays (coherent: a pixel tile's rays
- * towards one light) and, when width and height are multiples of 8 (camera rays in 8x8-tile
- * order), camera rays are traced as packets (PHX_KERNEL_PACKET); bounce rays, and camera rays in
- * row-major order, by the persistent kernel; the bounce hits' shadow rays (scattered origins) one
- * ray per lane (PHX_KERNEL_ONE_RAY_PER_LANE). `rgba` (width*height*4 floats, row-major; host or device with
- * PHX_MEM_DEVICE; may be NULL) receives the output. Random numbers: PCG32 streams keyed by
- * (seed, frameIndex, pass, record). The frame's kernels read the per-frame values (jitter, frame
- * index, seed, TAA reset and history buffer) from a small device block written before the frame,
- * so with PHX_FRAME_GRAPH the launches are captured once as a hipGraph -- again whenever the
- * scene, camera, PHX_FRAME_TAA / PHX_FRAME_NO_CLAMP or the context's launch shape change -- and
- * replayed for the following frames; the output is the same as without it. */
+ * at half resolution; then, with PHX_FRAME_TAA, the TAA resolve (taa.frag:25-71) with camera
+ * motion (velocity_gen.frag:49-71): each pixel's velocity is the screen position of its primary
+ * hit (for a miss, of its ray direction) under `cam` minus that under the camera of this
+ * renderer's previous frame; the FP32 history is read bilinearly at the pixel centre minus the
+ * velocity (a position off the image starts the pixel afresh), trusted by velocity and roughness
+ * as taa.frag does, and clamped to the 3x3 neighbourhood. The primary hits' shadow rays
+ * (coherent: a pixel tile's rays towards one light) and, when width and height are multiples of 8
+ * (camera rays in 8x8-tile order), camera rays are traced as packets (PHX_KERNEL_PACKET); bounce
+ * rays, and camera rays in row-major order, by the persistent kernel; the bounce hits' shadow
+ * rays (scattered origins) one ray per lane (PHX_KERNEL_ONE_RAY_PER_LANE). `rgba`
+ * (width*height*4 floats, row-major; host or device with PHX_MEM_DEVICE; may be NULL) receives
+ * the output. Random numbers: PCG32 streams keyed by (seed, frameIndex, pass, record). The
+ * frame's kernels read the per-frame values (camera, previous camera, jitter, frame index, seed,
+ * TAA reset and history buffer) from a small device block written before the frame, so with
+ * PHX_FRAME_GRAPH the launches are captured once as a hipGraph -- again whenever the scene,
+ * PHX_FRAME_TAA / PHX_FRAME_NO_CLAMP or the context's launch shape change -- and replayed for the
+ * following frames, a moving camera included; the output is the same as without it. */
 phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* renderer, const PhxScene* scene,
                             const PhxCameraDef* cam, uint32_t frameIndex, uint64_t seed,
                             uint32_t flags, float* rgba);
@@ -593,6 +598,7 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* renderer, const PhxSce
 #define PHX_BUFFER_GBUFFER 2u     /* 3 float4 per primary record (pixel_id order)             */
 #define PHX_BUFFER_BOUNCE_RAYS 3u /* PhxRayIn per half-resolution record                      */
 #define PHX_BUFFER_HISTORY 4u     /* float4 per pixel: TAA history (rgb, sample count)        */
+#define PHX_BUFFER_VELOCITY 5u    /* float2 per pixel, row-major: the last TAA frame's velocity */
 
 /* Device address and size of one of the renderer's buffers (valid until the next frame). */
 phx_status phx_renderer_buffer(PhxRenderer* renderer, uint32_t which, const void** devPtr,

@@ -671,29 +671,114 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 	free(inl);
 }
 
-/* TAA resolve (taa.frag:25-71) with zero velocity (static camera), FP32 history and a 3x3
- * neighbourhood clamp of the history colour to the current frame's min/max (clampHistory != 0).
- * cur/histIn/histOut/out: w*h RGBA row-major; roughness: w*h (may be NULL = 0). reset != 0 starts
- * the history from this frame. */
-void oracle_taa(const float* cur, const float* histIn, const float* roughness, uint32_t w,
-                uint32_t h, int reset, int clampHistory, float* histOut, float* out)
+/* Continuous pixel coordinates (px, py) -- the (x + jx, y + jy) of the primary rays -- at which
+ * the camera cam (CameraDef: origin, dir, dX, dY) sees the point origin + v; v may be a direction
+ * (a point at infinity). Inverts the pinhole formula normalize(dir + cx dX + cy dY), cx = px / w *
+ * 2 - 1, using that dir, dX and dY are orthogonal: cx = (v . dX) / ((v . dir) |dX|^2). This is
+ * getUVCoord of velocity_gen.frag:40-47 for the tracer's camera. Returns 0 for a point that is
+ * not in front of the camera. */
+static int project(const float cam[12], V3 v, uint32_t w, uint32_t h, float* px, float* py)
+{
+	const V3 dir = v3(cam[3], cam[4], cam[5]), dX = v3(cam[6], cam[7], cam[8]),
+	         dY = v3(cam[9], cam[10], cam[11]);
+	const float a = dot(v, dir);
+	if (!(a > 0.0f)) return 0;
+	const float cx = dot(v, dX) / (a * dot(dX, dX));
+	const float cy = dot(v, dY) / (a * dot(dY, dY));
+	*px = ((cx + 1.0f) * 0.5f) * (float)w;
+	*py = ((cy + 1.0f) * 0.5f) * (float)h;
+	return 1;
+}
+
+/* Screen-space velocity of every pixel (velocity_gen.frag:49-71), in pixels, row-major (w*h*2):
+ * the pixel's primary hit (G-buffer record position; for a miss the primary ray's direction, a
+ * point at infinity) projected by the current camera minus its projection by the previous camera.
+ * A point behind the previous camera gets FLT_MAX (its history is off screen). */
+void oracle_velocity(const float cam[12], const float prevCam[12], const OrHitInfo* gbuf,
+                     const OrRay* prim, uint32_t w, uint32_t h, float* vel)
+{
+	for (uint32_t y = 0; y < h; y++)
+		for (uint32_t x = 0; x < w; x++) {
+			const uint32_t rec = pixel_id(x, y, w, h);
+			V3 vc, vp;
+			if (gbuf[rec].d3[3] >= 0.0f) {
+				const V3 pos = ld3(gbuf[rec].d1);
+				vc = sub(pos, v3(cam[0], cam[1], cam[2]));
+				vp = sub(pos, v3(prevCam[0], prevCam[1], prevCam[2]));
+			} else {
+				vc = vp = ld3(prim[rec].d);
+			}
+			float cx = 0.0f, cy = 0.0f, px = 0.0f, py = 0.0f;
+			float* o = &vel[2 * ((size_t)y * w + x)];
+			if (project(cam, vc, w, h, &cx, &cy) && project(prevCam, vp, w, h, &px, &py)) {
+				o[0] = cx - px;
+				o[1] = cy - py;
+			} else {
+				o[0] = FLT_MAX;
+				o[1] = FLT_MAX;
+			}
+		}
+}
+
+/* Bilinear sample of a w x h image of `ch` floats per pixel at continuous pixel coordinates
+ * (hx, hy) (texel centres at i + 0.5), edge texels repeated (GL_LINEAR with clamp to edge):
+ * rows blended with lerp (1 - f) * a + f * b (MathHelpers.inl:66-70), first along x. */
+static void bilinear(const float* img, uint32_t ch, uint32_t w, uint32_t h, float hx, float hy,
+                     float* o)
+{
+	const float tx = hx - 0.5f, ty = hy - 0.5f;
+	const float x0f = floorf(tx), y0f = floorf(ty);
+	const float fx = tx - x0f, fy = ty - y0f;
+	const int x0 = (int)x0f, y0 = (int)y0f;
+	const int xa = x0 < 0 ? 0 : (x0 > (int)w - 1 ? (int)w - 1 : x0);
+	const int xb = x0 + 1 < 0 ? 0 : (x0 + 1 > (int)w - 1 ? (int)w - 1 : x0 + 1);
+	const int ya = y0 < 0 ? 0 : (y0 > (int)h - 1 ? (int)h - 1 : y0);
+	const int yb = y0 + 1 < 0 ? 0 : (y0 + 1 > (int)h - 1 ? (int)h - 1 : y0 + 1);
+	for (uint32_t c = 0; c < ch; c++) {
+		const float a = img[((size_t)ya * w + xa) * ch + c], b = img[((size_t)ya * w + xb) * ch + c];
+		const float d = img[((size_t)yb * w + xa) * ch + c], e = img[((size_t)yb * w + xb) * ch + c];
+		const float top = a * (1.0f - fx) + b * fx;
+		const float bot = d * (1.0f - fx) + e * fx;
+		o[c] = top * (1.0f - fy) + bot * fy;
+	}
+}
+
+/* TAA resolve (taa.frag:25-71), FP32 history. With this frame's velocity vel (NULL: zero) the
+ * history is read bilinearly at (x + 0.5, y + 0.5) - vel; a reprojected position outside the
+ * image starts the pixel afresh. The previous frame's velocity velIn (NULL: zero) at that position
+ * sets the velocity distrust min(1, 1.2 sqrt(| |v| - |v_hist| |)), the roughness the roughness
+ * distrust mix(3, 1, roughness). With clampHistory != 0 the history colour is clamped to the
+ * current frame's 3x3 neighbourhood min/max (not in the reference). cur/histIn/histOut/out: w*h
+ * RGBA row-major; roughness: w*h (may be NULL = 0); vel, velIn: w*h*2. reset != 0 starts the
+ * history from this frame. */
+void oracle_taa(const float* cur, const float* histIn, const float* roughness, const float* velIn,
+                const float* vel, uint32_t w, uint32_t h, int reset, int clampHistory,
+                float* histOut, float* out)
 {
 	for (uint32_t y = 0; y < h; y++)
 		for (uint32_t x = 0; x < w; x++) {
 			size_t p = (size_t)y * w + x;
 			const float* c = &cur[4 * p];
+			const float vx = vel ? vel[2 * p] : 0.0f, vy = vel ? vel[2 * p + 1] : 0.0f;
+			const float hx = ((float)x + 0.5f) - vx, hy = ((float)y + 0.5f) - vy;
+			const int onScreen = hx >= 0.0f && hx <= (float)w && hy >= 0.0f && hy <= (float)h;
 			float o[3], n2;
-			if (reset) {
+			if (reset || !onScreen) {
 				o[0] = c[0];
 				o[1] = c[1];
 				o[2] = c[2];
 				n2 = 1.0f;
 			} else {
+				float pv[2] = {0.0f, 0.0f};
+				if (velIn) bilinear(velIn, 2u, w, h, hx, hy, pv);
+				const float lv = sqrtf(vx * vx + vy * vy);
+				const float lp = sqrtf(pv[0] * pv[0] + pv[1] * pv[1]);
 				float rough = roughness ? roughness[p] : 0.0f;
-				float vDistrust = min_num(1.0f, 1.2f * sqrtf(fabsf(0.0f - 0.0f)));
+				float vDistrust = min_num(1.0f, 1.2f * sqrtf(fabsf(lv - lp)));
 				float rDistrust = 3.0f * (1.0f - rough) + 1.0f * rough;
 				float trust = clamp_num(1.0f - rDistrust * vDistrust, 0.0f, 1.0f);
-				const float* hs = &histIn[4 * p];
+				float hs[4];
+				bilinear(histIn, 4u, w, h, hx, hy, hs);
 				float hc[3] = {hs[0], hs[1], hs[2]};
 				if (clampHistory) {
 					float lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[0], c[1], c[2]};

@@ -159,8 +159,11 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
                   uint32_t numLights, const float cam[12], uint32_t w, uint32_t h, float jx,
                   float jy, uint64_t seed, uint32_t frame, float* color, OrHitInfo* gbufOut,
                   OrRay* bounceOut, uint32_t threads);
-void oracle_taa(const float* cur, const float* histIn, const float* roughness, uint32_t w,
-                uint32_t h, int reset, int clampHistory, float* histOut, float* out);
+void oracle_velocity(const float cam[12], const float prevCam[12], const OrHitInfo* gbuf,
+                     const OrRay* prim, uint32_t w, uint32_t h, float* vel);
+void oracle_taa(const float* cur, const float* histIn, const float* roughness, const float* velIn,
+                const float* vel, uint32_t w, uint32_t h, int reset, int clampHistory,
+                float* histOut, float* out);
 void oracle_halton23(uint32_t f, float* jx, float* jy);
 
 #ifdef __cplusplus

@@ -90,7 +90,8 @@ def _configure(L: C.CDLL) -> C.CDLL:
     L.oracle_texel_coord.argtypes = [F, _U32]
     L.oracle_texel_coord.restype = _U32
     L.oracle_linear_table.argtypes = [_P]
-    L.oracle_taa.argtypes = [_P, _P, _P, _U32, _U32, C.c_int, C.c_int, _P, _P]
+    L.oracle_taa.argtypes = [_P, _P, _P, _P, _P, _U32, _U32, C.c_int, C.c_int, _P, _P]
+    L.oracle_velocity.argtypes = [_P, _P, _P, _P, _U32, _U32, _P]
     L.oracle_halton23.argtypes = [_U32, C.POINTER(F), C.POINTER(F)]
     L.oracle_sample_ggx.argtypes = [F, F, F, _P, _P, _P]
     L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, _U32, _P]
@@ -280,18 +281,30 @@ def linear_table() -> np.ndarray:
     return out
 
 
-def taa(cur, hist_in, roughness, reset: bool, clamp: bool = True):
-    """Returns (history out, output), both (h, w, 4) float32."""
+def taa(cur, hist_in, roughness, reset: bool, clamp: bool = True, vel_in=None, vel=None):
+    """Returns (history out, output), both (h, w, 4) float32. vel / vel_in: this and the previous
+    frame's velocity, (h, w, 2) pixels (None: zero)."""
     h, w = cur.shape[:2]
     cur = np.ascontiguousarray(cur, np.float32)
     hist_in = np.ascontiguousarray(hist_in if hist_in is not None else np.zeros_like(cur),
                                    np.float32)
     rough = None if roughness is None else np.ascontiguousarray(roughness, np.float32)
+    vel_in = None if vel_in is None else np.ascontiguousarray(vel_in, np.float32)
+    vel = None if vel is None else np.ascontiguousarray(vel, np.float32)
     hist_out = np.empty_like(cur)
     out = np.empty_like(cur)
-    olib().oracle_taa(_p(cur), _p(hist_in), _p(rough), w, h, int(reset), int(clamp),
-                      _p(hist_out), _p(out))
+    olib().oracle_taa(_p(cur), _p(hist_in), _p(rough), _p(vel_in), _p(vel), w, h, int(reset),
+                      int(clamp), _p(hist_out), _p(out))
     return hist_out, out
+
+
+def velocity(cam12, prev_cam12, gbuf, prim, w: int, h: int) -> np.ndarray:
+    """(h, w, 2) screen-space velocity in pixels (current minus previous camera's projection)."""
+    cam = np.ascontiguousarray(cam12, np.float32)
+    prev = np.ascontiguousarray(prev_cam12, np.float32)
+    vel = np.zeros((h, w, 2), np.float32)
+    olib().oracle_velocity(_p(cam), _p(prev), _p(gbuf), _p(prim), w, h, _p(vel))
+    return vel
 
 
 def halton23(f: int):

@@ -151,7 +151,6 @@ __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict
 __global__ __launch_bounds__(kBlock1D) void k_shade(const float4* __restrict__ info,
                                                     const float4* __restrict__ rays,
                                                     const float4* __restrict__ throughput,
-                                                    float cx, float cy, float cz,
                                                     const PhxSphereLight* __restrict__ lights,
                                                     uint32_t numLights, uint32_t count,
                                                     const FrameParams* __restrict__ fp,
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(kBlock1D) void k_shade(const float4* __restrict__ i
 		view = v3(-rd.x, -rd.y, -rd.z);
 		mask = xyz(throughput[id]);
 	} else {
-		view = normalize(sub(v3(cx, cy, cz), p));
+		view = normalize(sub(v3(fp->cam.origin[0], fp->cam.origin[1], fp->cam.origin[2]), p));
 		mask = v3(1.0f, 1.0f, 1.0f);
 	}
 	Pcg32 rng = frame_rng(seed, frame, purpose, id);
@@ -223,8 +222,7 @@ __global__ __launch_bounds__(kBlock1D) void k_shade(const float4* __restrict__ i
 // on roughness, throughput albedo * 0.6, tMin 1e-3; alpha <= 0.5 takes the reference's glass
 // path. A block pixel without a hit gets the dummy ray and zero throughput.
 __global__ __launch_bounds__(kBlock1D) void k_gen_secondary(const float4* __restrict__ gbuf,
-                                                            uint32_t w, uint32_t h, float cx,
-                                                            float cy, float cz,
+                                                            uint32_t w, uint32_t h,
                                                             const FrameParams* __restrict__ fp,
                                                             float4* __restrict__ rays,
                                                             float4* __restrict__ throughput)
@@ -249,7 +247,8 @@ __global__ __launch_bounds__(kBlock1D) void k_gen_secondary(const float4* __rest
 	}
 	const V3 gp = xyz(d1), gn = xyz(d2), alb = xyz(d3);
 	const float rough = d1.w;
-	const V3 fromCam = normalize(sub(gp, v3(cx, cy, cz)));
+	const V3 fromCam =
+		normalize(sub(gp, v3(fp->cam.origin[0], fp->cam.origin[1], fp->cam.origin[2])));
 	const float r1 = rng.uniform();
 	const float r2 = rng.uniform();
 	float tMin = 0.001f;
@@ -305,12 +304,64 @@ __global__ __launch_bounds__(kBlock1D) void k_shadow_logic(float4* __restrict__ 
 	*px = make_float4(c0, c1, c2, 1.0f);
 }
 
-// TAA resolve (taa.frag:25-71) for a static camera (zero velocity: trust 1, history sample at the
-// same pixel), FP32 history with the effective sample count in alpha, and the history colour
-// clamped to the current frame's 3x3 neighbourhood min/max (clampHistory).
+// Continuous pixel coordinates (px, py) -- the (x + jx, y + jy) of the primary rays -- at which
+// camera `cam` sees the point cam.origin + v (v may be a direction: a point at infinity), the
+// inverse of the pinhole formula normalize(dir + cx dX + cy dY), cx = px / w * 2 - 1, for
+// orthogonal dir, dX, dY: cx = (v . dX) / ((v . dir) |dX|^2) (getUVCoord, velocity_gen.frag:40-47,
+// for the tracer's camera). False for a point not in front of the camera.
+__device__ __forceinline__ bool project(const PhxCameraDef& cam, V3 v, uint32_t w, uint32_t h,
+                                        float& px, float& py)
+{
+	const V3 dir = v3(cam.dir[0], cam.dir[1], cam.dir[2]);
+	const V3 dX = v3(cam.dX[0], cam.dX[1], cam.dX[2]), dY = v3(cam.dY[0], cam.dY[1], cam.dY[2]);
+	const float a = dot(v, dir);
+	if (!(a > 0.0f)) return false;
+	const float cx = dot(v, dX) / (a * dot(dX, dX));
+	const float cy = dot(v, dY) / (a * dot(dY, dY));
+	px = ((cx + 1.0f) * 0.5f) * float(w);
+	py = ((cy + 1.0f) * 0.5f) * float(h);
+	return true;
+}
+
+// Bilinear tap of a w x h image at continuous pixel coordinates (hx, hy), texel centres at
+// i + 0.5, edge texels repeated (GL_LINEAR, clamp to edge); rows blended first, each with
+// lerp (1 - f) a + f b (MathHelpers.inl:66-70).
+struct Taps {
+	size_t i00, i01, i10, i11;
+	float fx, fy;
+};
+
+__device__ __forceinline__ Taps taps(uint32_t w, uint32_t h, float hx, float hy)
+{
+	const float tx = hx - 0.5f, ty = hy - 0.5f;
+	const float x0f = floorf(tx), y0f = floorf(ty);
+	const int x0 = int(x0f), y0 = int(y0f);
+	const int xa = min(max(x0, 0), int(w) - 1), xb = min(max(x0 + 1, 0), int(w) - 1);
+	const int ya = min(max(y0, 0), int(h) - 1), yb = min(max(y0 + 1, 0), int(h) - 1);
+	return Taps{size_t(ya) * w + xa, size_t(ya) * w + xb, size_t(yb) * w + xa,
+	            size_t(yb) * w + xb, tx - x0f, ty - y0f};
+}
+
+__device__ __forceinline__ float lerp2(float a, float b, float d, float e, const Taps& t)
+{
+	const float top = a * (1.0f - t.fx) + b * t.fx;
+	const float bot = d * (1.0f - t.fx) + e * t.fx;
+	return top * (1.0f - t.fy) + bot * t.fy;
+}
+
+// TAA resolve (taa.frag:25-71) with the screen-space velocity of velocity_gen.frag:49-71.
+// Velocity of a pixel, in pixels: its primary hit (record position; for a miss the primary ray's
+// direction) projected by this frame's camera minus its projection by the previous frame's
+// (FLT_MAX when the point is behind the previous camera). The history is read bilinearly at
+// (x + 0.5, y + 0.5) - velocity; a position off the image starts the pixel afresh. Trust =
+// clamp(1 - mix(3, 1, roughness) * min(1, 1.2 sqrt(| |v| - |v_hist| |)), 0, 1) with v_hist the
+// previous frame's velocity read at the same position. FP32 history with the effective sample
+// count in alpha; the history colour is clamped to the current frame's 3x3 neighbourhood
+// min/max (clampHistory; not in the reference).
 __global__ __launch_bounds__(kBlock1D) void k_taa(const float4* __restrict__ cur,
-                                                  float4* hist0, float4* hist1,
-                                                  const float4* __restrict__ gbuf, uint32_t w,
+                                                  float4* hist0, float4* hist1, float2* vel0,
+                                                  float2* vel1, const float4* __restrict__ gbuf,
+                                                  const float4* __restrict__ prim, uint32_t w,
                                                   uint32_t h, uint32_t clampHistory,
                                                   const FrameParams* __restrict__ fp,
                                                   float4* __restrict__ out)
@@ -321,21 +372,56 @@ __global__ __launch_bounds__(kBlock1D) void k_taa(const float4* __restrict__ cur
 	const uint32_t reset = fp->taaReset;
 	const float4* histIn = fp->histIn ? hist1 : hist0;
 	float4* histOut = fp->histIn ? hist0 : hist1;
+	const float2* velIn = fp->histIn ? vel1 : vel0;
+	float2* velOut = fp->histIn ? vel0 : vel1;
 	const size_t p = size_t(y) * w + x;
+	const uint32_t rec = pixel_id(x, y, w, h);
+
+	// velocity of this pixel
+	const float4 g1 = gbuf[3 * size_t(rec)];
+	const float4 g3 = gbuf[3 * size_t(rec) + 2];
+	V3 vc, vp;
+	if (g3.w >= 0.0f) {
+		const V3 pos = xyz(g1);
+		vc = sub(pos, v3(fp->cam.origin[0], fp->cam.origin[1], fp->cam.origin[2]));
+		vp = sub(pos, v3(fp->prevCam.origin[0], fp->prevCam.origin[1], fp->prevCam.origin[2]));
+	} else {
+		vc = vp = xyz(prim[2 * size_t(rec) + 1]);
+	}
+	float cx = 0.0f, cy = 0.0f, px = 0.0f, py = 0.0f;
+	float vx = FLT_MAX, vy = FLT_MAX;
+	if (project(fp->cam, vc, w, h, cx, cy) && project(fp->prevCam, vp, w, h, px, py)) {
+		vx = cx - px;
+		vy = cy - py;
+	}
+	velOut[p] = make_float2(vx, vy);
+
 	const float4 c = cur[p];
+	const float hx = (float(x) + 0.5f) - vx, hy = (float(y) + 0.5f) - vy;
+	const bool onScreen = hx >= 0.0f && hx <= float(w) && hy >= 0.0f && hy <= float(h);
 	float o0, o1, o2, n2;
-	if (reset) {
+	if (reset || !onScreen) {
 		o0 = c.x;
 		o1 = c.y;
 		o2 = c.z;
 		n2 = 1.0f;
 	} else {
-		const float rough = gbuf[3 * size_t(pixel_id(x, y, w, h))].w;
-		const float vDistrust = min_num(1.0f, 1.2f * sqrtf(fabsf(0.0f - 0.0f)));
+		const Taps t = taps(w, h, hx, hy);
+		const float2 v00 = velIn[t.i00], v01 = velIn[t.i01], v10 = velIn[t.i10],
+		             v11 = velIn[t.i11];
+		const float pvx = lerp2(v00.x, v01.x, v10.x, v11.x, t);
+		const float pvy = lerp2(v00.y, v01.y, v10.y, v11.y, t);
+		const float lv = sqrtf(vx * vx + vy * vy);
+		const float lp = sqrtf(pvx * pvx + pvy * pvy);
+		const float rough = g1.w;
+		const float vDistrust = min_num(1.0f, 1.2f * sqrtf(fabsf(lv - lp)));
 		const float rDistrust = 3.0f * (1.0f - rough) + 1.0f * rough;
 		const float trust = clamp_num(1.0f - rDistrust * vDistrust, 0.0f, 1.0f);
-		const float4 hs = histIn[p];
-		float h0 = hs.x, h1 = hs.y, h2 = hs.z;
+		const float4 a = histIn[t.i00], b = histIn[t.i01], d = histIn[t.i10], e = histIn[t.i11];
+		float h0 = lerp2(a.x, b.x, d.x, e.x, t);
+		float h1 = lerp2(a.y, b.y, d.y, e.y, t);
+		float h2 = lerp2(a.z, b.z, d.z, e.z, t);
+		const float hn = lerp2(a.w, b.w, d.w, e.w, t);
 		if (clampHistory) {
 			float lo0 = c.x, lo1 = c.y, lo2 = c.z, hi0 = c.x, hi1 = c.y, hi2 = c.z;
 			const uint32_t y0 = y ? y - 1u : 0u, y1 = y + 1u < h ? y + 1u : h - 1u;
@@ -354,7 +440,7 @@ __global__ __launch_bounds__(kBlock1D) void k_taa(const float4* __restrict__ cur
 			h1 = clamp_num(h1, lo1, hi1);
 			h2 = clamp_num(h2, lo2, hi2);
 		}
-		n2 = hs.w * trust + 1.0f;
+		n2 = hn * trust + 1.0f;
 		const float blend = 1.0f - 1.0f / n2;
 		o0 = c.x * (1.0f - blend) + h0 * blend;
 		o1 = c.y * (1.0f - blend) + h1 * blend;
@@ -381,26 +467,23 @@ hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t
 }
 
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
-                       const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
-                       uint32_t count, const FrameParams* fp, uint32_t purpose, float4* contrib,
-                       PhxRayIn* shadow, hipStream_t stream)
+                       const PhxSphereLight* lights, uint32_t numLights, uint32_t count,
+                       const FrameParams* fp, uint32_t purpose, float4* contrib, PhxRayIn* shadow,
+                       hipStream_t stream)
 {
 	if (count == 0 || numLights == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_shade, dim3(blocks1d(count)), dim3(kBlock1D), 0, stream, info,
-	                   reinterpret_cast<const float4*>(rays), throughput, camPos[0], camPos[1],
-	                   camPos[2], lights, numLights, count, fp, purpose, contrib,
-	                   reinterpret_cast<float4*>(shadow));
+	                   reinterpret_cast<const float4*>(rays), throughput, lights, numLights, count,
+	                   fp, purpose, contrib, reinterpret_cast<float4*>(shadow));
 	return hipGetLastError();
 }
 
-hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const float camPos[3],
-                              const FrameParams* fp, PhxRayIn* rays, float4* throughput,
-                              hipStream_t stream)
+hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const FrameParams* fp,
+                              PhxRayIn* rays, float4* throughput, hipStream_t stream)
 {
 	if (w < 2u || h < 2u) return hipSuccess;
 	hipLaunchKernelGGL(k_gen_secondary, grid2d(w / 2u, h / 2u), dim3(kBlock1D), 0, stream, gbuf, w,
-	                   h, camPos[0], camPos[1], camPos[2], fp, reinterpret_cast<float4*>(rays),
-	                   throughput);
+	                   h, fp, reinterpret_cast<float4*>(rays), throughput);
 	return hipGetLastError();
 }
 
@@ -414,13 +497,14 @@ hipError_t launchShadowLogic(float4* img, uint32_t w, uint32_t h, uint32_t scale
 	return hipGetLastError();
 }
 
-hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, const float4* gbuf,
-                     uint32_t w, uint32_t h, bool clampHistory, const FrameParams* fp, float4* out,
-                     hipStream_t stream)
+hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, float2* vel0, float2* vel1,
+                     const float4* gbuf, const PhxRayIn* prim, uint32_t w, uint32_t h,
+                     bool clampHistory, const FrameParams* fp, float4* out, hipStream_t stream)
 {
 	if (w == 0 || h == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_taa, grid2d(w, h), dim3(kBlock1D), 0, stream, cur, hist0, hist1, gbuf, w,
-	                   h, uint32_t(clampHistory), fp, out);
+	hipLaunchKernelGGL(k_taa, grid2d(w, h), dim3(kBlock1D), 0, stream, cur, hist0, hist1, vel0,
+	                   vel1, gbuf, reinterpret_cast<const float4*>(prim), w, h,
+	                   uint32_t(clampHistory), fp, out);
 	return hipGetLastError();
 }
 

@@ -127,15 +127,18 @@ struct FrameParams {
 	uint32_t frame;     // frame index (random-number streams)
 	uint32_t taaReset;  // TAA: start the history afresh
 	uint64_t seed;
-	uint32_t histIn;    // TAA: history buffer read (0 or 1); the other one is written
+	uint32_t histIn;    // TAA: history (and velocity) buffer read (0 or 1); the other is written
 	uint32_t pad;
+	PhxCameraDef cam;      // this frame's camera
+	PhxCameraDef prevCam;  // the previous frame's camera (TAA velocity)
 };
-static_assert(sizeof(FrameParams) == 32, "FrameParams layout");
+static_assert(sizeof(FrameParams) == 128, "FrameParams layout");
 
 // Writes `value` to *dst on the stream (a one-thread kernel: its argument is copied at launch).
 hipError_t launchSetFrameParams(FrameParams* dst, const FrameParams& value, hipStream_t stream);
 
-// Pinhole primary rays. With `fp` the jitter is read from fp->jx, fp->jy instead of jx, jy.
+// Pinhole primary rays. With `fp` the camera and jitter are read from fp->cam, fp->jx, fp->jy
+// instead of cam, jx, jy.
 hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h, float jx,
                                 float jy, float tMin, float tMax, uint32_t order, PhxRayIn* rays,
                                 hipStream_t stream, const FrameParams* fp = nullptr);
@@ -146,21 +149,22 @@ hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h,
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
                            const PhxTriangleData* datas, uint32_t numTris, const PhxMaterial* mats,
                            uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream);
-// The seed and frame index of the random-number streams come from fp.
+// The seed and frame index of the random-number streams, and the camera position, come from fp.
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
-                       const float camPos[3], const PhxSphereLight* lights, uint32_t numLights,
-                       uint32_t count, const FrameParams* fp, uint32_t purpose, float4* contrib,
-                       PhxRayIn* shadow, hipStream_t stream);
-hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const float camPos[3],
-                              const FrameParams* fp, PhxRayIn* rays, float4* throughput,
-                              hipStream_t stream);
+                       const PhxSphereLight* lights, uint32_t numLights, uint32_t count,
+                       const FrameParams* fp, uint32_t purpose, float4* contrib, PhxRayIn* shadow,
+                       hipStream_t stream);
+hipError_t launchGenSecondary(const float4* gbuf, uint32_t w, uint32_t h, const FrameParams* fp,
+                              PhxRayIn* rays, float4* throughput, hipStream_t stream);
 hipError_t launchShadowLogic(float4* img, uint32_t w, uint32_t h, uint32_t scale, bool addTo,
                              const uint8_t* inLight, const float4* contrib, uint32_t numLights,
                              hipStream_t stream);
-// TAA resolve: reads history hist[fp->histIn] (ignored if fp->taaReset), writes the other one.
-hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, const float4* gbuf,
-                     uint32_t w, uint32_t h, bool clampHistory, const FrameParams* fp, float4* out,
-                     hipStream_t stream);
+// TAA resolve with camera motion: writes every pixel's velocity (fp->cam against fp->prevCam)
+// to vel[1 - fp->histIn], reprojects the history hist[fp->histIn] and the previous velocity
+// vel[fp->histIn] (both ignored if fp->taaReset), writes the other history buffer.
+hipError_t launchTaa(const float4* cur, float4* hist0, float4* hist1, float2* vel0, float2* vel1,
+                     const float4* gbuf, const PhxRayIn* prim, uint32_t w, uint32_t h,
+                     bool clampHistory, const FrameParams* fp, float4* out, hipStream_t stream);
 
 }  // namespace phx
 

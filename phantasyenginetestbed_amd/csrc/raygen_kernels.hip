@@ -24,6 +24,7 @@ __global__ __launch_bounds__(256) void k_gen_primary(PhxCameraDef cam, uint32_t 
 	const uint32_t y = blockIdx.y * 16u + (threadIdx.x >> 4);
 	if (x >= w || y >= h) return;
 	if (fp) {
+		cam = fp->cam;
 		jx = fp->jx;
 		jy = fp->jy;
 	}

@@ -27,20 +27,23 @@ struct PhxRenderer {
 	uint8_t* inLight = nullptr;  // n * lightCap
 	float4* color = nullptr;
 	float4* hist[2] = {nullptr, nullptr};
+	float2* vel[2] = {nullptr, nullptr};  // TAA: velocity of the frame that wrote hist[i]
 	float4* out = nullptr;
 	uint32_t histIndex = 0;
 	bool haveHistory = false;
+	PhxCameraDef lastCam{};  // the previous frame's camera (TAA velocity)
+	bool haveLastCam = false;
 	float4* lastOut = nullptr;
 	FrameParams* params = nullptr;  // per-frame values the frame kernels read (device)
 	// The bounce rays' closest hits run on a side stream beside the primary shadow rays (fork
 	// after the bounce rays are generated, join before the bounce records).
 	hipStream_t side = nullptr;
 	hipEvent_t fork = nullptr, join = nullptr;
-	// The captured frame (PHX_FRAME_GRAPH) and what it was captured for.
+	// The captured frame (PHX_FRAME_GRAPH) and what it was captured for. The cameras, jitter and
+	// seeds are read from the per-frame block, so a moving camera replays the same graph.
 	struct GraphKey {
 		const PhxScene* scene;
-		uint64_t sceneVersion;  // bumped when the scene's textures are replaced
-		PhxCameraDef cam;
+		uint64_t sceneVersion;  // unique per upload, renewed when the scene's textures are replaced
 		uint32_t flags;  // PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP
 		uint32_t numLights;
 		const void* lightBuffers;
@@ -69,7 +72,8 @@ void freeRenderer(PhxRenderer* r)
 	if (r->side) hipStreamDestroy(r->side);
 	void* bufs[] = {r->prim,    r->primHits, r->gbuf,   r->bounce,  r->throughput,
 	                r->bounceHits, r->bounceInfo, r->contrib, r->shadow, r->inLight,
-	                r->color,   r->hist[0],  r->hist[1], r->out,    r->params};
+	                r->color,   r->hist[0],  r->hist[1], r->out,    r->params,
+	                r->vel[0],  r->vel[1]};
 	for (void* b : bufs) hipFree(b);
 	delete r;
 }
@@ -114,7 +118,6 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 {
 	const uint32_t L = scene->numLights;
 	const bool taa = (flags & PHX_FRAME_TAA) != 0;
-	const float camPos[3] = {cam->origin[0], cam->origin[1], cam->origin[2]};
 	const uint32_t order = (r->w % 8u == 0u && r->h % 8u == 0u) ? PHX_RAYS_TILED_8X8 : PHX_RAYS_ROW_MAJOR;
 	const SceneView sv = view(scene);
 	const TraceLaunch closest = traceLaunch(ctx, 0, false);
@@ -142,12 +145,11 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->numTris,
 	                        scene->materials, scene->numMaterials, scene->tex, r->gbuf, s),
 	        what);
-	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, camPos, scene->lights, L, r->n, r->params, 1u,
+	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, scene->lights, L, r->n, r->params, 1u,
 	                    r->contrib, r->shadow, s),
 	        what);
 	// Bounce rays (they need only the hit records): incoherent, traced by the persistent kernel.
-	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, camPos, r->params, r->bounce, r->throughput,
-	                           s),
+	PHX_HIP(launchGenSecondary(r->gbuf, r->w, r->h, r->params, r->bounce, r->throughput, s),
 	        what);
 	// The bounce rays' closest hits (the persistent kernel, whose last rays take long) overlap the
 	// primary shadow rays and their shadow logic; the bounce records need the shading buffers back,
@@ -169,8 +171,8 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 		                        scene->materials, scene->numMaterials, scene->tex, r->bounceInfo,
 		                        s),
 		        what);
-		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, camPos, scene->lights, L,
-		                    r->m, r->params, 3u, r->contrib, r->shadow, s),
+		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, scene->lights, L, r->m,
+		                    r->params, 3u, r->contrib, r->shadow, s),
 		        what);
 		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, bounceShadows, s), what);
 		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
@@ -178,8 +180,9 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	}
 
 	if (taa)
-		PHX_HIP(launchTaa(r->color, r->hist[0], r->hist[1], r->gbuf, r->w, r->h,
-		                  !(flags & PHX_FRAME_NO_CLAMP), r->params, r->out, s),
+		PHX_HIP(launchTaa(r->color, r->hist[0], r->hist[1], r->vel[0], r->vel[1], r->gbuf,
+		                  r->prim, r->w, r->h, !(flags & PHX_FRAME_NO_CLAMP), r->params, r->out,
+		                  s),
 		        what);
 	return PHX_OK;
 }
@@ -230,6 +233,8 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 	if (st == PHX_OK) st = devAlloc(r->color, r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->hist[0], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->hist[1], r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->vel[0], r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->vel[1], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->out, r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->params, 1, what);
 	if (st == PHX_OK && (hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) != hipSuccess ||
@@ -266,6 +271,7 @@ phx_status phx_renderer_buffer(PhxRenderer* r, uint32_t which, const void** devP
 	case PHX_BUFFER_GBUFFER: p = r->gbuf; b = size_t(r->n) * 48u; break;
 	case PHX_BUFFER_BOUNCE_RAYS: p = r->bounce; b = size_t(r->m) * sizeof(PhxRayIn); break;
 	case PHX_BUFFER_HISTORY: p = r->hist[r->histIndex]; b = size_t(r->n) * 16u; break;
+	case PHX_BUFFER_VELOCITY: p = r->vel[r->histIndex]; b = size_t(r->n) * 8u; break;
 	default: return fail(PHX_ERR_INVALID_ARGUMENT, "phx_renderer_buffer: unknown buffer %u", which);
 	}
 	*devPtr = p;
@@ -303,12 +309,13 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 	fp.taaReset = (flags & PHX_FRAME_TAA_RESET) || !r->haveHistory ? 1u : 0u;
 	fp.seed = seed;
 	fp.histIn = r->histIndex;
+	fp.cam = *cam;
+	fp.prevCam = r->haveLastCam ? r->lastCam : *cam;
 	PHX_HIP(launchSetFrameParams(r->params, fp, s), what);
 	if (flags & PHX_FRAME_GRAPH) {
 		PhxRenderer::GraphKey k{};
 		k.scene = scene;
 		k.sceneVersion = scene->version;
-		k.cam = *cam;
 		k.flags = flags & (PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP);
 		k.numLights = scene->numLights;
 		k.lightBuffers = r->contrib;
@@ -339,6 +346,8 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 	}
 
 	r->lastOut = r->color;
+	r->lastCam = *cam;
+	r->haveLastCam = true;
 	if (taa) {
 		r->histIndex ^= 1u;
 		r->haveHistory = true;

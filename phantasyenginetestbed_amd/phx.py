@@ -53,7 +53,8 @@ FRAME_TAA = 0x10000
 FRAME_TAA_RESET = 0x20000
 FRAME_NO_CLAMP = 0x40000
 FRAME_GRAPH = 0x80000
-BUFFER_COLOR, BUFFER_OUTPUT, BUFFER_GBUFFER, BUFFER_BOUNCE_RAYS, BUFFER_HISTORY = range(5)
+BUFFER_COLOR, BUFFER_OUTPUT, BUFFER_GBUFFER, BUFFER_BOUNCE_RAYS, BUFFER_HISTORY, BUFFER_VELOCITY = \
+    range(6)
 RAYS_ROW_MAJOR = 0
 RAYS_TILED_8X8 = 1
 

@@ -260,3 +260,45 @@ def test_plane_scene_frames_bit_identical(ctx, kind, w, h):
     bits_equal(gbuf.view(np.float32), ref_gbuf.view(np.float32), "hit records")
     bits_equal(bounce.view(np.float32), ref_bounce.view(np.float32), "bounce rays")
     bits_equal(out, ref_color, "frame colour")
+
+
+def test_taa_with_camera_motion_bit_identical(ctx, small):
+    """A camera that moves and turns from frame to frame: every frame's velocity buffer, TAA output
+    and history equal the oracle's (velocity of each pixel against the previous frame's camera,
+    bilinear reprojection, velocity/roughness trust), replayed from the captured graph as well."""
+    bvh, mats, lts, scene = small
+    w, h = 64, 40
+    ids = O.pixel_ids(w, h)
+    cams = [phx.camera_look_at((0.5 + 0.01 * f, 0.6 - 0.004 * f, -0.3 + 0.006 * f),
+                               (0.5 + 0.02 * f, 0.05, 0.6), (0.0, 1.0, 0.0), math.radians(60), w,
+                               h) for f in range(8)]
+    direct, graph = ctx.renderer(w, h), ctx.renderer(w, h)
+    hist = vel_prev = None
+    try:
+        for f, cam in enumerate(cams):
+            say(f"[gpu-frame] moving camera, frame {f}")
+            flags = phx.FRAME_TAA | (phx.FRAME_TAA_RESET if f == 0 else 0)
+            out = direct.render(scene, cam, f, SEED, flags)
+            vel = direct.download(phx.BUFFER_VELOCITY, np.float32).reshape(h, w, 2)
+            bits_equal(graph.render(scene, cam, f, SEED, flags | phx.FRAME_GRAPH), out,
+                       f"graph replay, frame {f}")
+            jx, jy = phx.halton23(f)
+            color, gbuf, _ = O.frame(bvh.nodes, bvh.tris, bvh.datas, mats, lts, cam.as_array(), w,
+                                     h, jx, jy, SEED, f)
+            prim = O.primary_rays_jittered(cam.as_array(), w, h, jx, jy)
+            prev = cams[max(f - 1, 0)]
+            ref_vel = O.velocity(cam.as_array(), prev.as_array(), gbuf, prim, w, h)
+            bits_equal(vel, ref_vel, f"velocity, frame {f}")
+            if f > 0:
+                assert np.abs(ref_vel).max() > 0.5
+            hist, ref = O.taa(color, hist, gbuf["roughness"][ids], reset=(f == 0), clamp=True,
+                              vel_in=vel_prev, vel=ref_vel)
+            vel_prev = ref_vel
+            bits_equal(out, ref, f"TAA output, frame {f}")
+        bits_equal(direct.download(phx.BUFFER_HISTORY, np.float32).reshape(h, w, 4), hist,
+                   "TAA history")
+        n = hist[..., 3]
+        assert (n >= 1).all() and (n < 8).any() and (n > 2).any()  # some pixels reset, some kept
+    finally:
+        direct.release()
+        graph.release()
