@@ -354,6 +354,19 @@ void phx_scene_file_release(PhxSceneFile* file);
 phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                             uint32_t numRays, PhxRayHit* hits, uint32_t flags);
 
+/* phx_cast_closest on device buffers without allocating anything: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Results equal phx_cast_closest's. `flags` must contain
+ * PHX_MEM_DEVICE and may add PHX_ASYNC; `rays` and `hits` are device pointers. The persistent
+ * launch's ray counters and stack spill live in a static device array of the library (256 B +
+ * 20 MiB per GPU, held until the process ends): the launch uses the context's shape with a
+ * 32-entry LDS stack split where it fits and at most 5,242,880 / ((72 - split) * blockThreads)
+ * blocks. At most 2^31 - 1 rays (PHX_ERR_INVALID_ARGUMENT beyond). Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other contexts or threads, or PHX_ASYNC calls on
+ * different streams) race; serialise them. */
+phx_status phx_cast_closest_noalloc(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                                    uint32_t numRays, PhxRayHit* hits, uint32_t flags);
+
 /* Any hit (shadow rays). Replaces launchShadowRayCastKernel (RayCastKernel.hpp:63-64,
  * RayCastKernel.cu:577-801, 853-886): inLight[i] = 1 iff no triangle has tMin <= t < tMax, where
  * boxes are culled with the castRay box predicate (hit, tOut > tMin, tIn < tMax). One byte/ray. */

@@ -299,6 +299,15 @@ phx_status phx_create(int device, PhxContext** outCtx)
 		delete ctx;
 		return hipFail(e, "phx_create: persistent-kernel state");
 	}
+	// The allocation-free cast's scratch: its address on this device, looked up once.
+	if (e == hipSuccess) e = noAllocScratchInit(device);
+	if (e != hipSuccess) {
+		hipFree(ctx->work);
+		hipFree(ctx->spill);
+		hipStreamDestroy(ctx->stream);
+		delete ctx;
+		return hipFail(e, "phx_create: scratch of phx_cast_closest_noalloc");
+	}
 	registry().add(ctx);
 	// The traversal's reciprocal must equal IEEE division on this device (see phx_check_reciprocal).
 	uint64_t counts[2] = {0, 0};
@@ -702,6 +711,48 @@ phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRay
 			return launchClosest(sv, r, numRays, static_cast<PhxRayHit*>(out), L, nullptr, stream);
 		},
 		"phx_cast_closest");
+}
+
+phx_status phx_cast_closest_noalloc(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                                    uint32_t numRays, PhxRayHit* hits, uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (!(flags & PHX_MEM_DEVICE) || (flags & ~(PHX_MEM_DEVICE | PHX_ASYNC)))
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_noalloc: flags 0x%x (device "
+		            "pointers required; only PHX_MEM_DEVICE and PHX_ASYNC allowed)", flags);
+	if (numRays > 0x7FFFFFFFu)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_noalloc: %u rays (at most "
+		            "2^31 - 1)", numRays);
+	if (numRays == 0) return ok();
+	if (!rays || !hits) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_noalloc: NULL");
+	char* base = static_cast<char*>(noAllocScratch(ctx->device));
+	if (!base) return fail(PHX_ERR_HIP, "phx_cast_closest_noalloc: scratch not set up");
+	// The context's shape, with the largest LDS stack split that fits (the fewest spilled
+	// entries) and the grid capped so its spill columns fit the scratch.
+	TraceLaunch L = traceLaunch(ctx, 0, false);
+	uint32_t bpc = 0;
+	if (persistentBlocksPerCU(L.blockThreads, L.topNodes, L.helperWaves, 32, &bpc) ==
+	        hipSuccess && bpc > 0)
+		L.ldsDepth = 32;
+	const uint32_t perLane = kPersistentStackEntries - L.ldsDepth;
+	const uint32_t maxGrid = kNoAllocSpillWords / (perLane * L.blockThreads);
+	if (maxGrid == 0)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_noalloc: a block of %u threads "
+		            "needs more stack spill than the %u-word scratch holds", L.blockThreads,
+		            kNoAllocSpillWords);
+	L.gridBlocks = std::min(L.gridBlocks, maxGrid);
+	L.spillLanes = L.gridBlocks * L.blockThreads;
+	L.work = reinterpret_cast<uint32_t*>(base);
+	L.spill = reinterpret_cast<uint32_t*>(base + kNoAllocCounterBytes);
+	PHX_HIP(launchZeroWords(L.work, 2u, ctx->stream), "phx_cast_closest_noalloc");
+	PHX_HIP(launchClosest(view(scene), rays, numRays, hits, L, nullptr, ctx->stream),
+	        "phx_cast_closest_noalloc");
+	if (!(flags & PHX_ASYNC))
+		PHX_HIP(hipStreamSynchronize(ctx->stream), "phx_cast_closest_noalloc");
+	return ok();
 }
 
 phx_status phx_cast_anyhit(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,

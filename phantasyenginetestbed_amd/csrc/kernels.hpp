@@ -57,6 +57,21 @@ struct TraceLaunch {
 
 // Words of the stack spill area for `lanes` resident lanes.
 uint32_t persistentSpillWords(uint32_t lanes);
+// Stack entries per lane of the persistent kernels (LDS part + spill part).
+constexpr uint32_t kPersistentStackEntries = 72;
+
+// Scratch of the allocation-free closest-hit cast: a static device array per GPU holding the
+// persistent launch's ray counters (first kNoAllocCounterBytes) and its stack spill area
+// (kNoAllocSpillWords words: 40 spilled entries for 131,072 lanes, 20 MiB).
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr uint32_t kNoAllocSpillWords = 40u * 131072u;
+constexpr int kNoAllocMaxDevices = 64;
+// Looks up the scratch array's address on `device` (which must be current) once; phx_create
+// calls it. noAllocScratch returns the address found (nullptr before).
+hipError_t noAllocScratchInit(int device);
+void* noAllocScratch(int device);
+// Zeroes n <= 64 words with a one-block kernel (graph-capturable, allocates nothing).
+hipError_t launchZeroWords(uint32_t* p, uint32_t n, hipStream_t stream);
 // Dynamic LDS bytes of a persistent block.
 uint32_t persistentLdsBytes(uint32_t blockThreads, uint32_t topNodes, uint32_t ldsDepth);
 // True if `depth` LDS stack entries per lane is a supported split (4..32).

@@ -13,6 +13,8 @@
 // ("next") and the rest of castRay's stack in LDS, so the order -- and every hit -- is unchanged.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.hpp"
 #include "trace_device.hpp"
 
@@ -1548,6 +1550,54 @@ void persistentShape(const TraceLaunch& L, const SceneView& s, uint32_t n, uint3
 }
 
 }  // namespace
+
+// Device scratch of the allocation-free closest-hit cast (phx_cast_closest_noalloc): the ray
+// counter pair of its persistent launch in the first kNoAllocCounterBytes, then the stack spill
+// area, kNoAllocSpillWords words. A file-scope device array: every GPU the library runs on holds
+// its own copy from the module load until the process ends. Host code must not pass `scratch`
+// itself to a kernel (on the host it is only a stand-in for the symbol); it uses the address
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+static __device__ __align__(256) uint8_t
+	scratch[kNoAllocCounterBytes + size_t(kNoAllocSpillWords) * 4u];
+static_assert(kPersistentStack == int(kPersistentStackEntries), "stack entries per lane");
+
+namespace {
+
+std::mutex gScratchMu;
+void* gScratchPtr[kNoAllocMaxDevices] = {};
+
+// Zeroes `n` words (n <= 64); a kernel rather than a memset, so a caller may capture it in a graph.
+__global__ void k_zero_words(uint32_t* p, uint32_t n)
+{
+	if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+hipError_t noAllocScratchInit(int device)
+{
+	if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> g(gScratchMu);
+	if (gScratchPtr[device]) return hipSuccess;
+	void* ptr = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+	if (e != hipSuccess) return e;
+	gScratchPtr[device] = ptr;
+	return hipSuccess;
+}
+
+void* noAllocScratch(int device)
+{
+	if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+	std::lock_guard<std::mutex> g(gScratchMu);
+	return gScratchPtr[device];
+}
+
+hipError_t launchZeroWords(uint32_t* p, uint32_t n, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, stream, p, n);
+	return hipGetLastError();
+}
 
 uint32_t persistentSpillWords(uint32_t lanes)
 {

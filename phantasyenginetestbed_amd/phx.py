@@ -175,6 +175,7 @@ _SIGNATURES = {
     "phx_scene_set_textures": (C.c_int, [_P, _P, _P, _U32]),
     "phx_interpret_hits": (C.c_int, [_P, _P, _P, _P, _U32, _P, _U32]),
     "phx_cast_closest": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
+    "phx_cast_closest_noalloc": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_cast_anyhit": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_traversal_stats": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(C.c_uint64)]),
     "phx_gen_bounce_rays": (C.c_int, [_P, _P, _P, _P, _U32, C.c_uint64, _U32, _P, _U32]),
@@ -686,6 +687,12 @@ class Context:
                             mode: int = TRAVERSAL_REFERENCE_ORDER, asynchronous: bool = False):
         flags = MEM_DEVICE | mode | (ASYNC if asynchronous else 0)
         check(lib().phx_cast_closest(self.handle, scene.handle, rays_ptr, n, hits_ptr, flags))
+
+    def cast_closest_noalloc(self, scene: Scene, rays_ptr: int, n: int, hits_ptr: int,
+                             asynchronous: bool = False) -> None:
+        """phx_cast_closest_noalloc: device buffers, no allocation in the call."""
+        check(lib().phx_cast_closest_noalloc(self.handle, scene.handle, rays_ptr, n, hits_ptr,
+                                             MEM_DEVICE | (ASYNC if asynchronous else 0)))
 
     def cast_anyhit(self, scene: Scene, rays: np.ndarray, mode: int = 0) -> np.ndarray:
         rays = np.ascontiguousarray(rays, RAY)

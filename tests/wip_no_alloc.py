@@ -1,0 +1,102 @@
+"""phx_cast_closest_noalloc (the closest-hit cast that allocates nothing, its persistent launch's
+counters and stack spill in the library's static device scratch) against phx_cast_closest on the
+same device buffers, through the C ABI: two calls in a row, the second with a smaller input, on a
+random-triangle scene and on a 60-level tree whose stack spills past the LDS part."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from phantasyenginetestbed_amd import phx
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = phx.Context(0)
+    yield c
+    c.destroy()
+
+
+def random_rays(n, seed, lo=-0.2, hi=1.2):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, phx.RAY)
+    rays["o"] = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmin"], rays["tmax"] = np.float32(1e-4), np.float32(phx.FLT_MAX)
+    return rays
+
+
+def deep_scene():
+    from test_gpu_parity import caterpillar
+    return caterpillar(depth=60, seed=5)
+
+
+@pytest.mark.parametrize("which", ["random100k", "deep60"])
+def test_noalloc_matches_cast_closest(ctx, which):
+    if which == "random100k":
+        t, d = phx.random_triangles(100000, 0xC0FFEE01)
+        b = phx.BVH(t, d)
+        nodes, tris = b.nodes, b.tris
+        sizes = (400000, 7000)
+    else:
+        nodes, tris = deep_scene()
+        sizes = (200000, 3001)
+    s = ctx.upload_scene(nodes, tris)
+    try:
+        for call, n in enumerate(sizes):
+            say(f"[no-alloc] {which}: call {call}, {n} rays")
+            rays = random_rays(n, seed=call + 1)
+            rb = ctx.alloc(n * 32).upload(rays)
+            want_b, got_b = ctx.alloc(n * 16), ctx.alloc(n * 16)
+            try:
+                got_b.upload(np.full(n * 16, 0xAB, np.uint8))
+                ctx.cast_closest_device(s, rb.ptr, n, want_b.ptr)
+                ctx.cast_closest_noalloc(s, rb.ptr, n, got_b.ptr)
+                want = want_b.download(phx.HIT, n)
+                got = got_b.download(phx.HIT, n)
+                assert got.tobytes() == want.tobytes(), f"{which}: call {call}"
+                sample = np.ascontiguousarray(rays[::97])
+                assert got[::97].tobytes() == O.cast_closest(nodes, tris, sample).tobytes()
+                # asynchronous form plus an explicit synchronisation: the same hits
+                got_b.upload(np.full(n * 16, 0xAB, np.uint8))
+                ctx.cast_closest_noalloc(s, rb.ptr, n, got_b.ptr, asynchronous=True)
+                ctx.synchronize()
+                assert got_b.download(phx.HIT, n).tobytes() == want.tobytes()
+            finally:
+                rb.free()
+                want_b.free()
+                got_b.free()
+    finally:
+        s.release()
+
+
+def test_noalloc_rejects_what_it_cannot_do_without_allocating(ctx):
+    t, d = phx.random_triangles(2000, 7)
+    b = phx.BVH(t, d)
+    s = ctx.upload_scene(b.nodes, b.tris)
+    rays = random_rays(64, 3)
+    hits = np.zeros(64, phx.HIT)
+    L = phx.lib()
+    try:
+        # host buffers would need staging copies: refused
+        assert L.phx_cast_closest_noalloc(ctx.handle, s.handle, rays.ctypes.data, 64,
+                                          hits.ctypes.data, 0) == phx.ERR_INVALID_ARGUMENT
+        dev = ctx.alloc(256)
+        try:
+            assert L.phx_cast_closest_noalloc(ctx.handle, s.handle, dev.ptr, 0x80000000, dev.ptr,
+                                              phx.MEM_DEVICE) == phx.ERR_INVALID_ARGUMENT
+            assert L.phx_cast_closest_noalloc(ctx.handle, s.handle, dev.ptr, 0, dev.ptr,
+                                              phx.MEM_DEVICE) == phx.OK
+            assert L.phx_cast_closest_noalloc(ctx.handle, s.handle, dev.ptr, 4, dev.ptr,
+                                              phx.MEM_DEVICE | phx.KERNEL_PACKET) == \
+                phx.ERR_INVALID_ARGUMENT
+        finally:
+            dev.free()
+    finally:
+        s.release()
