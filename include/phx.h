@@ -381,6 +381,51 @@ phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const Phx
                                uint32_t numRays, uint32_t flags, uint64_t counters[5]);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Dynamic meshes: instances of uploaded meshes under a top tree (the reference's DynamicBVH /   */
+/* OuterBVH, ray_tracer_common/DynamicBVH.hpp:13-26, DynamicBVH.cpp:16-329, for DynObject       */
+/* {meshIndex, transform}, level/DynObject.hpp; its CUDA backend leaves them untraced,          */
+/* CudaTracerRenderer.cpp:317-327, :412-415)                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One placed mesh: `mesh` is an uploaded scene holding the mesh's object-space SBVH (and its
+ * TriangleData for hit records); objectToWorld is the affine transform, rows (R | t):
+ * world = R * p + t. */
+typedef struct PhxInstance {
+	const PhxScene* mesh;
+	float objectToWorld[12];
+} PhxInstance;
+
+typedef struct PhxInstanceSet PhxInstanceSet;
+
+/* Builds the two-level structure of `count` instances (0 allowed): world -> object transforms
+ * (the inverse in double, rounded once to float), world boxes (the mesh root box's corners mapped
+ * in double, widened by 1e-5 of the extent + 1e-6 of the magnitude, rounded outwards) and a top
+ * tree with one instance per leaf. Rebuild it when instances move (the meshes' SBVHs stay). The
+ * set holds its meshes until phx_instances_release. PHX_ERR_INVALID_ARGUMENT for a singular or
+ * non-finite transform. */
+phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, uint32_t count,
+                               PhxInstanceSet** outSet);
+void phx_instances_release(PhxInstanceSet* set);
+
+/* Closest hit against the instances (and, if `scene` is not NULL, a static scene). An instance is
+ * traced iff the ray enters its world box (castRay's box test with tIn <= min(tMax, best t)); it
+ * runs castRay on its mesh with the ray in object space (o' = A o + b, d' = A d, the same tMin and
+ * tMax, so t is the same parameter), from a miss. The closest result wins; on equal t the static
+ * scene, then the lowest instance index. hits[i].triangleIndex is the triangle in the winning
+ * mesh (or the static scene); instanceIds (may be NULL) receives the instance index, 0xFFFFFFFF
+ * for the static scene or a miss. `flags` = PHX_MEM_DEVICE | PHX_ASYNC. */
+phx_status phx_cast_closest_instances(PhxContext* ctx, const PhxScene* scene,
+                                      const PhxInstanceSet* set, const PhxRayIn* rays,
+                                      uint32_t numRays, PhxRayHit* hits, uint32_t* instanceIds,
+                                      uint32_t flags);
+
+/* Any hit against the instances (and `scene` if not NULL): inLight[i] = 0 iff some triangle of the
+ * static scene or of an instance whose world box the ray enters has tMin <= t < tMax. */
+phx_status phx_cast_anyhit_instances(PhxContext* ctx, const PhxScene* scene,
+                                     const PhxInstanceSet* set, const PhxRayIn* rays,
+                                     uint32_t numRays, uint8_t* inLight, uint32_t flags);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Batched form: many ray batches in one persistent fused launch                               */
 /* ------------------------------------------------------------------------------------------ */
 

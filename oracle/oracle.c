@@ -369,3 +369,150 @@ int oracle_aabb(const float o[3], const float d[3], const float mn[3], const flo
 	make_ray(&in, &r);
 	return aabb(&r, mn, mx, tIn, tOut);
 }
+
+/* ---- instances: the two-level path (phx_instances_build, phx_cast_*_instances) ----
+ *
+ * The build's definition, restated: an instance is traced iff the ray enters its world box
+ * (castRay's box test with tIn <= min(tMax, best t)); it runs castRay on its mesh with the ray in
+ * object space (o' = A o + b, d' = A d, the same tMin/tMax), from a miss; the closest result wins,
+ * on equal t the static scene, then the lowest instance index. Instances are taken here in index
+ * order, one after the other (the device walks a top tree instead; the box test never drops an
+ * instance that could win or tie, so the order does not change the result). */
+
+/* World -> object rows (A | b) of the object -> world rows (R | t): the adjugate of R over its
+ * determinant, b = -A t, in double, each value rounded once to float. */
+static int invert_affine(const float m[12], float out[12])
+{
+	double r[3][3], t[3];
+	for (int i = 0; i < 3; i++) {
+		for (int j = 0; j < 3; j++) r[i][j] = m[4 * i + j];
+		t[i] = m[4 * i + 3];
+	}
+	const double c00 = r[1][1] * r[2][2] - r[1][2] * r[2][1];
+	const double c01 = r[1][2] * r[2][0] - r[1][0] * r[2][2];
+	const double c02 = r[1][0] * r[2][1] - r[1][1] * r[2][0];
+	const double det = (r[0][0] * c00 + r[0][1] * c01) + r[0][2] * c02;
+	if (!(fabs(det) > 0.0) || !isfinite(det)) return 0;
+	double a[3][3];
+	a[0][0] = c00 / det;
+	a[1][0] = c01 / det;
+	a[2][0] = c02 / det;
+	a[0][1] = (r[0][2] * r[2][1] - r[0][1] * r[2][2]) / det;
+	a[1][1] = (r[0][0] * r[2][2] - r[0][2] * r[2][0]) / det;
+	a[2][1] = (r[0][1] * r[2][0] - r[0][0] * r[2][1]) / det;
+	a[0][2] = (r[0][1] * r[1][2] - r[0][2] * r[1][1]) / det;
+	a[1][2] = (r[0][2] * r[1][0] - r[0][0] * r[1][2]) / det;
+	a[2][2] = (r[0][0] * r[1][1] - r[0][1] * r[1][0]) / det;
+	for (int i = 0; i < 3; i++) {
+		const double b = -((a[i][0] * t[0] + a[i][1] * t[1]) + a[i][2] * t[2]);
+		for (int j = 0; j < 3; j++) out[4 * i + j] = (float)a[i][j];
+		out[4 * i + 3] = (float)b;
+	}
+	for (int k = 0; k < 12; k++)
+		if (!isfinite(out[k])) return 0;
+	return 1;
+}
+
+/* The mesh root box (union of node 0's child boxes) mapped by (R | t): the 8 corners in double,
+ * bounds widened by 1e-5 of the largest extent + 1e-6 of the largest magnitude, rounded outwards
+ * to float. */
+static int world_box(const float m[12], const OrNode* root, float wmn[3], float wmx[3])
+{
+	float mn[3], mx[3];
+	for (int k = 0; k < 3; k++) {
+		mn[k] = min_first(root->f[k], root->f[6 + k]);
+		mx[k] = max_first(root->f[3 + k], root->f[9 + k]);
+	}
+	double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+	for (int c = 0; c < 8; c++) {
+		const double p[3] = {(c & 1) ? mx[0] : mn[0], (c & 2) ? mx[1] : mn[1],
+		                     (c & 4) ? mx[2] : mn[2]};
+		for (int i = 0; i < 3; i++) {
+			const double w = (((double)m[4 * i] * p[0] + (double)m[4 * i + 1] * p[1]) +
+			                  (double)m[4 * i + 2] * p[2]) + (double)m[4 * i + 3];
+			lo[i] = (w < lo[i]) ? w : lo[i];
+			hi[i] = (hi[i] < w) ? w : hi[i];
+		}
+	}
+	double ext = 0.0, mag = 0.0;
+	for (int i = 0; i < 3; i++) {
+		const double e = hi[i] - lo[i];
+		ext = (ext < e) ? e : ext;
+		const double a = fabs(lo[i]), b = fabs(hi[i]);
+		const double g = (a < b) ? b : a;
+		mag = (mag < g) ? g : mag;
+	}
+	const double pad = 1e-5 * ext + 1e-6 * mag;
+	for (int i = 0; i < 3; i++) {
+		const float a = (float)(lo[i] - pad), b = (float)(hi[i] + pad);
+		wmn[i] = (double)a > lo[i] - pad ? nextafterf(a, -HUGE_VALF) : a;
+		wmx[i] = (double)b < hi[i] + pad ? nextafterf(b, HUGE_VALF) : b;
+		if (!isfinite(wmn[i]) || !isfinite(wmx[i])) return 0;
+	}
+	return 1;
+}
+
+int oracle_instance_prepare(const OrInstance* inst, float worldToObject[12], float wmn[3],
+                            float wmx[3])
+{
+	return invert_affine(inst->objectToWorld, worldToObject) &&
+	       world_box(inst->objectToWorld, &inst->nodes[0], wmn, wmx);
+}
+
+void oracle_cast_instances(const OrNode* sceneNodes, const OrTri* sceneTris,
+                           const OrInstance* inst, uint32_t numInst, const OrRay* rays, uint32_t n,
+                           OrHit* hits, uint32_t* ids, uint8_t* inLight)
+{
+	float* a = (float*)malloc(sizeof(float) * 12 * (numInst ? numInst : 1));
+	float* box = (float*)malloc(sizeof(float) * 6 * (numInst ? numInst : 1));
+	for (uint32_t i = 0; i < numInst; i++)
+		oracle_instance_prepare(&inst[i], &a[12 * i], &box[6 * i], &box[6 * i + 3]);
+	for (uint32_t r = 0; r < n; r++) {
+		OrHit best = {~0u, FLT_MAX, FLT_MAX, FLT_MAX};
+		uint32_t bestId = ~0u;
+		int occluded = 0;
+		if (sceneNodes) {
+			if (inLight)
+				occluded = anyhit_one(sceneNodes, sceneTris, &rays[r]);
+			else
+				cast_one(sceneNodes, sceneTris, &rays[r], &best, 0);
+		}
+		Ray w;
+		make_ray(&rays[r], &w);
+		for (uint32_t i = 0; i < numInst && !occluded; i++) {
+			float tIn, tOut;
+			const int hit = aabb(&w, &box[6 * i], &box[6 * i + 3], &tIn, &tOut);
+			const float lim = inLight ? w.tMax : min_first(w.tMax, best.t);
+			if (!(hit && tOut > w.tMin && (inLight ? tIn < lim : tIn <= lim))) continue;
+			const float* A = &a[12 * i];
+			OrRay o;
+			for (int k = 0; k < 3; k++) {
+				float p = A[4 * k] * w.o[0] + A[4 * k + 1] * w.o[1];
+				p = p + A[4 * k + 2] * w.o[2];
+				o.o[k] = p + A[4 * k + 3];
+				float q = A[4 * k] * w.d[0] + A[4 * k + 1] * w.d[1];
+				o.d[k] = q + A[4 * k + 2] * w.d[2];
+			}
+			o.tMin = w.tMin;
+			o.tMax = w.tMax;
+			if (inLight) {
+				occluded = anyhit_one(inst[i].nodes, inst[i].tris, &o);
+			} else {
+				OrHit c;
+				cast_one(inst[i].nodes, inst[i].tris, &o, &c, 0);
+				if (c.tri != ~0u && c.t < best.t) {
+					best = c;
+					bestId = i;
+				}
+			}
+		}
+		if (inLight) {
+			inLight[r] = (uint8_t)!occluded;
+		} else {
+			hits[r] = best;
+			if (ids) ids[r] = bestId;
+		}
+	}
+	free(a);
+	free(box);
+}

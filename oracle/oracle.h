@@ -94,6 +94,24 @@ int oracle_triangle(const float p0[3], const float p1[3], const float p2[3], con
 int oracle_aabb(const float o[3], const float d[3], const float mn[3], const float mx[3],
                 float* tIn, float* tOut);
 
+/* ---- instances (two-level scenes) --------------------------------------------------------- */
+
+/* A placed mesh: its SBVH arrays and the object -> world rows (R | t). */
+typedef struct OrInstance {
+	const OrNode* nodes;
+	const OrTri* tris;
+	float objectToWorld[12];
+} OrInstance;
+
+/* World -> object rows and world box of one instance (as phx_instances_build defines them). */
+int oracle_instance_prepare(const OrInstance* inst, float worldToObject[12], float wmn[3],
+                            float wmx[3]);
+/* Closest hit (inLight NULL: hits and, if not NULL, ids written) or any hit (inLight) against a
+ * static scene (sceneNodes may be NULL) and the instances, taken in index order. */
+void oracle_cast_instances(const OrNode* sceneNodes, const OrTri* sceneTris,
+                           const OrInstance* inst, uint32_t numInst, const OrRay* rays, uint32_t n,
+                           OrHit* hits, uint32_t* ids, uint8_t* inLight);
+
 /* ---- wavefront frame (frame.c) ---------------------------------------------------------- */
 
 /* TriangleData (Triangle.hpp:25-37), Material (Material.hpp:17-55), SphereLight

@@ -545,6 +545,13 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 	s->numLights = numLights;
 	s->depth = depth;
 	s->finiteBounds = 1;
+	for (int k = 0; k < 3; k++) {  // root box: union of node 0's child boxes (fData layout)
+		const float* f = nodes[0].fData;
+		const float lmn[3] = {f[0], f[1], f[2]}, lmx[3] = {f[3], f[4], f[5]};
+		const float rmn[3] = {f[6], f[7], f[8]}, rmx[3] = {f[9], f[10], f[11]};
+		s->rootMin[k] = std::min(lmn[k], rmn[k]);
+		s->rootMax[k] = std::max(lmx[k], rmx[k]);
+	}
 	for (uint32_t i = 0; i < numNodes && s->finiteBounds; i++)
 		for (int k = 0; k < 12; k++)
 			if (!std::isfinite(nodes[i].fData[k])) {

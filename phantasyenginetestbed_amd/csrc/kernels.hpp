@@ -81,6 +81,25 @@ bool validLdsDepth(uint32_t depth);
 hipError_t persistentBlocksPerCU(uint32_t blockThreads, uint32_t topNodes, uint32_t helperWaves,
                                  uint32_t ldsDepth, uint32_t* blocks);
 
+// One instance of a two-level scene on the device: its SBVH (the mesh scene's node and
+// pre-transformed triangle arrays) and the world-to-object affine transform, rows (A | b).
+struct DevInstance {
+	const float4* nodes;
+	const float4* tris;
+	const PhxTriangleData* datas;  // for hit records (may be null)
+	float worldToObject[12];
+	uint32_t index;    // the instance's position in the caller's list (ties: lower wins)
+	uint32_t numTris;  // triangles of the mesh (hit records check indices against it)
+};
+
+// Closest hit (inLight == nullptr; hits and optionally instOut written) or any hit (inLight) of
+// every ray against the instances under the top tree `top` (one instance per leaf; `inst` in
+// top-tree leaf order). `seeded`: hits / inLight hold the static scene's results on entry, which
+// win ties (closest) or already occlude (any hit).
+hipError_t launchInstances(const PhxBVHNode* top, const DevInstance* inst, uint32_t numInst,
+                           const PhxRayIn* rays, uint32_t n, PhxRayHit* hits, uint32_t* instOut,
+                           uint8_t* inLight, bool seeded, hipStream_t stream);
+
 // Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
 // counters of the batch are added to counters[0..4] (device memory).
 // Counts the floats a (exhaustive: all of 2^-126 <= |a| < 2^126; else every significand at four

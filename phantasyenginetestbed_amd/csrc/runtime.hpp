@@ -107,6 +107,7 @@ struct PhxScene {
 	uint32_t numNodes = 0, numTris = 0, numMaterials = 0, numLights = 0;
 	uint32_t depth = 0;
 	uint32_t finiteBounds = 0;
+	float rootMin[3] = {0, 0, 0}, rootMax[3] = {0, 0, 0};  // union of the root's child boxes
 	uint32_t topCount = 0;
 	size_t bytes = 0;
 	void* texArena = nullptr;  // material textures (phx_scene_set_textures)
@@ -139,6 +140,13 @@ inline phx_status checkScene(PhxContext* ctx, const PhxScene* scene)
 	if (scene->ctx != ctx) return fail(PHX_ERR_INVALID_HANDLE, "scene belongs to another context");
 	return PHX_OK;
 }
+
+// Views of an instance set (instances.cpp) for the renderer.
+const PhxBVHNode* instanceTop(const PhxInstanceSet* s);
+const DevInstance* instanceArray(const PhxInstanceSet* s);
+uint32_t instanceCount(const PhxInstanceSet* s);
+uint64_t instanceStamp(const PhxInstanceSet* s);
+bool instanceSetValid(PhxContext* ctx, const PhxInstanceSet* s);
 
 // A new value of the process-wide stamp counter (PhxScene::version).
 uint64_t nextSceneStamp();

@@ -118,20 +118,17 @@ __device__ __forceinline__ void add_counts(unsigned long long* counters, const C
 // level plus the two children of the node being expanded.
 constexpr int kSimpleStack = 72;
 
+// castRay<144> of one ray (BVHTraversal.hpp:37-128) into `c` (which holds the miss result on
+// entry): pop, decide both children, left first; a visited leaf is tested at once, a visited
+// inner node pushed.
 template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_closest_simple(
-	const float4* __restrict__ nodes, const float4* __restrict__ tris,
-	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
-	unsigned long long* __restrict__ counters)
+__device__ __forceinline__ void closest_one(const float4* __restrict__ nodes,
+                                            const float4* __restrict__ tris, const dev::Ray& r,
+                                            Closest& c, Counts& k)
 {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
 	uint32_t stack[kSimpleStack];
 	stack[0] = 0u;
 	int sp = 1;
-	Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
-	Counts k;
 	while (sp > 0) {
 		sp--;
 		const NodeWords nd = load_node(nodes, stack[sp]);
@@ -159,18 +156,13 @@ __global__ __launch_bounds__(kBlock) void k_closest_simple(
 			}
 		}
 	}
-	hits[i] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
-	if (kCount) add_counts(counters, k);
 }
 
-__global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restrict__ nodes,
-                                                          const float4* __restrict__ tris,
-                                                          const float4* __restrict__ rays,
-                                                          uint8_t* __restrict__ inLight, uint32_t n)
+// Any hit of one ray: castRay's traversal with tCurrMax = tMax, stopping at the first triangle
+// with tMin <= t < tMax.
+__device__ __forceinline__ bool anyhit_one(const float4* __restrict__ nodes,
+                                           const float4* __restrict__ tris, const dev::Ray& r)
 {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
 	uint32_t stack[kSimpleStack];
 	stack[0] = 0u;
 	int sp = 1;
@@ -198,7 +190,129 @@ __global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restri
 			}
 		}
 	}
-	inLight[i] = occluded ? 0 : 1;
+	return occluded;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_closest_simple(
+	const float4* __restrict__ nodes, const float4* __restrict__ tris,
+	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
+	unsigned long long* __restrict__ counters)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
+	Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+	Counts k;
+	closest_one<kCount>(nodes, tris, r, c, k);
+	hits[i] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
+	if (kCount) add_counts(counters, k);
+}
+
+__global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restrict__ nodes,
+                                                          const float4* __restrict__ tris,
+                                                          const float4* __restrict__ rays,
+                                                          uint8_t* __restrict__ inLight, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
+	inLight[i] = anyhit_one(nodes, tris, r) ? 0 : 1;
+}
+
+// ---- instances (two-level traversal) ------------------------------------------------------------
+//
+// A top tree over the instances' world boxes (one instance per leaf, PhxBVHNode layout) is walked
+// in castRay's order. An instance is entered iff the ray's slab test on its world box hits with
+// tOut > tMin and tIn <= min(tMax, best t) -- `<=`, so an instance that could tie the best hit is
+// still traced. Each entered instance runs castRay on its own SBVH with the ray moved into object
+// space (o' = A o + b, d' = A d, the same t range; t is the same parameter in both spaces), from a
+// miss, so its result is the one castRay gives for that instance alone. The closest of those
+// results wins, the lower instance index on equal t. With `hits` holding a static-scene result on
+// entry (seeded), that result counts as instance -1: an instance must be strictly closer.
+
+__device__ __forceinline__ dev::Ray object_ray(const DevInstance& in, const dev::Ray& w)
+{
+	const float* A = in.worldToObject;
+	float o[3], d[3];
+	for (int k = 0; k < 3; k++) {
+		float a = A[4 * k] * w.ox + A[4 * k + 1] * w.oy;
+		a = a + A[4 * k + 2] * w.oz;
+		o[k] = a + A[4 * k + 3];
+		float b = A[4 * k] * w.dx + A[4 * k + 1] * w.dy;
+		d[k] = b + A[4 * k + 2] * w.dz;
+	}
+	return dev::make_ray(make_float4(o[0], o[1], o[2], w.tMin), make_float4(d[0], d[1], d[2], w.tMax));
+}
+
+template <bool kAnyHit>
+__global__ __launch_bounds__(kBlock) void k_trace_instances(
+	const float4* __restrict__ top, const DevInstance* __restrict__ inst, uint32_t numInst,
+	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t* __restrict__ instOut,
+	uint8_t* __restrict__ inLight, uint32_t n, uint32_t seeded)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
+	Closest best = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+	uint32_t bestInst = kInvalid;
+	bool seededHit = false;
+	if (!kAnyHit && seeded) {
+		const uint4 h = hits[i];
+		best = Closest{h.x, __uint_as_float(h.y), __uint_as_float(h.z), __uint_as_float(h.w)};
+		seededHit = h.x != kInvalid;
+	}
+	bool occluded = false;
+	if (kAnyHit && seeded) occluded = inLight[i] == 0;
+	uint32_t stack[kSimpleStack];
+	int sp = numInst ? 1 : 0;
+	stack[0] = 0u;
+	while (sp > 0 && !occluded) {
+		sp--;
+		const NodeWords nd = load_node(top, stack[sp]);
+		float inL, outL, inR, outR;
+		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
+		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
+		const float lim = kAnyHit ? r.tMax : dev::min_first(r.tMax, best.t);
+		const bool visit[2] = {(outL >= inL) && outL > r.tMin && (kAnyHit ? inL < lim : inL <= lim),
+		                       (outR >= inR) && outR > r.tMin && (kAnyHit ? inR < lim : inR <= lim)};
+		const int32_t raw[2] = {nd.w3.x, nd.w3.y}, cnt[2] = {nd.w3.z, nd.w3.w};
+		for (int side = 0; side < 2 && !occluded; side++) {
+			if (!visit[side]) continue;
+			if (cnt[side] == 0) {
+				if (sp < kSimpleStack) stack[sp++] = child_index(raw[side]);
+				continue;
+			}
+			const uint32_t first = child_index(raw[side]);
+			for (int32_t j = 0; j < cnt[side] && !occluded; j++) {
+				const DevInstance& in = inst[first + uint32_t(j)];
+				const uint32_t id = in.index;
+				const dev::Ray o = object_ray(in, r);
+				if (kAnyHit) {
+					occluded = anyhit_one(in.nodes, in.tris, o);
+				} else {
+					Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+					Counts none;
+					closest_one<false>(in.nodes, in.tris, o, c, none);
+					const bool tieWins = c.t == best.t && !seededHit &&
+					                     (bestInst == kInvalid || id < bestInst);
+					if (c.tri != kInvalid && (c.t < best.t || tieWins)) {
+						best = c;
+						bestInst = id;
+						seededHit = false;
+					}
+				}
+			}
+		}
+	}
+	if (kAnyHit) {
+		inLight[i] = occluded ? 0 : 1;
+	} else {
+		if (bestInst != kInvalid || !seeded)
+			hits[i] = make_uint4(best.tri, __float_as_uint(best.t), __float_as_uint(best.u),
+			                     __float_as_uint(best.v));
+		if (instOut) instOut[i] = bestInst;
+	}
 }
 
 // ---- persistent wavefronts ----------------------------------------------------------------------
@@ -1668,6 +1782,24 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, nodes, tris, r4, h4, n,
 	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, shape.helperWaves,
 	                   L.ldsDepth, L.window, counters);
+	return hipGetLastError();
+}
+
+hipError_t launchInstances(const PhxBVHNode* top, const DevInstance* inst, uint32_t numInst,
+                          const PhxRayIn* rays, uint32_t n, PhxRayHit* hits, uint32_t* instOut,
+                          uint8_t* inLight, bool seeded, hipStream_t stream)
+{
+	if (n == 0) return hipSuccess;
+	const float4* t4 = reinterpret_cast<const float4*>(top);
+	const float4* r4 = reinterpret_cast<const float4*>(rays);
+	if (inLight)
+		hipLaunchKernelGGL(k_trace_instances<true>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0,
+		                   stream, t4, inst, numInst, r4, nullptr, nullptr, inLight, n,
+		                   uint32_t(seeded));
+	else
+		hipLaunchKernelGGL(k_trace_instances<false>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0,
+		                   stream, t4, inst, numInst, r4, reinterpret_cast<uint4*>(hits), instOut,
+		                   nullptr, n, uint32_t(seeded));
 	return hipGetLastError();
 }
 
