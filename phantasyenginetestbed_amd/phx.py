@@ -176,6 +176,10 @@ _SIGNATURES = {
     "phx_interpret_hits": (C.c_int, [_P, _P, _P, _P, _U32, _P, _U32]),
     "phx_cast_closest": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_cast_closest_noalloc": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
+    "phx_instances_build": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
+    "phx_instances_release": (None, [_P]),
+    "phx_cast_closest_instances": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P, _U32]),
+    "phx_cast_anyhit_instances": (C.c_int, [_P, _P, _P, _P, _U32, _P, _U32]),
     "phx_cast_anyhit": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_traversal_stats": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(C.c_uint64)]),
     "phx_gen_bounce_rays": (C.c_int, [_P, _P, _P, _P, _U32, C.c_uint64, _U32, _P, _U32]),
@@ -451,6 +455,54 @@ class Scene:
     def release(self) -> None:
         if self.handle:
             lib().phx_scene_release(self.handle)
+            self.handle = None
+
+
+class Instance(C.Structure):
+    """PhxInstance: an uploaded mesh scene and its object -> world rows (R | t)."""
+    _fields_ = [("mesh", C.c_void_p), ("objectToWorld", C.c_float * 12)]
+
+
+class InstanceSet:
+    """Instances of uploaded meshes under a top tree (phx_instances_build).
+    `instances` = [(Scene, 3x4 object -> world matrix)]."""
+
+    def __init__(self, ctx: "Context", instances):
+        self.ctx = ctx
+        arr = (Instance * max(1, len(instances)))()
+        for i, (scene, m) in enumerate(instances):
+            arr[i] = Instance(scene.handle, (C.c_float * 12)(*np.asarray(m, np.float32).ravel()))
+        h = C.c_void_p()
+        check(lib().phx_instances_build(ctx.handle, arr, len(instances), C.byref(h)))
+        self.handle = h.value
+        self.count = len(instances)
+
+    def cast_closest(self, rays: np.ndarray, scene: "Scene | None" = None):
+        """(hits, instance ids) of host rays; ids are 0xFFFFFFFF for the static scene or a miss."""
+        rays = np.ascontiguousarray(rays, RAY)
+        hits = np.empty(len(rays), HIT)
+        ids = np.empty(len(rays), np.uint32)
+        check(lib().phx_cast_closest_instances(self.ctx.handle, scene.handle if scene else None,
+                                               self.handle, _ptr(rays), len(rays), _ptr(hits),
+                                               _ptr(ids), 0))
+        return hits, ids
+
+    def cast_closest_device(self, rays_ptr: int, n: int, hits_ptr: int, ids_ptr: int | None,
+                            scene: "Scene | None" = None, asynchronous: bool = False) -> None:
+        check(lib().phx_cast_closest_instances(self.ctx.handle, scene.handle if scene else None,
+                                               self.handle, rays_ptr, n, hits_ptr, ids_ptr,
+                                               MEM_DEVICE | (ASYNC if asynchronous else 0)))
+
+    def cast_anyhit(self, rays: np.ndarray, scene: "Scene | None" = None) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY)
+        out = np.empty(len(rays), np.uint8)
+        check(lib().phx_cast_anyhit_instances(self.ctx.handle, scene.handle if scene else None,
+                                              self.handle, _ptr(rays), len(rays), _ptr(out), 0))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            lib().phx_instances_release(self.handle)
             self.handle = None
 
 
