@@ -651,6 +651,15 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* renderer, const PhxSce
                             const PhxCameraDef* cam, uint32_t frameIndex, uint64_t seed,
                             uint32_t flags, float* rgba);
 
+/* Instances drawn by this renderer's frames besides the frame's scene (NULL: none): camera and
+ * bounce rays take the closest of the scene's and the instances' hits (phx_cast_closest_instances
+ * with the scene), shadow rays are blocked by either, and a hit of an instance reads its mesh's
+ * TriangleData (material index into the frame scene's materials) with the interpolated normal
+ * mapped to world space by the inverse transpose of the instance transform. The set must stay
+ * valid while it is set (a released set makes phx_render_frame fail with
+ * PHX_ERR_INVALID_HANDLE); rebuild and set it again when instances move. */
+phx_status phx_renderer_set_instances(PhxRenderer* renderer, const PhxInstanceSet* set);
+
 #define PHX_BUFFER_COLOR 0u       /* float4 per pixel, row-major: the frame before TAA        */
 #define PHX_BUFFER_OUTPUT 1u      /* float4 per pixel: the last output (TAA or frame colour)  */
 #define PHX_BUFFER_GBUFFER 2u     /* 3 float4 per primary record (pixel_id order)             */

@@ -318,18 +318,25 @@ static const uint8_t* texel_of(const OrTexture* t, float u, float v)
  * reads the default material (albedo 1, roughness 1, metallic 0), a texture index outside
  * [0, numTex) means no texture. Albedo textures are read only with 4 bytes per texel
  * (CPUTracerRenderer.cpp:200-206; the CUDA path reads uchar4). */
-void oracle_interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const OrTriData* datas,
-                      const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
-                      uint32_t numTex, OrHitInfo* out)
+/* Hit records; with instances (instIds not NULL), a hit of instance k reads that instance's mesh
+ * data and its interpolated normal is mapped to world space by the transpose of the instance's
+ * world -> object matrix (A, rows of 4) before normalisation. */
+static void interpret_records(const OrRay* rays, const OrHit* hits, const uint32_t* instIds,
+                              uint32_t n, const OrTriData* datas, const OrInstance* inst,
+                              const float* instA, uint32_t numInst, const OrMaterial* mats,
+                              uint32_t numMats, const OrTexture* texs, uint32_t numTex,
+                              OrHitInfo* out)
 {
 	float lin[256];
 	oracle_linear_table(lin);
 	for (uint32_t i = 0; i < n; i++) {
 		OrHitInfo hi;
 		memset(&hi, 0, sizeof(hi));
+		const uint32_t ik = instIds ? instIds[i] : 0xFFFFFFFFu;
+		const OrTriData* dtab = ik < numInst ? inst[ik].datas : datas;
 		if (hits[i].tri != 0xFFFFFFFFu) {
 			const OrRay* r = &rays[i];
-			const OrTriData* d = &datas[hits[i].tri];
+			const OrTriData* d = &dtab[hits[i].tri];
 			float t = hits[i].t, bu = hits[i].u, bv = hits[i].v;
 			for (int k = 0; k < 3; k++) hi.d1[k] = r->o[k] + r->d[k] * t;
 			V3 nn;
@@ -338,6 +345,15 @@ void oracle_interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const Or
 				float a = (d->n1[k] - d->n0[k]) * bu;
 				float b = (d->n2[k] - d->n0[k]) * bv;
 				np[k] = (d->n0[k] + a) + b;
+			}
+			if (ik < numInst) {
+				const float* A = &instA[12 * ik];
+				float w[3];
+				for (int c = 0; c < 3; c++) {
+					const float a = A[c] * np[0] + A[4 + c] * np[1];
+					w[c] = a + A[8 + c] * np[2];
+				}
+				nn = v3(w[0], w[1], w[2]);
 			}
 			nn = nrm(nn);
 			hi.d2[0] = nn.x;
@@ -379,6 +395,13 @@ void oracle_interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const Or
 		}
 		out[i] = hi;
 	}
+}
+
+void oracle_interpret(const OrRay* rays, const OrHit* hits, uint32_t n, const OrTriData* datas,
+                      const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
+                      uint32_t numTex, OrHitInfo* out)
+{
+	interpret_records(rays, hits, 0, n, datas, 0, 0, 0u, mats, numMats, texs, numTex, out);
 }
 
 /* shadeHit, MaterialKernel.cu:49-114, for one record: contribution and shadow ray per light,
@@ -537,17 +560,25 @@ void oracle_bounce_rays(const OrTriData* datas, const OrRay* rays, const OrHit* 
  * every light + shadow rays, any-hit, shadow logic; one bounce per 2x2 pixel block (random pixel
  * of the block, diffuse or GGX by roughness, throughput albedo * 0.6), closest hit, interpret,
  * shade x throughput + shadow rays, any-hit, shadow logic added at half resolution.
+ * With instances (numInst > 0), closest hits and any hits are oracle_cast_instances' over the scene
+ * and the instances, and hit records of instance hits read the instance's mesh data.
  * Outputs: colour (w*h RGBA, row-major); optional G-buffer (record order) and bounce rays. */
 void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas,
                   const OrMaterial* mats, uint32_t numMats, const OrTexture* texs,
                   uint32_t numTex, const OrLight* lights,
                   uint32_t numLights, const float cam[12], uint32_t w, uint32_t h, float jx,
                   float jy, uint64_t seed, uint32_t frame, float* color, OrHitInfo* gbufOut,
-                  OrRay* bounceOut, uint32_t threads)
+                  OrRay* bounceOut, uint32_t threads, const OrInstance* inst, uint32_t numInst)
 {
 	const uint32_t N = w * h;
 	const uint32_t hw = w / 2u, hh = h / 2u, M = hw * hh;
 	const V3 camPos = v3(cam[0], cam[1], cam[2]);
+	float* instA = (float*)malloc(sizeof(float) * 12 * (numInst ? numInst : 1));
+	for (uint32_t k = 0; k < numInst; k++) {
+		float mn[3], mx[3];
+		oracle_instance_prepare(&inst[k], &instA[12 * k], mn, mx);
+	}
+	uint32_t* ids = (uint32_t*)malloc(sizeof(uint32_t) * (N ? N : 1));
 	OrRay* prim = (OrRay*)malloc(sizeof(OrRay) * N);
 	OrHit* ph = (OrHit*)malloc(sizeof(OrHit) * N);
 	OrHitInfo* gb = (OrHitInfo*)malloc(sizeof(OrHitInfo) * N);
@@ -557,8 +588,14 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 	uint8_t* inl = (uint8_t*)malloc(ns ? ns : 1);
 
 	oracle_primary_rays_jittered(cam, w, h, jx, jy, 1e-4f, FLT_MAX, prim);
-	oracle_cast_closest(nodes, tris, prim, N, ph, threads);
-	oracle_interpret(prim, ph, N, datas, mats, numMats, texs, numTex, gb);
+	if (numInst) {
+		oracle_cast_instances(nodes, tris, inst, numInst, prim, N, ph, ids, 0);
+		interpret_records(prim, ph, ids, N, datas, inst, instA, numInst, mats, numMats, texs,
+		                  numTex, gb);
+	} else {
+		oracle_cast_closest(nodes, tris, prim, N, ph, threads);
+		oracle_interpret(prim, ph, N, datas, mats, numMats, texs, numTex, gb);
+	}
 
 	/* gBufferMaterialKernel (MaterialKernel.cu:119-140); a pixel without a hit gets dummy rays
 	 * and no contribution, as materialKernel does for a secondary miss (:151-164). */
@@ -578,7 +615,10 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 		shade_hit(id, N, v3(1.0f, 1.0f, 1.0f), &rng, n, toCam, p, ld3(g->d3), g->d2[3], g->d1[3],
 		          lights, numLights, contrib, shadow);
 	}
-	oracle_cast_anyhit(nodes, tris, shadow, (uint32_t)ns, inl, threads);
+	if (numInst)
+		oracle_cast_instances(nodes, tris, inst, numInst, shadow, (uint32_t)ns, 0, 0, inl);
+	else
+		oracle_cast_anyhit(nodes, tris, shadow, (uint32_t)ns, inl, threads);
 	shadow_logic(color, w, h, 1u, 0, inl, contrib, numLights);
 
 	/* createSecondaryRaysKernel (MaterialKernel.cu:251-312) on the half-resolution grid. */
@@ -634,8 +674,14 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 				tp[3 * hid + 2] = t3.z;
 			}
 		if (bounceOut) memcpy(bounceOut, sec, sizeof(OrRay) * M);
-		oracle_cast_closest(nodes, tris, sec, M, sh, threads);
-		oracle_interpret(sec, sh, M, datas, mats, numMats, texs, numTex, sgb);
+		if (numInst) {
+			oracle_cast_instances(nodes, tris, inst, numInst, sec, M, sh, ids, 0);
+			interpret_records(sec, sh, ids, M, datas, inst, instA, numInst, mats, numMats, texs,
+			                  numTex, sgb);
+		} else {
+			oracle_cast_closest(nodes, tris, sec, M, sh, threads);
+			oracle_interpret(sec, sh, M, datas, mats, numMats, texs, numTex, sgb);
+		}
 		/* materialKernel (MaterialKernel.cu:145-184) */
 		size_t ms = (size_t)M * numLights;
 		for (uint32_t id = 0; id < M; id++) {
@@ -655,7 +701,10 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 			shade_hit(id, M, ld3(&tp[3 * id]), &rng, ld3(g->d2), v, ld3(g->d1), ld3(g->d3),
 			          g->d2[3], g->d1[3], lights, numLights, contrib, shadow);
 		}
-		oracle_cast_anyhit(nodes, tris, shadow, (uint32_t)ms, inl, threads);
+		if (numInst)
+			oracle_cast_instances(nodes, tris, inst, numInst, shadow, (uint32_t)ms, 0, 0, inl);
+		else
+			oracle_cast_anyhit(nodes, tris, shadow, (uint32_t)ms, inl, threads);
 		shadow_logic(color, w, h, 2u, 1, inl, contrib, numLights);
 		free(sec);
 		free(tp);
@@ -663,6 +712,8 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
 		free(sgb);
 	}
 	if (gbufOut) memcpy(gbufOut, gb, sizeof(OrHitInfo) * N);
+	free(instA);
+	free(ids);
 	free(prim);
 	free(ph);
 	free(gb);

@@ -101,6 +101,7 @@ typedef struct OrInstance {
 	const OrNode* nodes;
 	const OrTri* tris;
 	float objectToWorld[12];
+	const struct OrTriData* datas; /* hit records of the frame (may be NULL for casts) */
 } OrInstance;
 
 /* World -> object rows and world box of one instance (as phx_instances_build defines them). */
@@ -176,7 +177,7 @@ void oracle_frame(const OrNode* nodes, const OrTri* tris, const OrTriData* datas
                   uint32_t numTex, const OrLight* lights,
                   uint32_t numLights, const float cam[12], uint32_t w, uint32_t h, float jx,
                   float jy, uint64_t seed, uint32_t frame, float* color, OrHitInfo* gbufOut,
-                  OrRay* bounceOut, uint32_t threads);
+                  OrRay* bounceOut, uint32_t threads, const OrInstance* inst, uint32_t numInst);
 void oracle_velocity(const float cam[12], const float prevCam[12], const OrHitInfo* gbuf,
                      const OrRay* prim, uint32_t w, uint32_t h, float* vel);
 void oracle_taa(const float* cur, const float* histIn, const float* roughness, const float* velIn,

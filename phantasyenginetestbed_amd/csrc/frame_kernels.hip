@@ -79,10 +79,16 @@ __device__ __forceinline__ const uint8_t* texel_at(const TexSet& tex, uint32_t i
 // c / 255 (:86-93; other texel sizes keep the constant, CPUTracerRenderer.cpp:200-206); roughness
 // and metallic from byte 0 as c / 255 (:96-111). Record layout (48 B): (pos, roughness)
 // (normal, metallic) (albedo, alpha), alpha negative for a miss.
+// With instances (instIds not null): a hit of instance k reads that instance's mesh data and its
+// interpolated normal is mapped to world space by the transpose of the world -> object matrix
+// (the inverse transpose of the object -> world one) before normalisation.
 __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict__ rays,
                                                         const uint4* __restrict__ hits, uint32_t n,
                                                         const PhxTriangleData* __restrict__ datas,
                                                         uint32_t numTris,
+                                                        const uint32_t* __restrict__ instIds,
+                                                        const DevInstance* __restrict__ inst,
+                                                        uint32_t numInst,
                                                         const PhxMaterial* __restrict__ mats,
                                                         uint32_t numMats, TexSet tex,
                                                         float4* __restrict__ info)
@@ -90,6 +96,12 @@ __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const uint4 h = hits[i];
+	const uint32_t k = instIds ? instIds[i] : 0xFFFFFFFFu;
+	const DevInstance* in = k < numInst ? &inst[k] : nullptr;
+	if (in) {
+		datas = in->datas;
+		numTris = datas ? in->numTris : 0u;
+	}
 	if (h.x >= numTris) {  // a miss (0xFFFFFFFF), or an index of another scene
 		info[3 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		info[3 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -105,6 +117,17 @@ __global__ __launch_bounds__(kBlock1D) void k_interpret(const float4* __restrict
 		const float a = (d.n1[k] - d.n0[k]) * bu;
 		const float b = (d.n2[k] - d.n0[k]) * bv;
 		nn[k] = (d.n0[k] + a) + b;
+	}
+	if (in) {
+		const float* A = in->worldToObject;
+		float w[3];
+		for (int c = 0; c < 3; c++) {
+			const float a = A[c] * nn[0] + A[4 + c] * nn[1];
+			w[c] = a + A[8 + c] * nn[2];
+		}
+		nn[0] = w[0];
+		nn[1] = w[1];
+		nn[2] = w[2];
 	}
 	const V3 nrm = normalize(v3(nn[0], nn[1], nn[2]));
 	float4 alb = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
@@ -457,12 +480,13 @@ dim3 grid2d(uint32_t w, uint32_t h) { return dim3((w + 15u) / 16u, (h + 15u) / 1
 
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
                            const PhxTriangleData* datas, uint32_t numTris, const PhxMaterial* mats,
-                           uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream)
+                           uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream,
+                           const uint32_t* instIds, const DevInstance* inst, uint32_t numInst)
 {
 	if (n == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_interpret, dim3(blocks1d(n)), dim3(kBlock1D), 0, stream,
 	                   reinterpret_cast<const float4*>(rays), reinterpret_cast<const uint4*>(hits),
-	                   n, datas, numTris, mats, numMats, tex, info);
+	                   n, datas, numTris, instIds, inst, numInst, mats, numMats, tex, info);
 	return hipGetLastError();
 }
 

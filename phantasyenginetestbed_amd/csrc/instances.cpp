@@ -26,6 +26,7 @@ struct PhxInstanceSet {
 	uint32_t count = 0;
 	PhxBVHNode* top = nullptr;    // device: top tree, one instance per leaf
 	DevInstance* inst = nullptr;  // device: instances in top-tree leaf order
+	DevInstance* byIndex = nullptr;  // device: instances in the caller's order (hit records)
 	std::vector<const PhxScene*> meshes;  // distinct meshes, held until release
 	uint64_t stamp = 0;
 };
@@ -183,6 +184,7 @@ void freeSet(PhxInstanceSet* s)
 {
 	hipFree(s->top);
 	hipFree(s->inst);
+	hipFree(s->byIndex);
 	for (const PhxScene* m : s->meshes) dropScene(m);
 	delete s;
 }
@@ -193,6 +195,7 @@ namespace phx {
 
 const PhxBVHNode* instanceTop(const PhxInstanceSet* s) { return s->top; }
 const DevInstance* instanceArray(const PhxInstanceSet* s) { return s->inst; }
+const DevInstance* instanceByIndex(const PhxInstanceSet* s) { return s->byIndex; }
 uint32_t instanceCount(const PhxInstanceSet* s) { return s->count; }
 uint64_t instanceStamp(const PhxInstanceSet* s) { return s->stamp; }
 bool instanceSetValid(PhxContext* ctx, const PhxInstanceSet* s) { return checkSet(ctx, s) == PHX_OK; }
@@ -253,6 +256,8 @@ phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, ui
 		if (hipMalloc(reinterpret_cast<void**>(&s->top), tb.nodes.size() * sizeof(PhxBVHNode)) !=
 		        hipSuccess ||
 		    hipMalloc(reinterpret_cast<void**>(&s->inst), laid.size() * sizeof(DevInstance)) !=
+		        hipSuccess ||
+		    hipMalloc(reinterpret_cast<void**>(&s->byIndex), dev.size() * sizeof(DevInstance)) !=
 		        hipSuccess) {
 			freeSet(s);
 			return fail(PHX_ERR_OUT_OF_MEMORY, "phx_instances_build: device allocation");
@@ -262,6 +267,9 @@ phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, ui
 		                              ctx->stream);
 		if (e == hipSuccess)
 			e = hipMemcpyAsync(s->inst, laid.data(), laid.size() * sizeof(DevInstance),
+			                   hipMemcpyHostToDevice, ctx->stream);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(s->byIndex, dev.data(), dev.size() * sizeof(DevInstance),
 			                   hipMemcpyHostToDevice, ctx->stream);
 		if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
 		if (e != hipSuccess) {

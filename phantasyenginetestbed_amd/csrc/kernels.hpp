@@ -179,10 +179,13 @@ hipError_t launchGenPrimaryRays(const PhxCameraDef& cam, uint32_t w, uint32_t h,
 
 // Wavefront frame kernels (frame_kernels.hip). Hit records are 3 float4 each:
 // (position, roughness) (normal, metallic) (albedo, alpha; alpha < 0 = no hit).
-// A hit whose triangle index is not below numTris gets the miss record.
+// A hit whose triangle index is not below numTris gets the miss record. With instIds, a hit of
+// instance k < numInst (inst in instance-index order) reads that instance's mesh data.
 hipError_t launchInterpret(const PhxRayIn* rays, const PhxRayHit* hits, uint32_t n,
                            const PhxTriangleData* datas, uint32_t numTris, const PhxMaterial* mats,
-                           uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream);
+                           uint32_t numMats, const TexSet& tex, float4* info, hipStream_t stream,
+                           const uint32_t* instIds = nullptr, const DevInstance* inst = nullptr,
+                           uint32_t numInst = 0);
 // The seed and frame index of the random-number streams, and the camera position, come from fp.
 hipError_t launchShade(const float4* info, const PhxRayIn* rays, const float4* throughput,
                        const PhxSphereLight* lights, uint32_t numLights, uint32_t count,

@@ -33,6 +33,11 @@ struct PhxRenderer {
 	bool haveHistory = false;
 	PhxCameraDef lastCam{};  // the previous frame's camera (TAA velocity)
 	bool haveLastCam = false;
+	// Instances drawn besides the scene (phx_renderer_set_instances) and the instance index of
+	// each primary / bounce hit.
+	const PhxInstanceSet* instances = nullptr;
+	uint32_t* primInst = nullptr;
+	uint32_t* bounceInst = nullptr;
 	float4* lastOut = nullptr;
 	FrameParams* params = nullptr;  // per-frame values the frame kernels read (device)
 	// The bounce rays' closest hits run on a side stream beside the primary shadow rays (fork
@@ -44,6 +49,7 @@ struct PhxRenderer {
 	struct GraphKey {
 		const PhxScene* scene;
 		uint64_t sceneVersion;  // unique per upload, renewed when the scene's textures are replaced
+		uint64_t instanceStamp;  // the instance set's (0: none)
 		uint32_t flags;  // PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP
 		uint32_t numLights;
 		const void* lightBuffers;
@@ -73,7 +79,7 @@ void freeRenderer(PhxRenderer* r)
 	void* bufs[] = {r->prim,    r->primHits, r->gbuf,   r->bounce,  r->throughput,
 	                r->bounceHits, r->bounceInfo, r->contrib, r->shadow, r->inLight,
 	                r->color,   r->hist[0],  r->hist[1], r->out,    r->params,
-	                r->vel[0],  r->vel[1]};
+	                r->vel[0],  r->vel[1], r->primInst, r->bounceInst};
 	for (void* b : bufs) hipFree(b);
 	delete r;
 }
@@ -136,14 +142,24 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 	TraceLaunch primary = closest;
 	if (order == PHX_RAYS_TILED_8X8) primary.kernel = TraceKernel::Packet;
 	const char* what = "phx_render_frame";
+	// Instances besides the scene: their closest hits are merged into the scene's (the scene
+	// wins ties), their any hits occlude too, and hit records read the instance's mesh.
+	const PhxInstanceSet* inst = r->instances && instanceCount(r->instances) ? r->instances : nullptr;
+	const uint32_t numInst = inst ? instanceCount(inst) : 0u;
+	const DevInstance* instByIndex = inst ? instanceByIndex(inst) : nullptr;
 
 	// Primary hits -> hit records -> contributions and shadow rays of every light.
 	PHX_HIP(launchGenPrimaryRays(*cam, r->w, r->h, 0.0f, 0.0f, 1e-4f, FLT_MAX, order, r->prim, s,
 	                             r->params),
 	        what);
 	PHX_HIP(launchClosest(sv, r->prim, r->n, r->primHits, primary, nullptr, s), what);
+	if (inst)
+		PHX_HIP(launchInstances(instanceTop(inst), instanceArray(inst), numInst, r->prim, r->n,
+		                        r->primHits, r->primInst, nullptr, true, s),
+		        what);
 	PHX_HIP(launchInterpret(r->prim, r->primHits, r->n, scene->datas, scene->numTris,
-	                        scene->materials, scene->numMaterials, scene->tex, r->gbuf, s),
+	                        scene->materials, scene->numMaterials, scene->tex, r->gbuf, s,
+	                        inst ? r->primInst : nullptr, instByIndex, numInst),
 	        what);
 	PHX_HIP(launchShade(r->gbuf, nullptr, nullptr, scene->lights, L, r->n, r->params, 1u,
 	                    r->contrib, r->shadow, s),
@@ -159,9 +175,17 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 		PHX_HIP(hipEventRecord(r->fork, s), what);
 		PHX_HIP(hipStreamWaitEvent(r->side, r->fork, 0), what);
 		PHX_HIP(launchClosest(sv, r->bounce, r->m, r->bounceHits, closest, nullptr, r->side), what);
+		if (inst)
+			PHX_HIP(launchInstances(instanceTop(inst), instanceArray(inst), numInst, r->bounce,
+			                        r->m, r->bounceHits, r->bounceInst, nullptr, true, r->side),
+			        what);
 		PHX_HIP(hipEventRecord(r->join, r->side), what);
 	}
 	PHX_HIP(launchAnyHit(sv, r->shadow, r->n * L, r->inLight, shadows, s), what);
+	if (inst)
+		PHX_HIP(launchInstances(instanceTop(inst), instanceArray(inst), numInst, r->shadow,
+		                        r->n * L, nullptr, nullptr, r->inLight, true, s),
+		        what);
 	PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 1u, false, r->inLight, r->contrib, L, s), what);
 
 	// Bounce hits: records, shading x throughput with shadow rays, any hit, added at half res.
@@ -169,12 +193,16 @@ phx_status enqueueFrame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
 		PHX_HIP(hipStreamWaitEvent(s, r->join, 0), what);
 		PHX_HIP(launchInterpret(r->bounce, r->bounceHits, r->m, scene->datas, scene->numTris,
 		                        scene->materials, scene->numMaterials, scene->tex, r->bounceInfo,
-		                        s),
+		                        s, inst ? r->bounceInst : nullptr, instByIndex, numInst),
 		        what);
 		PHX_HIP(launchShade(r->bounceInfo, r->bounce, r->throughput, scene->lights, L, r->m,
 		                    r->params, 3u, r->contrib, r->shadow, s),
 		        what);
 		PHX_HIP(launchAnyHit(sv, r->shadow, r->m * L, r->inLight, bounceShadows, s), what);
+		if (inst)
+			PHX_HIP(launchInstances(instanceTop(inst), instanceArray(inst), numInst, r->shadow,
+			                        r->m * L, nullptr, nullptr, r->inLight, true, s),
+			        what);
 		PHX_HIP(launchShadowLogic(r->color, r->w, r->h, 2u, true, r->inLight, r->contrib, L, s),
 		        what);
 	}
@@ -233,6 +261,8 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 	if (st == PHX_OK) st = devAlloc(r->color, r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->hist[0], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->hist[1], r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->primInst, r->n, what);
+	if (st == PHX_OK) st = devAlloc(r->bounceInst, std::max(r->m, 1u), what);
 	if (st == PHX_OK) st = devAlloc(r->vel[0], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->vel[1], r->n, what);
 	if (st == PHX_OK) st = devAlloc(r->out, r->n, what);
@@ -279,6 +309,15 @@ phx_status phx_renderer_buffer(PhxRenderer* r, uint32_t which, const void** devP
 	return ok();
 }
 
+phx_status phx_renderer_set_instances(PhxRenderer* r, const PhxInstanceSet* set)
+{
+	if (!r || !registry().has(r)) return fail(PHX_ERR_INVALID_HANDLE, "invalid renderer handle");
+	if (set && !instanceSetValid(r->ctx, set))
+		return fail(PHX_ERR_INVALID_HANDLE, "phx_renderer_set_instances: invalid instance set");
+	r->instances = set;
+	return ok();
+}
+
 phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* scene,
                             const PhxCameraDef* cam, uint32_t frameIndex, uint64_t seed,
                             uint32_t flags, float* rgba)
@@ -290,6 +329,9 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 	if (!r || !registry().has(r) || r->ctx != ctx)
 		return fail(PHX_ERR_INVALID_HANDLE, "invalid renderer handle");
 	if (!cam) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: NULL camera");
+	if (r->instances && !instanceSetValid(ctx, r->instances))
+		return fail(PHX_ERR_INVALID_HANDLE, "phx_render_frame: the renderer's instance set was "
+		            "released");
 	if (!scene->datas) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: scene has no TriangleData");
 	if (uint64_t(r->n) * scene->numLights + r->m > 0x7FFFFFFFull)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_render_frame: more than 2^31 shadow rays");
@@ -316,6 +358,7 @@ phx_status phx_render_frame(PhxContext* ctx, PhxRenderer* r, const PhxScene* sce
 		PhxRenderer::GraphKey k{};
 		k.scene = scene;
 		k.sceneVersion = scene->version;
+		k.instanceStamp = r->instances ? instanceStamp(r->instances) : 0u;
 		k.flags = flags & (PHX_FRAME_TAA | PHX_FRAME_NO_CLAMP);
 		k.numLights = scene->numLights;
 		k.lightBuffers = r->contrib;

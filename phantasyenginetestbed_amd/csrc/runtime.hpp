@@ -144,6 +144,7 @@ inline phx_status checkScene(PhxContext* ctx, const PhxScene* scene)
 // Views of an instance set (instances.cpp) for the renderer.
 const PhxBVHNode* instanceTop(const PhxInstanceSet* s);
 const DevInstance* instanceArray(const PhxInstanceSet* s);
+const DevInstance* instanceByIndex(const PhxInstanceSet* s);
 uint32_t instanceCount(const PhxInstanceSet* s);
 uint64_t instanceStamp(const PhxInstanceSet* s);
 bool instanceSetValid(PhxContext* ctx, const PhxInstanceSet* s);

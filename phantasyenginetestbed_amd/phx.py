@@ -204,6 +204,7 @@ _SIGNATURES = {
     "phx_render_frame": (C.c_int, [_P, _P, _P, C.POINTER(CameraDef), _U32, C.c_uint64, _U32,
                                    _P]),
     "phx_renderer_buffer": (C.c_int, [_P, _U32, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "phx_renderer_set_instances": (C.c_int, [_P, _P]),
     "phx_halton23": (None, [_U32, _PF, _PF]),
     "phx_server_create": (C.c_int, [_P, C.c_uint64, _U32, _U32, C.POINTER(_P)]),
     "phx_server_destroy": (None, [_P]),
@@ -654,6 +655,9 @@ class Renderer:
         check(lib().phx_render_frame(self.ctx.handle, self.handle, scene.handle, C.byref(cam),
                                      frame, seed, flags, _ptr(out)))
         return out
+
+    def set_instances(self, iset: "InstanceSet | None") -> None:
+        check(lib().phx_renderer_set_instances(self.handle, iset.handle if iset else None))
 
     def buffer(self, which: int) -> tuple[int, int]:
         p, n = C.c_void_p(), C.c_size_t()

@@ -1,0 +1,189 @@
+"""Two-level (instance) casts on the GPU through the C ABI, bit-identical to the oracle's
+restatement: closest hits with the winning instance, any hits, with and without a static scene,
+one instance, no instances, device buffers, instances moved between builds, and a mesh released
+while a set still uses it."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from phantasyenginetestbed_amd import phx
+from test_instances_oracle import random_rays, transform
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = phx.Context(0)
+    yield c
+    c.destroy()
+
+
+@pytest.fixture(scope="module")
+def meshes(ctx):
+    out = []
+    for k, n in enumerate((300, 4000, 20000)):
+        t, d = phx.random_triangles(n, 200 + k, 0.01, 0.1)
+        b = phx.BVH(t, d)
+        out.append((b, ctx.upload_scene(b.nodes, b.tris, b.datas)))
+    yield out
+    for _, s in out:
+        s.release()
+
+
+def placed(meshes, count, seed, spread=6.0):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        b, s = meshes[k % len(meshes)]
+        m = transform(rng, mirror=(k % 7 == 3))
+        m[:, 3] = rng.uniform(-spread, spread, 3)
+        specs.append((b, s, m))
+    return specs
+
+
+def check(ctx, specs, rays, scene=None):
+    iset = phx.InstanceSet(ctx, [(s, m) for _, s, m in specs])
+    try:
+        hits, ids = iset.cast_closest(rays, scene=scene[1] if scene else None)
+        lit = iset.cast_anyhit(rays, scene=scene[1] if scene else None)
+    finally:
+        iset.release()
+    inst = [(b.nodes, b.tris, m) for b, _, m in specs]
+    sc = (scene[0].nodes, scene[0].tris) if scene else None
+    ref_hits, ref_ids = O.cast_instances(inst, rays, scene=sc)
+    ref_lit = O.cast_instances(inst, rays, scene=sc, anyhit=True)
+    assert hits.tobytes() == ref_hits.tobytes(), f"{(hits != ref_hits).sum()} hits differ"
+    assert (ids == ref_ids).all(), f"{(ids != ref_ids).sum()} instance ids differ"
+    assert (lit == ref_lit).all(), f"{(lit != ref_lit).sum()} any-hit flags differ"
+    return hits, ids, lit
+
+
+@pytest.mark.parametrize("count", [1, 2, 37, 300])
+def test_instances_match_oracle(ctx, meshes, count):
+    say(f"[gpu-inst] {count} instances")
+    specs = placed(meshes, count, seed=count)
+    rays = random_rays(np.random.default_rng(count), 30000, lo=-9, hi=9, aim=(-6, 6))
+    rays["tmax"][::3] = np.float32(5.0)  # finite tMax for a third of the rays
+    hits, ids, lit = check(ctx, specs, rays)
+    hit = hits["tri"] != phx.MISS
+    assert hit.mean() > 0.02 and (lit == 0).any()
+    if count > 1:
+        assert len(np.unique(ids[hit])) > 1
+
+
+def test_instances_with_a_static_scene(ctx, meshes):
+    say("[gpu-inst] instances over a static heightfield")
+    t, d = phx.heightfield_boxes(64, 50, 0xC0FFEE02)
+    b = phx.BVH(t, d)
+    s = ctx.upload_scene(b.nodes, b.tris, b.datas)
+    try:
+        specs = placed(meshes, 40, seed=9, spread=0.6)
+        for _, _, m in specs:
+            m[:, :3] *= np.float32(0.1)
+            m[1, 3] = abs(m[1, 3]) * 0.2
+        rays = random_rays(np.random.default_rng(9), 30000, lo=-1, hi=2, aim=(0, 1))
+        hits, ids, _ = check(ctx, specs, rays, scene=(b, s))
+        hit = hits["tri"] != phx.MISS
+        assert (ids[hit] == phx.MISS).any() and (ids[hit] != phx.MISS).any()
+    finally:
+        s.release()
+
+
+def test_empty_set_device_buffers_and_moves(ctx, meshes):
+    rays = random_rays(np.random.default_rng(4), 8192, lo=-9, hi=9, aim=(-6, 6))
+    empty = phx.InstanceSet(ctx, [])
+    try:
+        hits, ids = empty.cast_closest(rays)
+        assert (hits["tri"] == phx.MISS).all() and (ids == phx.MISS).all()
+        assert (empty.cast_anyhit(rays) == 1).all()
+    finally:
+        empty.release()
+    specs = placed(meshes, 20, seed=11)
+    rb = ctx.alloc(len(rays) * 32).upload(rays)
+    hb, ib = ctx.alloc(len(rays) * 16), ctx.alloc(len(rays) * 4)
+    try:
+        for move in range(3):
+            say(f"[gpu-inst] device buffers, move {move}")
+            for _, _, m in specs:
+                m[:, 3] += np.float32(0.3)
+            iset = phx.InstanceSet(ctx, [(s, m) for _, s, m in specs])
+            try:
+                iset.cast_closest_device(rb.ptr, len(rays), hb.ptr, ib.ptr)
+                ref_hits, ref_ids = O.cast_instances([(b.nodes, b.tris, m) for b, _, m in specs],
+                                                     rays)
+                assert hb.download(phx.HIT, len(rays)).tobytes() == ref_hits.tobytes()
+                assert (ib.download(np.uint32, len(rays)) == ref_ids).all()
+            finally:
+                iset.release()
+    finally:
+        rb.free()
+        hb.free()
+        ib.free()
+
+
+def test_set_holds_released_meshes_and_rejects_bad_transforms(ctx):
+    t, d = phx.random_triangles(2000, 5, 0.02, 0.2)
+    b = phx.BVH(t, d)
+    s = ctx.upload_scene(b.nodes, b.tris, b.datas)
+    m = np.hstack([np.eye(3), np.zeros((3, 1))]).astype(np.float32)
+    with pytest.raises(phx.PhxError) as ei:
+        phx.InstanceSet(ctx, [(s, np.zeros((3, 4), np.float32))])
+    assert ei.value.status == phx.ERR_INVALID_ARGUMENT
+    iset = phx.InstanceSet(ctx, [(s, m)])
+    s.release()  # the set keeps the mesh's arrays
+    try:
+        rays = random_rays(np.random.default_rng(5), 4096, aim=(0, 1))
+        hits, ids = iset.cast_closest(rays)
+        assert hits.tobytes() == O.cast_closest(b.nodes, b.tris, rays).tobytes()
+    finally:
+        iset.release()
+
+
+def test_frame_with_instances_bit_identical(ctx, meshes):
+    """A frame of a heightfield plus 30 instances (the renderer's instance set): hit records,
+    bounce rays and colour equal the oracle's frame over the same scene and instances, also
+    replayed from a captured graph; moving the instances takes effect on the next frame."""
+    from phantasyenginetestbed_amd import workloads as W
+    t, d = phx.heightfield_boxes(48, 40, 0xC0FFEE02, num_materials=8)
+    b = phx.BVH(t, d)
+    mats = phx.materials_random(8, 3)
+    lts = phx.lights_random(4, 3)
+    s = ctx.upload_scene(b.nodes, b.tris, b.datas, mats, lts)
+    w, h = 64, 40
+    cam = W.camera(w, h)
+    rng = np.random.default_rng(7)
+    specs = []
+    for k in range(30):
+        mb, ms = meshes[k % len(meshes)]
+        m = transform(rng, scale=rng.uniform(0.05, 0.15, 3))
+        m[:, 3] = (rng.uniform(0.1, 0.9), rng.uniform(0.0, 0.1), rng.uniform(0.2, 0.9))
+        specs.append((mb, ms, m))
+    r = ctx.renderer(w, h)
+    try:
+        for step in range(3):
+            say(f"[gpu-inst] frame with 30 instances, step {step}")
+            iset = phx.InstanceSet(ctx, [(ms, m) for _, ms, m in specs])
+            r.set_instances(iset)
+            flags = phx.FRAME_GRAPH if step else 0
+            out = r.render(s, cam, step, 11, flags)
+            gbuf = r.download(phx.BUFFER_GBUFFER, phx.HITINFO)
+            bounce = r.download(phx.BUFFER_BOUNCE_RAYS, phx.RAY)
+            ref, ref_gbuf, ref_bounce = O.frame(
+                b.nodes, b.tris, b.datas, mats, lts, cam.as_array(), w, h, 0.0, 0.0, 11, step,
+                instances=[(mb.nodes, mb.tris, m, mb.datas) for mb, _, m in specs])
+            assert gbuf.tobytes() == ref_gbuf.tobytes(), "hit records"
+            assert bounce.tobytes() == ref_bounce.tobytes(), "bounce rays"
+            assert out.tobytes() == ref.tobytes(), "frame colour"
+            r.set_instances(None)
+            iset.release()
+            for _, _, m in specs:
+                m[:, 3] += np.float32(0.02)
+        assert (ref_gbuf["alpha"] >= 0).mean() > 0.3
+    finally:
+        r.release()
+        s.release()
