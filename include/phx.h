@@ -376,7 +376,11 @@ phx_status phx_cast_anyhit(PhxContext* ctx, const PhxScene* scene, const PhxRayI
 /* castDebugRay's counters (BVHTraversal.hpp:29-35, 130-235) of phx_cast_closest on the same rays,
  * summed over the batch, computed on the GPU by a counting variant of the traversal kernel:
  * counters = {nodesVisited, trianglesIntersected, triangleIntersectionTests,
- * aabbIntersectionTests, aabbIntersections}. Synchronous; hits are discarded. */
+ * aabbIntersectionTests, aabbIntersections}. Synchronous; hits are discarded. With
+ * PHX_STATS_ANYHIT the counters are those of phx_cast_anyhit's traversal instead (castRay's
+ * order with tCurrMax = tMax, ending at the first triangle with tMin <= t < tMax;
+ * trianglesIntersected = rays occluded); no counterpart in the reference. */
+#define PHX_STATS_ANYHIT 0x4000u
 phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                                uint32_t numRays, uint32_t flags, uint64_t counters[5]);
 

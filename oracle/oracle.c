@@ -168,7 +168,10 @@ static void cast_one(const OrNode* nodes, const OrTri* tris, const OrRay* in, Or
 	}
 }
 
-static int anyhit_one(const OrNode* nodes, const OrTri* tris, const OrRay* in)
+/* Any hit: castRay's traversal with tCurrMax = tMax, ending at the first triangle with
+ * tMin <= t < tMax. `cnt` (may be NULL) receives castDebugRay-style counters of that traversal
+ * (trianglesIntersected = 1 if occluded). */
+static int anyhit_counted(const OrNode* nodes, const OrTri* tris, const OrRay* in, uint32_t* cnt)
 {
 	Ray ray;
 	make_ray(in, &ray);
@@ -176,17 +179,22 @@ static int anyhit_one(const OrNode* nodes, const OrTri* tris, const OrRay* in)
 	uint32_t stack[STACK_MAX];
 	stack[0] = 0u;
 	uint32_t sp = 1u;
-	while (sp > 0u) {
+	uint32_t c[5] = {0, 0, 0, 0, 0};
+	int occluded = 0;
+	while (sp > 0u && !occluded) {
 		sp--;
 		const OrNode* node = &nodes[stack[sp]];
+		c[0]++;
+		c[3] += 2;
 		float lIn, lOut, rIn, rOut;
 		int lHit = aabb(&ray, &node->f[0], &node->f[3], &lIn, &lOut);
 		int rHit = aabb(&ray, &node->f[6], &node->f[9], &rIn, &rOut);
 		int visit[2];
 		visit[0] = lHit && lOut > tMin && lIn < tMax;
 		visit[1] = rHit && rOut > tMin && rIn < tMax;
-		for (int side = 0; side < 2; side++) {
+		for (int side = 0; side < 2 && !occluded; side++) {
 			if (!visit[side]) continue;
+			c[4]++;
 			int32_t raw = node->i[side];
 			uint32_t idx = (uint32_t)(raw < 0 ? ~raw : raw);
 			int32_t count = node->i[2 + side];
@@ -198,13 +206,24 @@ static int anyhit_one(const OrNode* nodes, const OrTri* tris, const OrRay* in)
 			for (int32_t k = 0; k < count; k++) {
 				const OrTri* tri = &tris[idx + (uint32_t)k];
 				float t, u, v;
+				c[2]++;
 				if (triangle(tri->v[0], tri->v[1], tri->v[2], ray.o, ray.d, &t, &u, &v) &&
-				    tMin <= t && t < tMax)
-					return 1;
+				    tMin <= t && t < tMax) {
+					c[1]++;
+					occluded = 1;
+					break;
+				}
 			}
 		}
 	}
-	return 0;
+	if (cnt)
+		for (int k = 0; k < 5; k++) cnt[k] = c[k];
+	return occluded;
+}
+
+static int anyhit_one(const OrNode* nodes, const OrTri* tris, const OrRay* in)
+{
+	return anyhit_counted(nodes, tris, in, 0);
 }
 
 /* ---- threading: chunks of rays pulled from an atomic counter ---- */
@@ -233,7 +252,11 @@ static void* worker(void* arg)
 		if (b >= j->n) break;
 		uint32_t e = b + CHUNK < j->n ? b + CHUNK : j->n;
 		for (uint32_t i = b; i < e; i++) {
-			if (j->mode == 2) {
+			if (j->mode == 3) {
+				uint32_t c[5];
+				j->inLight[i] = (uint8_t)!anyhit_counted(j->nodes, j->tris, &j->rays[i], c);
+				for (int k = 0; k < 5; k++) local[k] += c[k];
+			} else if (j->mode == 2) {
 				j->inLight[i] = (uint8_t)!anyhit_one(j->nodes, j->tris, &j->rays[i]);
 			} else if (j->mode == 1) {
 				uint32_t c[5];
@@ -296,6 +319,19 @@ void oracle_cast_anyhit(const OrNode* nodes, const OrTri* tris, const OrRay* ray
 	Job j = {nodes, tris, rays, n, 0, inLight, 0};
 	j.mode = 2;
 	run(&j, numThreads);
+}
+
+void oracle_anyhit_debug(const OrNode* nodes, const OrTri* tris, const OrRay* rays, uint32_t n,
+                         uint8_t* inLight, OrCounters* total, uint32_t numThreads)
+{
+	Job j = {nodes, tris, rays, n, 0, inLight, 0};
+	j.mode = 3;
+	run(&j, numThreads);
+	total->nodesVisited = atomic_load(&j.sum[0]);
+	total->trianglesIntersected = atomic_load(&j.sum[1]);
+	total->triangleIntersectionTests = atomic_load(&j.sum[2]);
+	total->aabbIntersectionTests = atomic_load(&j.sum[3]);
+	total->aabbIntersections = atomic_load(&j.sum[4]);
 }
 
 void oracle_brute_closest(const OrTri* tris, uint32_t numTris, const OrRay* rays, uint32_t n,

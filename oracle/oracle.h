@@ -76,6 +76,11 @@ void oracle_cast_debug(const OrNode* nodes, const OrTri* tris, const OrRay* rays
 void oracle_cast_anyhit(const OrNode* nodes, const OrTri* tris, const OrRay* rays, uint32_t n,
                         uint8_t* inLight, uint32_t numThreads);
 
+/* oracle_cast_anyhit plus castDebugRay-style counters of the any-hit traversal, summed
+ * (trianglesIntersected = rays occluded). */
+void oracle_anyhit_debug(const OrNode* nodes, const OrTri* tris, const OrRay* rays, uint32_t n,
+                         uint8_t* inLight, OrCounters* total, uint32_t numThreads);
+
 /* Every triangle tested with Intersection.hpp:24-67 in index order; accepts t < best.t &&
  * tMin <= t <= tMax (the first index wins ties). No BVH. */
 void oracle_brute_closest(const OrTri* tris, uint32_t numTris, const OrRay* rays, uint32_t n,

@@ -68,6 +68,7 @@ def _configure(L: C.CDLL) -> C.CDLL:
     L.oracle_cast_closest.argtypes = [_P, _P, _P, _U32, _P, _U32]
     L.oracle_cast_debug.argtypes = [_P, _P, _P, _U32, _P, C.POINTER(Counters), _P, _U32]
     L.oracle_cast_anyhit.argtypes = [_P, _P, _P, _U32, _P, _U32]
+    L.oracle_anyhit_debug.argtypes = [_P, _P, _P, _U32, _P, C.POINTER(Counters), _U32]
     L.oracle_brute_closest.argtypes = [_P, _U32, _P, _U32, _P]
     L.oracle_primary_rays.argtypes = [_P, _U32, _U32, C.c_float, C.c_float, _U32, _P]
     L.oracle_triangle.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P]
@@ -152,6 +153,15 @@ def cast_anyhit(nodes, tris, rays, threads: int = 0):
     out = np.empty(len(rays), np.uint8)
     olib().oracle_cast_anyhit(_p(nodes), _p(tris), _p(rays), len(rays), _p(out), threads)
     return out
+
+
+def anyhit_debug(nodes, tris, rays, threads: int = 0):
+    """(inLight, summed counters of the any-hit traversal)."""
+    out = np.empty(len(rays), np.uint8)
+    tot = Counters()
+    olib().oracle_anyhit_debug(_p(nodes), _p(tris), _p(rays), len(rays), _p(out), C.byref(tot),
+                               threads)
+    return out, {f: getattr(tot, f) for f, _ in Counters._fields_}
 
 
 def brute_closest(tris, rays):

@@ -13,6 +13,8 @@ HipTracerRenderer::HipTracerRenderer(int device) noexcept
 HipTracerRenderer::~HipTracerRenderer() noexcept
 {
 	if (mRenderer) phx_renderer_release(mRenderer);
+	if (mInstances) phx_instances_release(mInstances);
+	for (PhxScene* m : mMeshes) phx_scene_release(m);
 	if (mScene) phx_scene_release(mScene);
 	if (mCtx) phx_destroy(mCtx);
 }
@@ -96,16 +98,79 @@ void HipTracerRenderer::setStaticScene(const StaticScene& staticScene) noexcept
 	uploadScene();
 }
 
-void HipTracerRenderer::setDynamicMeshes(const std::vector<RawMesh>&) noexcept
+// One mesh as an object-space scene: triangle i is vertices[indices[3i..3i+2]] with their normals
+// and uvs, material materialIndices[indices[3i]] (as createDynamicBvh, DynamicBVH.cpp:127-160,
+// reads them), under its own SBVH. Null (and the error recorded) for an unusable mesh.
+PhxScene* HipTracerRenderer::uploadMesh(const RawMesh& mesh) noexcept
 {
-	mStatus = PHX_ERR_INVALID_ARGUMENT;
-	mError = "HipTracerRenderer::setDynamicMeshes: dynamic meshes are not supported";
+	const size_t nt = mesh.indices.size() / 3;
+	if (nt == 0 || mesh.indices.size() % 3 != 0) {
+		mStatus = PHX_ERR_INVALID_SCENE;
+		mError = "HipTracerRenderer: a dynamic mesh needs a whole number of triangles";
+		return nullptr;
+	}
+	std::vector<PhxTriangleVertices> tris(nt);
+	std::vector<PhxTriangleData> datas(nt);
+	for (size_t t = 0; t < nt; t++) {
+		const Vertex* v[3];
+		for (int k = 0; k < 3; k++) {
+			const uint32_t idx = mesh.indices[3 * t + k];
+			if (idx >= mesh.vertices.size()) {
+				mStatus = PHX_ERR_INVALID_SCENE;
+				mError = "HipTracerRenderer: a dynamic mesh index is out of range";
+				return nullptr;
+			}
+			v[k] = &mesh.vertices[idx];
+		}
+		float* p[3] = {tris[t].v0, tris[t].v1, tris[t].v2};
+		float* n[3] = {datas[t].n0, datas[t].n1, datas[t].n2};
+		float* uv[3] = {datas[t].uv0, datas[t].uv1, datas[t].uv2};
+		for (int k = 0; k < 3; k++) {
+			p[k][0] = v[k]->pos.x, p[k][1] = v[k]->pos.y, p[k][2] = v[k]->pos.z, p[k][3] = 0.0f;
+			n[k][0] = v[k]->normal.x, n[k][1] = v[k]->normal.y, n[k][2] = v[k]->normal.z;
+			uv[k][0] = v[k]->uv[0], uv[k][1] = v[k]->uv[1];
+		}
+		const uint32_t first = mesh.indices[3 * t];
+		datas[t].materialIndex = first < mesh.materialIndices.size() ? mesh.materialIndices[first] : 0u;
+	}
+	PhxBVH* bvh = nullptr;
+	if (!record(phx_bvh_build(tris.data(), datas.data(), uint32_t(nt), 0, &bvh))) return nullptr;
+	const PhxBVHNode* nodes = nullptr;
+	const PhxTriangleVertices* v = nullptr;
+	const PhxTriangleData* d = nullptr;
+	uint32_t nn = 0, nr = 0;
+	PhxScene* s = nullptr;
+	if (record(phx_bvh_arrays(bvh, &nodes, &nn, &v, &d, &nr)))
+		record(phx_scene_upload(mCtx, nodes, nn, v, d, nr, nullptr, 0, nullptr, 0, &s));
+	phx_bvh_release(bvh);
+	return mStatus == PHX_OK ? s : nullptr;
 }
 
-void HipTracerRenderer::addDynamicMesh(const RawMesh&) noexcept
+// The reference's CUDA backend leaves this a TODO (CudaTracerRenderer.cpp:317-327): here every
+// mesh gets its SBVH once; render() places them.
+void HipTracerRenderer::setDynamicMeshes(const std::vector<RawMesh>& meshes) noexcept
 {
-	mStatus = PHX_ERR_INVALID_ARGUMENT;
-	mError = "HipTracerRenderer::addDynamicMesh: dynamic meshes are not supported";
+	if (!mCtx) return;
+	std::vector<PhxScene*> up;
+	for (const RawMesh& m : meshes) {
+		PhxScene* s = uploadMesh(m);
+		if (!s) {
+			for (PhxScene* u : up) phx_scene_release(u);
+			return;  // error recorded, previous meshes kept
+		}
+		up.push_back(s);
+	}
+	for (PhxScene* m : mMeshes) phx_scene_release(m);  // instance sets keep what they still use
+	mMeshes = up;
+	record(PHX_OK);
+}
+
+// Unimplemented in the reference (CudaTracerRenderer.cpp:412-415).
+void HipTracerRenderer::addDynamicMesh(const RawMesh& mesh) noexcept
+{
+	if (!mCtx) return;
+	PhxScene* s = uploadMesh(mesh);
+	if (s) mMeshes.push_back(s);
 }
 
 void HipTracerRenderer::targetResolutionUpdated() noexcept
@@ -120,7 +185,7 @@ void HipTracerRenderer::targetResolutionUpdated() noexcept
 	mFrame = 0;
 }
 
-RenderResult HipTracerRenderer::render(const std::vector<DynObject>&,
+RenderResult HipTracerRenderer::render(const std::vector<DynObject>& objects,
                                        const std::vector<PhxSphereLight>&) noexcept
 {
 	RenderResult res;
@@ -130,6 +195,26 @@ RenderResult HipTracerRenderer::render(const std::vector<DynObject>&,
 		mError = "HipTracerRenderer::render: needs a static scene and a target resolution";
 		return res;
 	}
+	// The objects' meshes as this frame's instances (objects without a mesh are skipped).
+	std::vector<PhxInstance> inst;
+	for (const DynObject& o : objects) {
+		if (o.meshIndex >= mMeshes.size()) continue;
+		PhxInstance in;
+		in.mesh = mMeshes[o.meshIndex];
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 4; c++) in.objectToWorld[4 * r + c] = o.transform[4 * c + r];
+		inst.push_back(in);
+	}
+	PhxInstanceSet* set = nullptr;
+	if (!inst.empty() &&
+	    !record(phx_instances_build(mCtx, inst.data(), uint32_t(inst.size()), &set)))
+		return res;
+	if (!record(phx_renderer_set_instances(mRenderer, set))) {
+		if (set) phx_instances_release(set);
+		return res;
+	}
+	if (mInstances) phx_instances_release(mInstances);
+	mInstances = set;
 	// generateCameraDef (GenerateRays.cpp:12-35) with the frustum's vertical FOV in radians.
 	PhxCameraDef cam;
 	const float pos[3] = {mCamera.pos.x, mCamera.pos.y, mCamera.pos.z};

@@ -56,8 +56,25 @@ struct RawImage {
 	uint32_t bytesPerPixel = 0;
 	uint32_t pitch = 0;
 };
-struct RawMesh {};    // dynamic meshes are not supported (nor by the reference's CUDA backend)
-struct DynObject {};
+// rendering/RawMesh.hpp:21-37: vertices, a material index per vertex, three indices a triangle.
+struct Vertex {
+	vec3 pos;
+	vec3 normal;
+	float uv[2] = {0.0f, 0.0f};
+};
+struct RawMesh {
+	std::vector<Vertex> vertices;
+	std::vector<uint16_t> materialIndices;
+	std::vector<uint32_t> indices;
+};
+
+// level/DynObject.hpp:16-20: a mesh of setDynamicMeshes placed by `transform`, stored like
+// sfz::mat4 (column-major, elements[column][row]; world = transform * (p, 1)).
+struct DynObject {
+	uint32_t meshIndex = ~0u;
+	float transform[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+	vec3 velocity;
+};
 
 // RenderResult (BaseRenderer.hpp:29-41): the reference returns GL texture names; this backend
 // returns the device address of the RGBA32F output (row-major, renderedRes.x * renderedRes.y).
@@ -98,7 +115,12 @@ protected:
 // CudaTracerRenderer's replacement. The static scene's sphere lights are the lights rendered (as
 // in the reference, whose render() ignores its `lights` argument for the static path). Each
 // render() is one wavefront frame; with TAA on (the default, as GameScreen enables it) frames
-// are jittered by Halton(2,3) and resolved into an FP32 history.
+// are jittered by Halton(2,3) and resolved into an FP32 history. Dynamic meshes (left untraced by
+// the reference's CUDA backend, CudaTracerRenderer.cpp:317-327, :412-415) each get an
+// object-space SBVH once; every render() places the DynObjects' meshes as instances (a top tree
+// rebuilt per frame) that the frame traces besides the static scene. Their normals take the
+// inverse transpose of the object transform (the reference's createDynamicBvh applies the plain
+// transform, DynamicBVH.cpp:152-154).
 class HipTracerRenderer final : public BaseRenderer {
 public:
 	HipTracerRenderer(const HipTracerRenderer&) = delete;
@@ -127,6 +149,9 @@ public:
 	const std::string& lastError() const noexcept { return mError; }
 	PhxContext* context() const noexcept { return mCtx; }
 	const PhxScene* scene() const noexcept { return mScene; }
+	// The uploaded dynamic meshes (object space) and the instance set of the last render().
+	const std::vector<PhxScene*>& dynamicMeshes() const noexcept { return mMeshes; }
+	const PhxInstanceSet* instances() const noexcept { return mInstances; }
 
 protected:
 	void targetResolutionUpdated() noexcept override;
@@ -135,10 +160,13 @@ private:
 	bool record(phx_status st) noexcept;
 	void uploadScene() noexcept;
 	void uploadTextures(PhxScene* scene) noexcept;
+	PhxScene* uploadMesh(const RawMesh& mesh) noexcept;
 
 	PhxContext* mCtx = nullptr;
 	PhxScene* mScene = nullptr;
 	PhxRenderer* mRenderer = nullptr;
+	std::vector<PhxScene*> mMeshes;
+	PhxInstanceSet* mInstances = nullptr;
 	std::vector<PhxMaterial> mMaterials;
 	std::vector<RawImage> mTextures;
 	StaticScene mStatic;

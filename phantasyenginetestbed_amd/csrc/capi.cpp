@@ -793,6 +793,7 @@ phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const Phx
 	if (flags & PHX_TRAVERSAL_MASK)
 		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: reserved flag bits");
 	if (flags & PHX_ASYNC) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_traversal_stats: no PHX_ASYNC");
+	const bool anyhit = (flags & PHX_STATS_ANYHIT) != 0;
 	for (int k = 0; k < 5; k++) counters[k] = 0;
 	if (numRays == 0) return ok();
 	unsigned long long* dCounters = nullptr;
@@ -801,12 +802,14 @@ phx_status phx_traversal_stats(PhxContext* ctx, const PhxScene* scene, const Phx
 	PhxRayHit* dHits = nullptr;
 	if (e == hipSuccess) e = hipMalloc(&dHits, size_t(numRays) * sizeof(PhxRayHit));
 	SceneView sv = view(scene);
-	const TraceLaunch L = traceLaunch(ctx, flags, false);
+	const TraceLaunch L = traceLaunch(ctx, flags & ~PHX_STATS_ANYHIT, false);
 	if (e == hipSuccess) {
 		st = castCommon(
 			ctx, rays, numRays, dHits, sizeof(PhxRayHit), (flags & PHX_MEM_DEVICE),
 			[&](const PhxRayIn* r, void*) {
-				return launchClosest(sv, r, numRays, dHits, L, dCounters, ctx->stream);
+				return anyhit ? launchAnyHitCounted(sv, r, numRays, reinterpret_cast<uint8_t*>(dHits),
+				                                    dCounters, ctx->stream)
+				              : launchClosest(sv, r, numRays, dHits, L, dCounters, ctx->stream);
 			},
 			"phx_traversal_stats");
 	}

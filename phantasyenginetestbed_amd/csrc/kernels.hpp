@@ -100,6 +100,11 @@ hipError_t launchInstances(const PhxBVHNode* top, const DevInstance* inst, uint3
                            const PhxRayIn* rays, uint32_t n, PhxRayHit* hits, uint32_t* instOut,
                            uint8_t* inLight, bool seeded, hipStream_t stream);
 
+// Any hit of every ray (one ray per lane) adding castDebugRay-style counters of the any-hit
+// traversal (trianglesIntersected = rays occluded) to counters[0..4].
+hipError_t launchAnyHitCounted(const SceneView& s, const PhxRayIn* rays, uint32_t n,
+                               uint8_t* inLight, unsigned long long* counters, hipStream_t stream);
+
 // Closest hit in the CPU castRay visiting order. If `counters` is not null the castDebugRay
 // counters of the batch are added to counters[0..4] (device memory).
 // Counts the floats a (exhaustive: all of 2^-126 <= |a| < 2^126; else every significand at four

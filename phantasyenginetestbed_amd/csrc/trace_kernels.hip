@@ -158,10 +158,29 @@ __device__ __forceinline__ void closest_one(const float4* __restrict__ nodes,
 	}
 }
 
+// Any-hit leaf that counts its triangle tests (up to and including the first hit).
+__device__ __forceinline__ bool anyhit_leaf_counted(const dev::Ray& r,
+                                                    const float4* __restrict__ tris,
+                                                    uint32_t first, int32_t count, Counts& k)
+{
+	for (int32_t j = 0; j < count; j++) {
+		const float4* p = tris + 3u * (first + uint32_t(j));
+		float t, u, v;
+		k.triTests++;
+		if (dev::triangle_pre(r, p[0], p[1], p[2], t, u, v) && r.tMin <= t && t < r.tMax) {
+			k.accepted++;
+			return true;
+		}
+	}
+	return false;
+}
+
 // Any hit of one ray: castRay's traversal with tCurrMax = tMax, stopping at the first triangle
-// with tMin <= t < tMax.
+// with tMin <= t < tMax. kCount adds castDebugRay-style counters of that traversal to k.
+template <bool kCount>
 __device__ __forceinline__ bool anyhit_one(const float4* __restrict__ nodes,
-                                           const float4* __restrict__ tris, const dev::Ray& r)
+                                           const float4* __restrict__ tris, const dev::Ray& r,
+                                           Counts& k)
 {
 	uint32_t stack[kSimpleStack];
 	stack[0] = 0u;
@@ -170,23 +189,28 @@ __device__ __forceinline__ bool anyhit_one(const float4* __restrict__ nodes,
 	while (sp > 0 && !occluded) {
 		sp--;
 		const NodeWords nd = load_node(nodes, stack[sp]);
+		if (kCount) k.nodes++;
 		float inL, outL, inR, outR;
 		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
 		dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
 		const bool visitL = (outL >= inL) && outL > r.tMin && inL < r.tMax;
 		const bool visitR = (outR >= inR) && outR > r.tMin && inR < r.tMax;
 		if (visitL) {
+			if (kCount) k.boxHits++;
 			if (nd.w3.z == 0) {
 				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.x);
 			} else {
-				occluded = anyhit_leaf(r, tris, child_index(nd.w3.x), nd.w3.z);
+				occluded = kCount ? anyhit_leaf_counted(r, tris, child_index(nd.w3.x), nd.w3.z, k)
+				                  : anyhit_leaf(r, tris, child_index(nd.w3.x), nd.w3.z);
 			}
 		}
 		if (visitR && !occluded) {
+			if (kCount) k.boxHits++;
 			if (nd.w3.w == 0) {
 				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.y);
 			} else {
-				occluded = anyhit_leaf(r, tris, child_index(nd.w3.y), nd.w3.w);
+				occluded = kCount ? anyhit_leaf_counted(r, tris, child_index(nd.w3.y), nd.w3.w, k)
+				                  : anyhit_leaf(r, tris, child_index(nd.w3.y), nd.w3.w);
 			}
 		}
 	}
@@ -209,15 +233,19 @@ __global__ __launch_bounds__(kBlock) void k_closest_simple(
 	if (kCount) add_counts(counters, k);
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restrict__ nodes,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ rays,
-                                                          uint8_t* __restrict__ inLight, uint32_t n)
+                                                          uint8_t* __restrict__ inLight, uint32_t n,
+                                                          unsigned long long* __restrict__ counters)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
-	inLight[i] = anyhit_one(nodes, tris, r) ? 0 : 1;
+	Counts k;
+	inLight[i] = anyhit_one<kCount>(nodes, tris, r, k) ? 0 : 1;
+	if (kCount) add_counts(counters, k);
 }
 
 // ---- instances (two-level traversal) ------------------------------------------------------------
@@ -289,7 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 				const uint32_t id = in.index;
 				const dev::Ray o = object_ray(in, r);
 				if (kAnyHit) {
-					occluded = anyhit_one(in.nodes, in.tris, o);
+					Counts none;
+					occluded = anyhit_one<false>(in.nodes, in.tris, o, none);
 				} else {
 					Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
 					Counts none;
@@ -1785,6 +1814,17 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	return hipGetLastError();
 }
 
+hipError_t launchAnyHitCounted(const SceneView& s, const PhxRayIn* rays, uint32_t n,
+                               uint8_t* inLight, unsigned long long* counters, hipStream_t stream)
+{
+	if (n == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_anyhit_simple<true>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, stream,
+	                   reinterpret_cast<const float4*>(s.nodes),
+	                   reinterpret_cast<const float4*>(s.trisPre),
+	                   reinterpret_cast<const float4*>(rays), inLight, n, counters);
+	return hipGetLastError();
+}
+
 hipError_t launchInstances(const PhxBVHNode* top, const DevInstance* inst, uint32_t numInst,
                           const PhxRayIn* rays, uint32_t n, PhxRayHit* hits, uint32_t* instOut,
                           uint8_t* inLight, bool seeded, hipStream_t stream)
@@ -1816,8 +1856,8 @@ hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, ui
 		return hipGetLastError();
 	}
 	if (L.kernel == TraceKernel::Simple || n > 0x7FFFFFFFu) {
-		hipLaunchKernelGGL(k_anyhit_simple, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, stream,
-		                   nodes, tris, r4, inLight, n);
+		hipLaunchKernelGGL(k_anyhit_simple<false>, dim3(gridFor(n, kBlock)), dim3(kBlock), 0,
+		                   stream, nodes, tris, r4, inLight, n, nullptr);
 		return hipGetLastError();
 	}
 	uint32_t grid, block, top, lds;

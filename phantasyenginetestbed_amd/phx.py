@@ -47,6 +47,7 @@ BATCH_ANYHIT = 1
 PLAN_TRACK_COMPLETION = 0x10
 PLAN_USE_GRAPH = 0x20
 AUTOTUNE_ANYHIT = 0x1000
+STATS_ANYHIT = 0x4000
 CHECK_EXHAUSTIVE = 0x1
 SERVE_PLAIN = 0x1
 FRAME_TAA = 0x10000

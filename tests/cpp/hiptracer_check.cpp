@@ -110,9 +110,90 @@ extern "C" int hiptracer_check_run()
 		CHECK(sum > 0.0 && std::isfinite(sum), "frame %u is black or not finite", f);
 	}
 
-	// Unsupported engine calls record an error and change nothing.
-	renderer.setDynamicMeshes({RawMesh{}});
-	CHECK(renderer.lastStatus() == PHX_ERR_INVALID_ARGUMENT, "dynamic meshes");
+	// Dynamic meshes: two meshes (a unit quad facing up and a small box), three objects placed
+	// by sfz-layout transforms; each frame equals phx_render_frame with the same instances set on
+	// a renderer of the C ABI, and differs from the frame without them.
+	std::printf("hiptracer_check: dynamic meshes\n");
+	{
+		RawMesh quad;
+		const float qp[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}};
+		for (int k = 0; k < 4; k++) {
+			Vertex v;
+			v.pos = vec3{qp[k][0], qp[k][1], qp[k][2]};
+			v.normal = vec3{0, 1, 0};
+			quad.vertices.push_back(v);
+			quad.materialIndices.push_back(uint16_t(k % 3));
+		}
+		quad.indices = {0, 1, 2, 0, 2, 3};
+		RawMesh box;
+		for (int k = 0; k < 8; k++) {
+			Vertex v;
+			v.pos = vec3{float(k & 1), float((k >> 1) & 1), float((k >> 2) & 1)};
+			v.normal = normalized(v.pos.x - 0.5, v.pos.y - 0.5, v.pos.z - 0.5);
+			box.vertices.push_back(v);
+			box.materialIndices.push_back(uint16_t(5));
+		}
+		box.indices = {0, 1, 3, 0, 3, 2, 4, 6, 7, 4, 7, 5, 0, 4, 5, 0, 5, 1,
+		               2, 3, 7, 2, 7, 6, 0, 2, 6, 0, 6, 4, 1, 5, 7, 1, 7, 3};
+		RawMesh bad;
+		bad.vertices.resize(2);
+		bad.indices = {0, 1, 7};
+		renderer.setDynamicMeshes({quad, bad});
+		CHECK(renderer.lastStatus() == PHX_ERR_INVALID_SCENE && renderer.dynamicMeshes().empty(),
+		      "a mesh with an out-of-range index is refused");
+		renderer.setDynamicMeshes({quad});
+		renderer.addDynamicMesh(box);
+		CHECK(renderer.lastStatus() == PHX_OK && renderer.dynamicMeshes().size() == 2,
+		      "dynamic meshes: %s", renderer.lastError().c_str());
+		std::vector<DynObject> objs(4);
+		const float s0 = 0.3f, s1 = 0.08f;
+		// column-major: transform[4 * column + row]
+		objs[0].meshIndex = 0;
+		objs[0].transform[0] = s0, objs[0].transform[5] = 1.0f, objs[0].transform[10] = s0;
+		objs[0].transform[12] = 0.35f, objs[0].transform[13] = 0.12f, objs[0].transform[14] = 0.3f;
+		objs[1].meshIndex = 1;
+		objs[1].transform[0] = s1, objs[1].transform[5] = 2 * s1, objs[1].transform[10] = s1;
+		objs[1].transform[12] = 0.5f, objs[1].transform[13] = 0.05f, objs[1].transform[14] = 0.5f;
+		objs[2] = objs[1];
+		objs[2].transform[12] = 0.62f;
+		objs[3].meshIndex = 7;  // no such mesh: skipped
+		PhxRenderer *dyn = nullptr, *still = nullptr;
+		CHECK(phx_renderer_create(ctx, w, h, &dyn) == PHX_OK, "dyn renderer");
+		CHECK(phx_renderer_create(ctx, w, h, &still) == PHX_OK, "renderer without instances");
+		renderer.resetTaa();
+		for (uint32_t f = 0; f < 3; f++) {
+			objs[1].transform[13] += 0.01f;  // one object moves
+			RenderResult res = renderer.render(objs, {});
+			CHECK(res.colorDevice, "dynamic render %u: %s", f, renderer.lastError().c_str());
+			CHECK(phx_copy_to_host(ctx, a.data(), res.colorDevice, a.size() * 4) == PHX_OK, "copy");
+			std::vector<PhxInstance> inst(3);
+			for (int i = 0; i < 3; i++) {
+				inst[i].mesh = renderer.dynamicMeshes()[objs[i].meshIndex];
+				for (int r = 0; r < 3; r++)
+					for (int c = 0; c < 4; c++)
+						inst[i].objectToWorld[4 * r + c] = objs[i].transform[4 * c + r];
+			}
+			PhxInstanceSet* set = nullptr;
+			CHECK(phx_instances_build(ctx, inst.data(), 3, &set) == PHX_OK, "instances");
+			CHECK(phx_renderer_set_instances(dyn, set) == PHX_OK, "set instances");
+			const uint32_t flags = PHX_FRAME_TAA | (f == 0 ? PHX_FRAME_TAA_RESET : 0u);
+			CHECK(phx_render_frame(ctx, dyn, renderer.scene(), &def, f, 11, flags, b.data()) ==
+			          PHX_OK, "direct dynamic frame %u: %s", f, phx_last_error());
+			CHECK(std::memcmp(a.data(), b.data(), a.size() * 4) == 0, "dynamic frame %u differs", f);
+			phx_instances_release(set);  // released while set: the next frame must refuse it
+			CHECK(phx_render_frame(ctx, dyn, renderer.scene(), &def, f, 11, flags, nullptr) ==
+			          PHX_ERR_INVALID_HANDLE, "a released instance set was used");
+			CHECK(phx_renderer_set_instances(dyn, nullptr) == PHX_OK, "clear instances");
+			std::vector<float> c(a.size());
+			CHECK(phx_render_frame(ctx, still, renderer.scene(), &def, f, 11, flags, c.data()) ==
+			          PHX_OK, "frame without instances");
+			CHECK(std::memcmp(a.data(), c.data(), a.size() * 4) != 0, "instances had no effect");
+		}
+		phx_renderer_release(dyn);
+		phx_renderer_release(still);
+		renderer.render({}, {});  // no objects: back to the static scene alone
+		CHECK(renderer.lastStatus() == PHX_OK && renderer.instances() == nullptr, "no objects");
+	}
 
 	// Material textures: setMaterialsAndTextures + addTexture give the frames of a scene uploaded
 	// and textured through the C ABI, which differ from the untextured scene's frames.

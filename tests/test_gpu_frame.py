@@ -223,13 +223,20 @@ def test_frame_graph_replays_match_direct_frames(ctx, small):
             bits_equal(got, want, f"graph replay, frame {f}")
             ctx.cast_closest(scene, rays)  # eager work between replays
             ctx.synchronize()
-        # the replayed TAA sequence after the reset at frame 7 is the oracle's
-        hist, ids, cam = None, O.pixel_ids(w, h), cams[1]
+        # the replayed TAA sequence after the reset at frame 7 is the oracle's; frame 7's
+        # velocity is against frame 6's camera, and the frames after it see that velocity as
+        # the history's
+        hist, vel_prev, ids = None, None, O.pixel_ids(w, h)
         for f in range(7, 12):
             jx, jy = phx.halton23(f)
+            cam, prev = cams[1], cams[0] if f == 7 else cams[1]
             color, gbuf, _ = O.frame(bvh.nodes, bvh.tris, bvh.datas, mats, lts, cam.as_array(), w,
                                      h, jx, jy, SEED, f)
-            hist, ref = O.taa(color, hist, gbuf["roughness"][ids], reset=(f == 7), clamp=True)
+            prim = O.primary_rays_jittered(cam.as_array(), w, h, jx, jy)
+            vel = O.velocity(cam.as_array(), prev.as_array(), gbuf, prim, w, h)
+            hist, ref = O.taa(color, hist, gbuf["roughness"][ids], reset=(f == 7), clamp=True,
+                              vel_in=vel_prev, vel=vel)
+            vel_prev = vel
         bits_equal(got, ref, "graph replay vs oracle TAA")
     finally:
         direct.release()

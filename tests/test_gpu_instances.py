@@ -67,7 +67,14 @@ def check(ctx, specs, rays, scene=None):
 def test_instances_match_oracle(ctx, meshes, count):
     say(f"[gpu-inst] {count} instances")
     specs = placed(meshes, count, seed=count)
-    rays = random_rays(np.random.default_rng(count), 30000, lo=-9, hi=9, aim=(-6, 6))
+    rng = np.random.default_rng(count)
+    rays = random_rays(rng, 30000, lo=-9, hi=9, aim=(-6, 6))
+    # aim each ray near the centre of a random instance so that even one instance is hit often
+    centres = np.array([m[:, :3].astype(np.float64) @ np.full(3, 0.5) + m[:, 3]
+                        for _, _, m in specs])
+    aim = centres[rng.integers(0, count, len(rays))] + rng.uniform(-0.8, 0.8, (len(rays), 3))
+    d = aim - rays["o"]
+    rays["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
     rays["tmax"][::3] = np.float32(5.0)  # finite tMax for a third of the rays
     hits, ids, lit = check(ctx, specs, rays)
     hit = hits["tri"] != phx.MISS

@@ -526,3 +526,20 @@ def test_autotuners_apply_their_shapes(ctx, c2):
         rb.free()
         hb.free()
         restore_launch(ctx, old)
+
+
+@pytest.mark.parametrize("name", ["rand2k", "hf32", "edges"])
+def test_anyhit_counters_match_oracle(ctx, name):
+    """PHX_STATS_ANYHIT: the counters of the any-hit traversal equal the oracle's."""
+    fx = load_fixture(name)
+    s = ctx.upload_scene(fx["nodes"], fx["tris"])
+    try:
+        for tmax in (np.float32(0.3), np.float32(phx.FLT_MAX)):
+            rays = fx["rays"].copy()
+            rays["tmax"] = tmax
+            st = ctx.traversal_stats(s, rays, phx.STATS_ANYHIT)
+            lit, ref = O.anyhit_debug(fx["nodes"], fx["tris"], rays)
+            assert st == ref
+            assert st["trianglesIntersected"] == int((lit == 0).sum())
+    finally:
+        s.release()
