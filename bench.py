@@ -74,13 +74,23 @@ class HipEvents:
 PROFILE = ROOT / "profiles" / "closest_persistent_bounce1M.md"
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the closest-hit kernel on this workload, from the committed rocprofv3
-    FETCH_SIZE + WRITE_SIZE summary (scaled by a plain copy's readings); None without it."""
+def shape_key(launch: dict) -> str:
+    """grid,block,treelet,refill,helpers,raysPerFetch,ldsStack of a launch shape."""
+    return ",".join(str(launch[k]) for k in ("gridBlocksClosest", "blockThreads", "topNodes",
+                                              "refillMin", "helperWaves", "raysPerFetch",
+                                              "stackLdsEntries"))
+
+
+def profiled_traffic(key: str):
+    """HBM bytes per launch of the closest-hit kernel on this workload under launch shape `key`,
+    from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a plain copy's
+    readings): lines `traffic_bytes_per_launch[<shape>]: <bytes>`. None if that shape was not
+    profiled."""
     try:
         for line in PROFILE.read_text().splitlines():
-            if "traffic_bytes_per_launch:" in line:
-                return int(line.split("traffic_bytes_per_launch:")[1].split(")")[0].strip())
+            tag = f"traffic_bytes_per_launch[{key}]:"
+            if tag in line:
+                return int(line.split(tag)[1].split()[0].strip(" )`"))
     except OSError:
         pass
     return None
@@ -161,7 +171,7 @@ def set_shape(ctx, p: dict) -> None:
 
 
 def batched_form(ctx, scene, pools, ev, reps: int = 5, plan_launch=None) -> dict:
-    from phantasyenginetestbed_amd import workloads as W
+    from phantasyenginetestbed_amd import phx, workloads as W
     bs = W.batch_set(pools)
     W.upload_batches(ctx, bs)
     stream = ctx.stream
@@ -233,6 +243,18 @@ def batched_form(ctx, scene, pools, ev, reps: int = 5, plan_launch=None) -> dict
 
     fused_ms = timed(False)
     fused_lat = plan.batch_times_ms()
+    # algorithmic bytes of the fused launch: per batch, rays x (32 + out + 64 N_node + 48 N_tri)
+    # with N from the traversal counters of an evenly strided 1/64 of the batch (closest-hit
+    # counters for closest batches, any-hit traversal counters for shadow batches)
+    algo = 0.0
+    for k, r in zip(bs.kinds, bs.rays):
+        sample = np.ascontiguousarray(r[::64])
+        anyhit = k == "shadow"
+        st = ctx.traversal_stats(scene, sample, phx.STATS_ANYHIT if anyhit else 0)
+        per = 32 + (1 if anyhit else 16) + (64 * st["nodesVisited"] +
+                                            48 * st["triangleIntersectionTests"]) / len(sample)
+        algo += per * len(r)
+    fused_gbs = algo / (fused_ms / 1e3) / 1e9
     same_fused = all(a.tobytes() == b.tobytes() for a, b in zip(ref, W.download_outputs(bs)))
     graph_ms = timed(True)
     same_graph = all(a.tobytes() == b.tobytes() for a, b in zip(ref, W.download_outputs(bs)))
@@ -252,6 +274,12 @@ def batched_form(ctx, scene, pools, ev, reps: int = 5, plan_launch=None) -> dict
         "fused_batch_completion_p50_ms": round(float(np.percentile(fused_lat, 50)), 4),
         "fused_batch_completion_p99_ms": round(float(np.percentile(fused_lat, 99)), 4),
         "fused_vs_plain": round(plain_best / fused_ms, 2),
+        "fused_roofline": {"bound": "hbm", "achieved": round(fused_gbs, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(fused_gbs / HBM_PEAK_GBS, 4),
+                           "algorithmic_bytes": int(algo),
+                           "note": "per-kind node/triangle counts on a strided 1/64 of every "
+                                   "batch"},
         "plain_shape": plain_shape,
         "identical_outputs": bool(same_fused and same_graph),
         "fused_launch": {k: tuned[k] for k in ("gridBlocksClosest", "blockThreads", "topNodes",
@@ -306,8 +334,10 @@ def frame_form(ctx, scene, ev, lights: int, frames: int = 16) -> dict:
 def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int = 3) -> dict:
     """Poisson-arriving requests against 4 resident scenes through the request server, at
     saturation (all requests at once; after a warm-up, the median of 5 runs by makespan) and at
-    25/50/90% of the saturated request rate (one run each), beside the plain per-batch server on
-    the same requests (median of 5 saturated runs; identical outputs)."""
+    25/50/90% of the saturated request rate (3 runs each with their own arrival draws; medians of
+    the runs' rate, p50 and p99), beside the plain per-batch server on the same requests (median of
+    5 saturated runs; identical outputs). The same requests with device-resident rays and outputs
+    (no PCIe copies) show the engine's own serving rate."""
     from phantasyenginetestbed_amd import phx, workloads as W
     t0 = time.perf_counter()
     scenes = [W.served_scene(ctx, name) for name in ("100k", "1M", "4M", "8M")]
@@ -318,8 +348,8 @@ def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int
     reqs = W.server_requests(scenes, rs)
     total = rs.total
 
-    def run(server, arrivals):
-        lat, mk = server.replay(reqs, arrivals)
+    def run(server, arrivals, which=None):
+        lat, mk = server.replay(which or reqs, arrivals)
         return {"mrays": round(total / mk / 1e3, 1), "makespan_ms": round(mk, 2),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(float(np.percentile(lat, 99)), 3)}
@@ -329,8 +359,9 @@ def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int
         srv.add_scene(s.scene)
     zeros = np.zeros(n_requests)
 
-    def median_run(server, reps=5):
-        runs = sorted((run(server, zeros) for _ in range(reps)), key=lambda r: r["makespan_ms"])
+    def median_run(server, reps=5, which=None):
+        runs = sorted((run(server, zeros, which) for _ in range(reps)),
+                      key=lambda r: r["makespan_ms"])
         return runs[len(runs) // 2]
 
     run(srv, zeros)  # warm-up
@@ -339,10 +370,24 @@ def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int
     rate = n_requests / sat["makespan_ms"]  # requests per ms at saturation
     rates = {}
     for frac in (0.25, 0.5, 0.9):
-        arr = W.poisson_arrivals(n_requests, frac * rate, seed=int(frac * 100))
+        runs = [run(srv, W.poisson_arrivals(n_requests, frac * rate, seed=int(frac * 100) + 7 * k))
+                for k in range(3)]
+        med = {key: float(np.median([r[key] for r in runs]))
+               for key in ("mrays", "makespan_ms", "p50_ms", "p99_ms")}
         rates[f"{int(frac * 100)}%"] = {"offered_req_per_s": round(frac * rate * 1e3, 1),
-                                        **run(srv, arr)}
+                                        **{k: round(v, 3) for k, v in med.items()},
+                                        "runs_p50_ms": [r["p50_ms"] for r in runs],
+                                        "runs_p99_ms": [r["p99_ms"] for r in runs]}
     st = srv.stats()
+    # device-resident requests: the same mix with rays and outputs in HBM
+    dreqs, keep, dout = W.server_requests_device(ctx, scenes, rs)
+    run(srv, zeros, dreqs)  # warm-up
+    dsat = median_run(srv, which=dreqs)
+    drate = n_requests / dsat["makespan_ms"]
+    dpois = run(srv, W.poisson_arrivals(n_requests, 0.9 * drate, seed=90), dreqs)
+    same_device = bool(np.array_equal(dout.download(np.uint8, len(served_out)), served_out))
+    for b in keep:
+        b.free()
     srv.destroy()
 
     plain = phx.Server(ctx, int(budget_gb * (1 << 30)), slots=1, plain=True)
@@ -359,6 +404,10 @@ def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int
             "rays": total, "budget_gb": budget_gb, "slots": slots,
             "served_saturated": sat, "served_at_rates": rates, "plain_saturated": psat,
             "served_vs_plain": round(sat["mrays"] / psat["mrays"], 2),
+            "device_resident": {"saturated": dsat, "at_90%": {
+                "offered_req_per_s": round(0.9 * drate * 1e3, 1), **dpois},
+                "identical_outputs": same_device,
+                "note": "rays and outputs in HBM: the engine's serving rate without PCIe copies"},
             "identical_outputs": same, "launches": int(st["launches"]),
             "peak_request_bytes": int(st["peakBytesInUse"]),
             "note": "host-resident (pinned) request buffers: times include PCIe copies"}
@@ -467,6 +516,14 @@ def main() -> int:
     gpu_hits = out.download(phx.HIT, n)
     value = n / (ms_per_step / 1e3) / 1e6
     achieved = bytes_per_ray * n / (ms_per_step / 1e3) / 1e9
+    key = shape_key(launch)
+    traffic = profiled_traffic(key) if kernel == "persistent" else None
+    if traffic is not None:
+        traffic_source = (f"rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of this launch shape "
+                          f"[{key}] (grid,block,treelet,refill,helpers,raysPerFetch,ldsStack; "
+                          "profiles/closest_persistent_bounce1M.md)")
+    else:
+        traffic_source = f"no rocprofv3 profile of this launch shape [{key}]"
 
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
@@ -487,11 +544,8 @@ def main() -> int:
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic() if kernel == "persistent" else None,
-                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, launch 256 "
-                                       "blocks x (512 threads + 4 helper waves), no treelet, "
-                                       "refill 8, 128 rays per fetch, 16 LDS stack entries "
-                                       "(profiles/closest_persistent_bounce1M.md)",
+                     "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_batched:

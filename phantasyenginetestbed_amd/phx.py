@@ -815,6 +815,9 @@ class Context:
     def renderer(self, width: int, height: int) -> Renderer:
         return Renderer(self, width, height)
 
+    def copy_to_device(self, dst: int, src: int, nbytes: int) -> None:
+        check(lib().phx_copy_to_device(self.handle, dst, src, nbytes))
+
     def copy_on_device(self, dst: int, src: int, nbytes: int) -> None:
         check(lib().phx_copy_on_device(self.handle, dst, src, nbytes))
 

@@ -247,6 +247,34 @@ def server_requests(scenes: list, rs: RequestSet) -> list:
     return reqs
 
 
+def server_requests_device(ctx: phx.Context, scenes: list, rs: RequestSet):
+    """The same requests with device-resident rays and outputs (PHX_MEM_DEVICE): every scene's
+    ray pools copied to the device once, one device buffer for all outputs. Returns (requests,
+    buffers to free, output buffer)."""
+    pools = {}
+    keep = []
+    for si, s in enumerate(scenes):
+        for kind, hb in s.pools.items():
+            db = ctx.alloc(hb.array.nbytes)
+            ctx.copy_to_device(db.ptr, hb.ptr, hb.array.nbytes)
+            pools[(si, kind)] = db
+            keep.append(db)
+    per = np.array([1 if k == "shadow" else 16 for k in rs.kinds], np.int64)
+    nbytes = (rs.sizes * per + 255) // 256 * 256
+    offs = np.concatenate([[0], np.cumsum(nbytes)[:-1]])
+    out = ctx.alloc(int(nbytes.sum()))
+    keep.append(out)
+    reqs = []
+    for i in range(len(rs.sizes)):
+        si = int(rs.scene_index[i])
+        kind = phx.BATCH_ANYHIT if rs.kinds[i] == "shadow" else phx.BATCH_CLOSEST
+        reqs.append(phx.Server.request(scenes[si].scene,
+                                       pools[(si, rs.kinds[i])].ptr + int(rs.offsets[i]) * 32,
+                                       out.ptr + int(offs[i]), int(rs.sizes[i]), kind,
+                                       device=True))
+    return reqs, keep, out
+
+
 def request_output(rs: RequestSet, i: int) -> np.ndarray:
     n = int(rs.sizes[i])
     o = int(rs.out_offsets[i])
