@@ -72,6 +72,9 @@ class HipEvents:
 
 
 PROFILE = ROOT / "profiles" / "closest_persistent_bounce1M.md"
+FUSED_PROFILE = ROOT / "profiles" / "trace_batches_fused256_1M.md"
+# nominal L2 bandwidth: 8 XCDs x 16 channels x 128 B per clock at 2.4 GHz
+L2_NOMINAL_GBS = 8 * 16 * 128 * 2.4
 
 
 def shape_key(launch: dict) -> str:
@@ -81,19 +84,25 @@ def shape_key(launch: dict) -> str:
                                               "stackLdsEntries"))
 
 
+def profile_line(tag: str, path=PROFILE):
+    """Text after `tag` on the first line of a committed rocprofv3 summary that holds it, or
+    None."""
+    try:
+        for line in path.read_text().splitlines():
+            if tag in line:
+                return line.split(tag, 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def profiled_traffic(key: str):
     """HBM bytes per launch of the closest-hit kernel on this workload under launch shape `key`,
     from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a plain copy's
     readings): lines `traffic_bytes_per_launch[<shape>]: <bytes>`. None if that shape was not
     profiled."""
-    try:
-        for line in PROFILE.read_text().splitlines():
-            tag = f"traffic_bytes_per_launch[{key}]:"
-            if tag in line:
-                return int(line.split(tag)[1].split()[0].strip(" )`"))
-    except OSError:
-        pass
-    return None
+    text = profile_line(f"traffic_bytes_per_launch[{key}]:")
+    return int(text.split()[0].strip(" )`")) if text else None
 
 
 def cpu_baseline(nodes, tris, rays, gpu_hits, budget_s: float = 24.0) -> dict:
@@ -274,12 +283,16 @@ def batched_form(ctx, scene, pools, ev, reps: int = 5, plan_launch=None) -> dict
         "fused_batch_completion_p50_ms": round(float(np.percentile(fused_lat, 50)), 4),
         "fused_batch_completion_p99_ms": round(float(np.percentile(fused_lat, 99)), 4),
         "fused_vs_plain": round(plain_best / fused_ms, 2),
-        "fused_roofline": {"bound": "hbm", "achieved": round(fused_gbs, 1),
-                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(fused_gbs / HBM_PEAK_GBS, 4),
+        "fused_roofline": {"bound": "issue", "achieved": round(fused_gbs, 1), "unit": "GB/s",
+                           "hbm_peak": HBM_PEAK_GBS,
+                           "frac_of_hbm_peak": round(fused_gbs / HBM_PEAK_GBS, 4),
+                           "l2_nominal_peak": round(L2_NOMINAL_GBS, 1),
+                           "frac_of_l2_nominal": round(fused_gbs / L2_NOMINAL_GBS, 4),
                            "algorithmic_bytes": int(algo),
-                           "note": "per-kind node/triangle counts on a strided 1/64 of every "
-                                   "batch"},
+                           "limiter": profile_line("limiter:", FUSED_PROFILE),
+                           "note": "lane-visit bytes (per-kind node/triangle counts on a strided "
+                                   "1/64 of every batch); coherent primary batches share node "
+                                   "loads in L1/L2, so they can exceed what HBM delivers"},
         "plain_shape": plain_shape,
         "identical_outputs": bool(same_fused and same_graph),
         "fused_launch": {k: tuned[k] for k in ("gridBlocksClosest", "blockThreads", "topNodes",
@@ -546,6 +559,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_source": traffic_source,
+                     "limiter": profile_line("limiter:"),
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_batched:

@@ -572,7 +572,11 @@ typedef struct PhxServerStats {
  * admitted requests are coalesced, in order, into the next fused persistent launch on an idle
  * slot, up to 4M rays per launch (a larger request gets a launch of its own), so a backlog spreads
  * over the slots and one launch's copies overlap another's traversal; a launch formed while
- * another runs fills that launch's tail. A dispatcher thread owns the device work. The server
+ * another runs fills that launch's tail. Slot 0 launches on the context's stream (other work on
+ * that stream runs in order with it); each further slot has a stream of its own. HIP spreads a
+ * process's streams over a few hardware queues (4 by default) and slots that share a queue
+ * serialise, so more than 3 slots only pay where more queues exist. A dispatcher thread owns the
+ * device work. The server
  * launches with the default stack split and rays per fetch and no treelet or helper waves; it
  * takes the block size and refill threshold from the context. */
 phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t slots,

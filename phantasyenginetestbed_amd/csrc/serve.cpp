@@ -108,6 +108,7 @@ struct Request {
 
 struct Slot {
 	hipStream_t stream = nullptr;
+	bool ownStream = false;  // slot 0 launches on the context's stream
 	uint32_t* work = nullptr;
 	uint32_t* spill = nullptr;
 	BatchDesc* descs = nullptr;
@@ -334,7 +335,7 @@ void destroyServer(PhxServer* s)
 		if (slot.hostDescs) hipHostFree(slot.hostDescs);
 		if (slot.hostOffsets) hipHostFree(slot.hostOffsets);
 		if (slot.done) hipEventDestroy(slot.done);
-		if (slot.stream) hipStreamDestroy(slot.stream);
+		if (slot.stream && slot.ownStream) hipStreamDestroy(slot.stream);
 	}
 	hipFree(s->pool);
 	for (auto& kv : s->byTicket) delete kv.second;
@@ -379,8 +380,18 @@ phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t num
 	s->shape.spillLanes = s->shape.gridBlocks * s->shape.blockThreads;
 	s->slots.resize(numSlots);
 	hipError_t e = hipSuccess;
-	for (Slot& slot : s->slots) {
-		if (e == hipSuccess) e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking);
+	// Slot 0 launches on the context's stream, the others on streams of their own. HIP maps a
+	// process's streams onto a few hardware queues (4 by default), and two slots on one queue run
+	// one after the other: the null stream and the context's stream already hold two, so only this
+	// way do 3 slots get queues of their own in a process that has created no other streams.
+	for (size_t i = 0; i < s->slots.size(); i++) {
+		Slot& slot = s->slots[i];
+		if (i == 0) {
+			slot.stream = ctx->stream;
+		} else if (e == hipSuccess) {
+			e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking);
+			slot.ownStream = e == hipSuccess;
+		}
 		if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&slot.work), 64);
 		if (e == hipSuccess) e = hipMemset(slot.work, 0, 64);
 		if (e == hipSuccess)
