@@ -552,9 +552,11 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 		s->rootMin[k] = std::min(lmn[k], rmn[k]);
 		s->rootMax[k] = std::max(lmx[k], rmx[k]);
 	}
+	// Bounds within +-2^126 (false for inf and NaN) keep every b - o of a ray whose origin is in the
+	// same range finite: the condition of the kernels' NaN-free slab test.
 	for (uint32_t i = 0; i < numNodes && s->finiteBounds; i++)
 		for (int k = 0; k < 12; k++)
-			if (!std::isfinite(nodes[i].fData[k])) {
+			if (!(std::fabs(nodes[i].fData[k]) <= 0x1p126f)) {
 				s->finiteBounds = 0;
 				break;
 			}

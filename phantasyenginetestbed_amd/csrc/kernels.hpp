@@ -16,7 +16,7 @@ struct SceneView {
 	const PhxTriangleVertices* trisPre;
 	uint32_t numNodes;
 	uint32_t numTris;
-	uint32_t finiteBounds;  // every node bound is finite (enables the NaN-free slab path)
+	uint32_t finiteBounds;  // every node bound within +-2^126 (enables the NaN-free slab path)
 	uint32_t topCount;      // nodes [0, topCount) are the top levels in breadth-first order
 };
 

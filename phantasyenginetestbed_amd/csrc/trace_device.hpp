@@ -36,8 +36,11 @@ __device__ __forceinline__ Ray make_ray(const float4 a, const float4 b)
 	r.ix = 1.0f / b.x;
 	r.iy = 1.0f / b.y;
 	r.iz = 1.0f / b.z;
+	// |o| <= 2^126 and node bounds within the same range (finiteBounds) keep b - o finite, so
+	// (b - o) * (1/d) is never NaN, also where 1/d is 0 (an infinite direction).
 	r.nanFree = __builtin_isfinite(r.ix) && __builtin_isfinite(r.iy) && __builtin_isfinite(r.iz) &&
-	            __builtin_isfinite(r.ox) && __builtin_isfinite(r.oy) && __builtin_isfinite(r.oz);
+	            __builtin_fabsf(r.ox) <= 0x1p126f && __builtin_fabsf(r.oy) <= 0x1p126f &&
+	            __builtin_fabsf(r.oz) <= 0x1p126f;
 	return r;
 }
 
@@ -86,8 +89,9 @@ __device__ __forceinline__ float vmax3(float a, float b, float c)
 	return r;
 }
 
-// The same slab test for a ray whose 1/dir and origin are finite. Box bounds are finite (checked at
-// upload), so (b - o) is finite or +-inf and 1/dir is finite and non-zero: no slab value is NaN.
+// The same slab test for a ray whose 1/dir is finite and whose origin lies within +-2^126. Box
+// bounds lie within +-2^126 too (checked at upload), so (b - o) is finite and (b - o) * (1/dir) is
+// finite, +-inf or 0 (1/dir may be 0 for an infinite direction): no slab value is NaN.
 // Without NaNs, std::min/std::max and the hardware min/max can differ only in the sign of a zero
 // result, and tIn/tOut are only ever compared (-0 == +0), so every visit decision equals
 // slab()'s.

@@ -352,6 +352,69 @@ def test_deep_stack_matches_oracle(ctx, kernel):
         s.release()
 
 
+def scene_with_outlier(far):
+    """A 600-triangle random scene whose SBVH is put under a new root together with a one-triangle
+    leaf at about (far, far, far): the root's right box reaches `far`."""
+    t, d = phx.random_triangles(600, 0xC0FFEE07, 0.02, 0.2)
+    b = phx.BVH(t, d)
+    f = np.float32(far)
+    big = np.zeros(1, phx.TRI)
+    big["v"][0, :, :3] = [[f, f, f], [f * np.float32(0.99), f, f], [f, f * np.float32(0.99), f]]
+    big["v"][0, 0, 3] = -1.0  # the last (only) triangle of its leaf
+    tris = np.concatenate([b.tris, big])
+    nodes = np.zeros(len(b.nodes) + 1, phx.NODE)
+    nodes[1:] = b.nodes
+    inner = nodes["i"][1:, 2:4] == 0
+    nodes["i"][1:, 0:2] = np.where(inner, nodes["i"][1:, 0:2] + 1, nodes["i"][1:, 0:2])
+    root = b.nodes["f"][0]
+    lo = np.minimum(root[0:3], root[6:9])
+    hi = np.maximum(root[3:6], root[9:12])
+    v = big["v"][0, :, :3]
+    nodes["f"][0] = np.concatenate([lo, hi, v.min(axis=0), v.max(axis=0)])
+    nodes["i"][0] = (1, ~(len(tris) - 1), 0, 1)
+    return nodes, tris
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
+@pytest.mark.parametrize("far", [2.0 ** 100, 1.9 * 2.0 ** 127], ids=["2^100", "1.9*2^127"])
+def test_extreme_coordinates_match_oracle(ctx, far, kernel):
+    """A node bound at 2^100 (every bound within +-2^126: the NaN-free slab test applies to rays
+    whose origin is within +-2^126 too) or at 1.9 * 2^127 (it never applies). Rays: ordinary ones
+    aimed at the small triangles; origins at -1.9 * 2^127, where b - o overflows to inf; infinite
+    direction components (1/dir = 0, so (b - o) * 0 is NaN where b - o is inf); zero components.
+    Closest and any hit equal the oracle's castRay bit for bit."""
+    say(f"[gpu] outlier at {far:.3g}, kernel {kernel}")
+    nodes, tris = scene_with_outlier(far)
+    assert phx.validate(nodes, tris) >= 2
+    rng = np.random.default_rng(int(np.log2(far)))
+    n = 8192
+    rays = np.zeros(n, phx.RAY)
+    rays["o"] = rng.uniform(-1.0, 2.0, (n, 3)).astype(np.float32)
+    aim = tris["v"][rng.integers(0, len(tris) - 1, n), :, :3].mean(axis=1)
+    d = aim.astype(np.float64) - rays["o"]
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    far_o = rng.random(n) < 0.2
+    rays["o"][far_o] = np.float32(-1.9 * 2.0 ** 127)
+    inf_axis = rng.random(n) < 0.15
+    rays["d"][inf_axis, rng.integers(0, 3, inf_axis.sum())] = np.float32(np.inf)
+    zero_axis = (rng.random(n) < 0.1) & ~inf_axis
+    rays["d"][zero_axis, rng.integers(0, 3, zero_axis.sum())] = np.float32(0.0)
+    rays["tmin"] = 0.0
+    rays["tmax"] = phx.FLT_MAX
+    s = ctx.upload_scene(nodes, tris)
+    try:
+        ref = O.cast_closest(nodes, tris, rays)
+        assert (ref["tri"] != phx.MISS).sum() > n // 10
+        assert_same_hits(ctx.cast_closest(s, rays, kernel), ref, f"outlier {far:.3g} kernel {kernel}")
+        lit = ctx.cast_anyhit(s, rays, kernel)
+        assert (lit == O.cast_anyhit(nodes, tris, rays)).all()
+        # every visit decision too: node, box-hit and triangle-test counts equal castRay's
+        _, tot, _ = O.cast_debug(nodes, tris, rays)
+        assert ctx.traversal_stats(s, rays, kernel) == tot
+    finally:
+        s.release()
+
+
 def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
     """The traversal's 1/a (v_rcp_f32 + one Newton step) equals IEEE division for every float
     with 2^-126 <= |a| < 2^126 on this device."""
