@@ -143,7 +143,7 @@ struct PhxServer {
 	std::unordered_map<uint64_t, Request*> byTicket;
 	uint64_t nextTicket = 1;
 	bool stop = false;
-	bool paused = false;  // phx_server_pause: nothing new is admitted or launched
+	bool paused = false;  // phx_server_pause: nothing new is launched (admission goes on)
 	std::string error;  // first dispatcher failure
 	std::thread dispatcher;
 	PhxServerStats stats{};

@@ -370,6 +370,7 @@ constexpr int kMaxBlock = 1024;
 struct WaveQueue {
 	uint32_t base = 0;   // next unclaimed ray of this wave's window
 	uint32_t left = 0;   // rays left in the window
+	uint32_t start = 0;  // first ray of the window
 	bool exhausted = false;
 };
 
@@ -396,6 +397,7 @@ __device__ __forceinline__ uint32_t refill(WaveQueue& q, uint64_t idle, uint32_t
 			const uint32_t avail = (n - b) < window ? (n - b) : window;
 			take2 = (need - take1) < avail ? (need - take1) : avail;
 			base2 = b;
+			q.start = b;
 			q.base = b + take2;
 			q.left = avail - take2;
 		}
@@ -1361,6 +1363,27 @@ __device__ __forceinline__ uint32_t find_batch(const uint32_t* offsets, uint32_t
 	return lo;
 }
 
+// The batch of each new ray of a refill, without a search per lane: a wave's windows come in
+// increasing order, so the wave keeps the batch of its current window's first ray (`cur`) and of the
+// window before it (`prev`, whose last rays a refill may still hand out). A lane starts from its
+// window's batch and steps over the batch ends below its index -- usually none or one, since a
+// window (at most 4096 rays) is mostly shorter than a batch; after four steps it searches.
+struct BatchCursor {
+	uint32_t start = 0;  // first ray of the window
+	uint32_t batch = 0;  // its batch
+};
+
+__device__ __forceinline__ uint32_t batch_from(const uint32_t* offsets, uint32_t numBatches,
+                                               uint32_t from, uint32_t g)
+{
+	uint32_t b = from;
+	for (int k = 0; k < 4; k++) {
+		if (offsets[b + 1u] > g) return b;
+		b++;
+	}
+	return find_batch(offsets, numBatches, g);
+}
+
 // Launch-wide arguments of the fused kernel.
 struct MultiArgs {
 	const BatchDesc* descs;
@@ -1403,7 +1426,8 @@ __device__ __forceinline__ void track_done(const MultiArgs& P, uint32_t batch)
 // is written with its own batch's output before a new ray's batch replaces it.
 template <bool kTreelet, bool kTrack, bool kExitCheck>
 __device__ __forceinline__ int pipe_step_multi(const MultiArgs& P, MultiTracer& M, LaneStack& st,
-                                               WaveQueue& q, const Rec& A, Rec& B)
+                                               WaveQueue& q, BatchCursor& cur, BatchCursor& prev,
+                                               const Rec& A, Rec& B)
 {
 	__builtin_amdgcn_s_waitcnt(kWaitVmcntZero);
 	Tracer& L = M.t;
@@ -1445,8 +1469,16 @@ __device__ __forceinline__ int pipe_step_multi(const MultiArgs& P, MultiTracer& 
 	uint32_t fresh = kInvalid;
 	if (!q.exhausted) {
 		const uint64_t freeLanes = __ballot(!more);
-		if (freeLanes != 0 && (uint32_t(__popcll(freeLanes)) >= P.refillMin || freeLanes == ~0ull))
+		if (freeLanes != 0 && (uint32_t(__popcll(freeLanes)) >= P.refillMin || freeLanes == ~0ull)) {
+			const uint32_t start = q.start;
 			fresh = refill(q, freeLanes, P.work, P.total, !more, P.window);
+			if (q.start != start && !q.exhausted) {
+				// a new window (uniform): its batch, searched from the current one onwards
+				prev = cur;
+				cur.start = q.start;
+				cur.batch = batch_from(P.offsets, P.numBatches, cur.batch, cur.start);
+			}
+		}
 	}
 	const bool tri = more && L.s.triCur < L.s.triEnd;
 	const bool node = more && !tri;
@@ -1455,7 +1487,7 @@ __device__ __forceinline__ int pipe_step_multi(const MultiArgs& P, MultiTracer& 
 	uint32_t nb = 0u, local = 0u;
 	BatchDesc nd{};
 	if (ray) {
-		nb = find_batch(P.offsets, P.numBatches, fresh);
+		nb = batch_from(P.offsets, P.numBatches, fresh >= cur.start ? cur.batch : prev.batch, fresh);
 		local = fresh - P.offsets[nb];
 		nd = P.descs[nb];
 	}
@@ -1572,6 +1604,7 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 	st.lane = __lane_id();
 	st.sp = 0;
 	WaveQueue q;
+	BatchCursor cur, prev;
 	MultiTracer M;
 	Tracer& L = M.t;
 	L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
@@ -1593,8 +1626,8 @@ __device__ __forceinline__ void trace_batches(const BatchDesc* __restrict__ desc
 	Rec A = {z, z, z, z, z, z}, B = A;
 	int next = helper ? kStepDone : kStepGo;
 	while (next == kStepGo) {
-		pipe_step_multi<kTreelet, kTrack, false>(P, M, st, q, A, B);
-		next = pipe_step_multi<kTreelet, kTrack, true>(P, M, st, q, B, A);
+		pipe_step_multi<kTreelet, kTrack, false>(P, M, st, q, cur, prev, A, B);
+		next = pipe_step_multi<kTreelet, kTrack, true>(P, M, st, q, cur, prev, B, A);
 	}
 	if (!helper) {
 		// Solo: keep the first busy lane's ray, offer the others to the block, then trace offered
