@@ -287,6 +287,55 @@ def test_fused_plan_matches_per_batch_results(ctx, c1, c2):
             b.free()
 
 
+@pytest.mark.parametrize("rays_per_fetch", [8, 256, 4096])
+def test_fused_plan_with_tiny_batches(ctx, c1, rays_per_fetch):
+    """Many tiny and empty batches, so that one work window spans dozens of batches: every new
+    ray's batch (found from its window's batch, or by search after a few batch ends) and every
+    result equal the per-batch casts; a strided sample equals the oracle."""
+    say(f"[gpu] fused plan, tiny batches, {rays_per_fetch} rays per fetch")
+    b, s = c1
+    rng = np.random.default_rng(rays_per_fetch)
+    sizes = rng.choice([0, 1, 2, 3, 5, 8, 13, 64, 300, 5000], size=600,
+                       p=[.1, .2, .15, .1, .1, .1, .1, .07, .05, .03])
+    rays_all = np.zeros(int(sizes.sum()), phx.RAY)
+    rays_all["o"] = rng.uniform(-0.2, 1.2, (len(rays_all), 3)).astype(np.float32)
+    d = rng.normal(size=(len(rays_all), 3))
+    rays_all["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays_all["tmin"] = np.float32(1e-4)
+    rays_all["tmax"] = np.float32(phx.FLT_MAX)
+    rb = ctx.alloc(len(rays_all) * 32).upload(rays_all)
+    ob = ctx.alloc(len(rays_all) * 16)
+    edges = np.concatenate([[0], np.cumsum(sizes)])
+    kinds = [phx.BATCH_ANYHIT if i % 5 == 0 else phx.BATCH_CLOSEST for i in range(len(sizes))]
+    batches = [(s, rb.ptr + int(edges[i]) * 32, ob.ptr + int(edges[i]) * 16, int(sizes[i]), kinds[i])
+               for i in range(len(sizes))]
+    plan = phx.Plan(ctx, batches)
+    try:
+        plan.set_launch_params(grid=256, block_threads=512, rays_per_fetch=rays_per_fetch)
+        for graph in (False, True):
+            plan.launch(graph=graph)
+            raw = ob.download(np.uint8, len(rays_all) * 16)
+            closest = ctx.cast_closest(s, rays_all)
+            lit = ctx.cast_anyhit(s, rays_all)
+            for i in range(len(sizes)):
+                lo, n = int(edges[i]), int(sizes[i])
+                if not n:
+                    continue
+                if kinds[i] == phx.BATCH_ANYHIT:
+                    got = raw[lo * 16: lo * 16 + n]
+                    assert (got == lit[lo: lo + n]).all(), f"batch {i}"
+                else:
+                    got = raw[lo * 16: (lo + n) * 16].view(phx.HIT)
+                    assert_same_hits(got, closest[lo: lo + n], f"batch {i}")
+        sample = rays_all[::97]
+        assert_same_hits(ctx.cast_closest(s, sample), O.cast_closest(b.nodes, b.tris, sample),
+                         "per-batch cast vs oracle")
+    finally:
+        plan.release()
+        rb.free()
+        ob.free()
+
+
 def test_scene_file_upload_matches_oracle(ctx, tmp_path):
     """A scene written to and read back from a scene file traces exactly like the built one."""
     tris, datas = phx.heightfield_boxes(40, 30, 0xC0FFEE02)
