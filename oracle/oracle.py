@@ -126,7 +126,13 @@ def rlib(fastmath: bool = False) -> C.CDLL:
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data
+    """The data pointer of a C-contiguous array (the C functions read it as packed records); a
+    strided view would hand them the wrong memory, so it is refused."""
+    if a is None:
+        return None
+    if not a.flags.c_contiguous:
+        raise ValueError("oracle inputs must be C-contiguous arrays (use np.ascontiguousarray)")
+    return a.ctypes.data
 
 
 # ---- oracle (C restatement) -----------------------------------------------------------------

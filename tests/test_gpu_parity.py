@@ -327,7 +327,7 @@ def test_fused_plan_with_tiny_batches(ctx, c1, rays_per_fetch):
                 else:
                     got = raw[lo * 16: (lo + n) * 16].view(phx.HIT)
                     assert_same_hits(got, closest[lo: lo + n], f"batch {i}")
-        sample = rays_all[::97]
+        sample = np.ascontiguousarray(rays_all[::97])
         assert_same_hits(ctx.cast_closest(s, sample), O.cast_closest(b.nodes, b.tris, sample),
                          "per-batch cast vs oracle")
     finally:

@@ -194,3 +194,13 @@ def test_reference_band_split_and_restatement_agree():
     assert (chunked["tri"] != phx.MISS).sum() > len(rays) // 10
     assert chunked.tobytes() == O.ref_cast_bands(nodes, rtris, rays, 10).tobytes()
     assert chunked.tobytes() == O.cast_closest(nodes, rtris, rays).tobytes()
+
+
+def test_oracle_refuses_strided_inputs():
+    """The C restatement reads packed records from the data pointer: a strided view (every
+    other ray) would hand it the wrong rays, so the binding refuses it instead."""
+    fx = load_fixture("hf32")
+    with pytest.raises(ValueError):
+        O.cast_closest(fx["nodes"], fx["tris"], fx["rays"][::2])
+    rays = np.ascontiguousarray(fx["rays"][::2])
+    assert O.cast_closest(fx["nodes"], fx["tris"], rays).tobytes() == fx["hits"][::2].tobytes()
