@@ -452,7 +452,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the rays of a batch "
-              "shard across GPUs with no exchange step (each GPU holds the scene), and this build "
+              "shard across GPUs with no exchange step (each GPU holds the scene); this benchmark "
               "runs on a single MI355X", file=sys.stderr)
         return 2
 

@@ -1,6 +1,6 @@
 /*
  * frame.c -- CPU restatement of the wavefront frame and the TAA resolve. TEST INFRASTRUCTURE ONLY
- * (see oracle.h): loaded by tests/ as the checker of the device frame, never by the product.
+ * (see oracle.h): loaded by tests/ to check the device frame, never by the product.
  *
  * Built like oracle.c (-O2 -ffp-contract=off, no fast-math, SSE float): every expression is one
  * IEEE single-precision operation in the written order. The shading formulas avoid library

@@ -408,7 +408,7 @@ int oracle_aabb(const float o[3], const float d[3], const float mn[3], const flo
 
 /* ---- instances: the two-level path (phx_instances_build, phx_cast_*_instances) ----
  *
- * The build's definition, restated: an instance is traced iff the ray enters its world box
+ * The library's definition, restated: an instance is traced iff the ray enters its world box
  * (castRay's box test with tIn <= min(tMax, best t)); it runs castRay on its mesh with the ray in
  * object space (o' = A o + b, d' = A d, the same tMin/tMax), from a miss; the closest result wins,
  * on equal t the static scene, then the lowest instance index. Instances are taken here in index

@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of the reference's ray-query path. TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library,
- * and only as the checker (or, in the bench, as the timed CPU baseline). The product (libphx)
- * never links or calls it.
+ * and only to check the HIP library's results against it (or, in the bench, as the timed CPU
+ * baseline). The product (libphx) never links or calls it.
  *
  * Each function names the reference code it restates (paths relative to the reference root).
  * Parity pinning (what checks each function against something other than itself):

@@ -1,6 +1,6 @@
 """ctypes binding of the oracle (liboracle.so) and of the reference's own CPU code compiled from
 its sources (_ref/libphxref.so). TEST INFRASTRUCTURE ONLY: imported by tests/, by
-__graft_entry__.smoke() as the checker and by bench.py's cpu_baseline leg.
+__graft_entry__.smoke() to check the HIP library's results, and by bench.py's cpu_baseline leg.
 
 Arrays use the numpy dtypes of phantasyenginetestbed_amd.phx (same byte layouts).
 """

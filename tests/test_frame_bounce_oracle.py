@@ -57,7 +57,7 @@ def frame_rng(seed: int, frame: int, purpose: int, rid: int) -> Pcg32:
     return Pcg32((seed ^ (frame << 32)) & MASK64, (purpose << 32) | rid)
 
 
-# pcg32-demo of pcg-c-basic, "Round 1" 32-bit outputs of pcg32_srandom_r(42, 54).
+# pcg32-demo of pcg-c-basic: the first six 32-bit outputs it prints for pcg32_srandom_r(42, 54).
 PCG32_DEMO_42_54 = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
 
 

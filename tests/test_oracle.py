@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker:
+"""Pins the oracle (oracle/oracle.c) before GPU results are checked against it:
   - against castRay<144>/castDebugRay<144> outputs of the reference compiled from its sources
     (small-scene fixtures, and a digest of all 1,048,576 rays of the 100k-triangle workload);
   - against closed forms of the ray/triangle and ray/box tests, including the +-0 / inf / NaN
