@@ -527,6 +527,19 @@ def main() -> int:
     log(f"{args.steps} steps: events {total_ms:.2f} ms, wall {wall * 1e3:.2f} ms")
 
     gpu_hits = out.download(phx.HIT, n)
+    tail = None
+    if kernel == "persistent":
+        # one more launch of the same shape that also records when each ray finished
+        done = ctx.cast_closest_timeline(scene, rays.ptr, n, out.ptr)
+        same = out.download(phx.HIT, n).tobytes() == gpu_hits.tobytes()
+        q = np.sort(done)
+        tail = {"rays_done_ms": {f"{f:g}%": round(float(q[min(n - 1, int(np.ceil(f / 100 * n)) - 1)]), 4)
+                                 for f in (50, 90, 99, 99.9)},
+                "last_ms": round(float(q[-1]), 4),
+                "identical_hits": bool(same),
+                "note": "one extra launch (per-ray device clock at each result write), "
+                        "phx_cast_closest_timeline"}
+        log(f"tail: {tail}")
     value = n / (ms_per_step / 1e3) / 1e6
     achieved = bytes_per_ray * n / (ms_per_step / 1e3) / 1e9
     key = shape_key(launch)
@@ -555,6 +568,7 @@ def main() -> int:
                               if k in launch}},
         "p50_ms": round(float(np.percentile(step_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 4),
+        "tail": tail,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

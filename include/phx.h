@@ -354,6 +354,16 @@ void phx_scene_file_release(PhxSceneFile* file);
 phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                             uint32_t numRays, PhxRayHit* hits, uint32_t flags);
 
+/* Diagnostics, no counterpart in the reference: phx_cast_closest on device buffers (`flags` =
+ * PHX_MEM_DEVICE) with the persistent kernel at the context's launch shape, which also reports
+ * when each ray finished: msSinceStart[i] (host array of numRays floats) = milliseconds from the
+ * launch's first block start to the write of ray i's result, on the device's constant-rate clock.
+ * Hits equal phx_cast_closest's. For tail analysis (when the last rays finish); the clock writes
+ * make this launch slightly slower than the plain one. At most 2^31 - 1 rays. */
+phx_status phx_cast_closest_timeline(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                                     uint32_t numRays, PhxRayHit* hits, float* msSinceStart,
+                                     uint32_t flags);
+
 /* phx_cast_closest on device buffers without allocating anything: no counterpart in the reference;
  * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). Results equal phx_cast_closest's. `flags` must contain

@@ -722,6 +722,47 @@ phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRay
 		"phx_cast_closest");
 }
 
+phx_status phx_cast_closest_timeline(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                                     uint32_t numRays, PhxRayHit* hits, float* msSinceStart,
+                                     uint32_t flags)
+{
+	phx_status st = checkCtx(ctx);
+	if (st != PHX_OK) return st;
+	st = checkScene(ctx, scene);
+	if (st != PHX_OK) return st;
+	if (flags != PHX_MEM_DEVICE)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_timeline: flags 0x%x (device rays "
+		            "and hits, PHX_MEM_DEVICE only)", flags);
+	if (numRays > 0x7FFFFFFFu)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_timeline: %u rays (at most "
+		            "2^31 - 1)", numRays);
+	if (numRays == 0) return ok();
+	if (!rays || !hits || !msSinceStart)
+		return fail(PHX_ERR_INVALID_ARGUMENT, "phx_cast_closest_timeline: NULL");
+	int khz = 0;
+	PHX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device),
+	        "phx_cast_closest_timeline: clock rate");
+	if (khz <= 0) return fail(PHX_ERR_DEVICE, "phx_cast_closest_timeline: no device clock rate");
+	const size_t words = size_t(numRays) + 1u;
+	unsigned long long* clock = nullptr;
+	if (hipMalloc(reinterpret_cast<void**>(&clock), words * 8u) != hipSuccess)
+		return fail(PHX_ERR_OUT_OF_MEMORY, "phx_cast_closest_timeline: %zu bytes", words * 8u);
+	std::vector<unsigned long long> host(words);
+	const TraceLaunch L = traceLaunch(ctx, 0, false);
+	hipError_t e = hipMemsetAsync(clock, 0xFF, words * 8u, ctx->stream);
+	if (e == hipSuccess) e = launchClosestClock(view(scene), rays, numRays, hits, L, clock, ctx->stream);
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(host.data(), clock, words * 8u, hipMemcpyDeviceToHost, ctx->stream);
+	const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+	hipFree(clock);
+	if (e != hipSuccess) return hipFail(e, "phx_cast_closest_timeline");
+	if (e2 != hipSuccess) return hipFail(e2, "phx_cast_closest_timeline");
+	const unsigned long long start = host[numRays];
+	for (uint32_t i = 0; i < numRays; i++)
+		msSinceStart[i] = host[i] == ~0ull ? -1.0f : float(double(host[i] - start) / double(khz));
+	return ok();
+}
+
 phx_status phx_cast_closest_noalloc(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
                                     uint32_t numRays, PhxRayHit* hits, uint32_t flags)
 {

@@ -117,6 +117,13 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
                          const TraceLaunch& launch, unsigned long long* counters,
                          hipStream_t stream);
 
+// The persistent closest-hit launch of `launch`'s shape, also writing each ray's device clock
+// (wall_clock64) when its result is written to clock[i] and the earliest block start to clock[n]
+// (which the caller sets to ~0 first).
+hipError_t launchClosestClock(const SceneView& s, const PhxRayIn* rays, uint32_t n, PhxRayHit* hits,
+                              const TraceLaunch& launch, unsigned long long* clock,
+                              hipStream_t stream);
+
 // Any hit (shadow rays).
 hipError_t launchAnyHit(const SceneView& s, const PhxRayIn* rays, uint32_t n, uint8_t* inLight,
                         const TraceLaunch& launch, hipStream_t stream);

@@ -635,6 +635,7 @@ struct PipeArgs {
 	const float4* treelet;  // LDS copy of nodes [0, topCount)
 	const uint32_t* handoff;  // the block's handoff area (LDS)
 	uint32_t window;          // rays per fetch
+	unsigned long long* clock;  // completion clock per ray (completion-clock launches only)
 };
 
 // Loop-carried state of one lane.
@@ -756,7 +757,7 @@ __device__ __forceinline__ uint32_t solo_threshold(const uint32_t* handoff);
 // so work that only some lanes need sits in the branch of those lanes. kExitCheck: whether this
 // step tests if the wave is done or should go solo (kStepGo otherwise); the loop runs two steps
 // per turn and checks after the second.
-template <bool kAnyHit, bool kCount, bool kTreelet, bool kExitCheck>
+template <bool kAnyHit, bool kCount, bool kTreelet, bool kExitCheck, bool kClock = false>
 __device__ __forceinline__ int pipe_step(const PipeArgs& P, Tracer& L, LaneStack& st,
                                           WaveQueue& q, const Rec& A, Rec& B, Counts& k)
 {
@@ -820,7 +821,10 @@ __device__ __forceinline__ int pipe_step(const PipeArgs& P, Tracer& L, LaneStack
 	const uint32_t nextKind = tri ? kRecTri : node ? kRecNode : ray ? kRecRay : kRecIdle;
 	// The ended ray's result is written just before the loads, after the waits of this step (the
 	// stack pop's, the refill's), so its latency overlaps the triangle tests, not a wait.
-	if (L.doneIdx != kInvalid) store_result<kAnyHit>(P, L.doneIdx, ended);
+	if (L.doneIdx != kInvalid) {
+		store_result<kAnyHit>(P, L.doneIdx, ended);
+		if (kClock) P.clock[L.doneIdx] = wall_clock64();
+	}
 	// Every lane issues the same six loads: a node (words 0-3), a triangle pair (words 0-5), or
 	// the root node and a new ray's 32 B; idle lanes and words 4-5 of a node read the root node
 	// (always cached). Offsets are in 16-B words: 4 per node, 3 per triangle.
@@ -940,6 +944,7 @@ struct SoloScene {
 	const float4* tris;
 	const float4* rays;
 	void* out;  // uint4 hits (closest hit) or uint8_t (any hit), indexed by the ray's index
+	unsigned long long* clock = nullptr;  // completion clock per ray (completion-clock launches)
 };
 
 // Where a solo ray starts: the state the pipelined loop left for it (the record in flight is
@@ -984,7 +989,7 @@ __device__ __forceinline__ SoloStart solo_start_lane(const Tracer& L, const Lane
 }
 
 // Traces a ray to its end from S and writes its result.
-template <bool kAnyHit>
+template <bool kAnyHit, bool kClock = false>
 __device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 {
 	const uint32_t lane = __lane_id();
@@ -1068,6 +1073,7 @@ __device__ __forceinline__ void solo_run(const SoloScene& P, const SoloStart& S)
 	if (lane == 0u) {
 		const Closest res = {kAnyHit ? (c.any ? 0u : kInvalid) : c.tri, c.t, c.u, c.v};
 		store_result<kAnyHit>(P.out, rayIdx, res);
+		if (kClock) P.clock[rayIdx] = wall_clock64();
 	}
 }
 
@@ -1233,7 +1239,9 @@ __device__ __forceinline__ bool is_helper(uint32_t helpers)
 // Shared skeleton of the persistent closest-hit (kAnyHit = false) and any-hit kernels. Without a
 // treelet (kTreelet false) every record comes from global memory and no LDS path is compiled.
 // LDS stacks and spill columns belong to the lane-mode threads only.
-template <bool kAnyHit, bool kCount, bool kTreelet>
+// kClock: `counters` is instead a completion clock of n + 1 words: each ray's device clock
+// (wall_clock64) when its result is written, and in word n the earliest block start.
+template <bool kAnyHit, bool kCount, bool kTreelet, bool kClock = false>
 __device__ __forceinline__ void persistent_trace(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint8_t* __restrict__ inLight,
@@ -1247,19 +1255,21 @@ __device__ __forceinline__ void persistent_trace(
 	uint32_t* ldsStacks = reinterpret_cast<uint32_t*>(ldsRaw);
 	float4* treelet = ldsRaw + laneWaves * (uint32_t(depth) + 1u) * 64u / 4u;
 	uint32_t* handoff = reinterpret_cast<uint32_t*>(treelet + 4u * topCount);
+	if (kClock && threadIdx.x == 0) atomicMin(&counters[n], wall_clock64());
 	if (kTreelet)
 		for (uint32_t i = threadIdx.x; i < 4u * topCount; i += blockDim.x) treelet[i] = nodes[i];
 	handoff_init(handoff, laneWaves);
 	__syncthreads();
 
 	const SoloScene S = {nodes, tris, rays,
-	                     kAnyHit ? static_cast<void*>(inLight) : static_cast<void*>(hits)};
+	                     kAnyHit ? static_cast<void*>(inLight) : static_cast<void*>(hits),
+	                     kClock ? counters : nullptr};
 	const uint32_t stride = gridDim.x * laneThreads;
 	if (!kCount && is_helper(helpers)) {
 		__builtin_amdgcn_s_setprio(3);
 		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
 			uint32_t batch;
-			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, stride, depth,
+			solo_run<kAnyHit, kClock>(S, solo_start_slot(handoff, slot, ldsStacks, spill, stride, depth,
 			                                     batch));
 		}
 		retire_wave(work);
@@ -1275,7 +1285,8 @@ __device__ __forceinline__ void persistent_trace(
 	st.sp = 0;
 
 	const PipeArgs P = {nodes, tris, rays, hits, inLight, n, work, refillMin,
-	                    kTreelet ? topCount : 0u, finiteBounds, treelet, handoff, window};
+	                    kTreelet ? topCount : 0u, finiteBounds, treelet, handoff, window,
+	                    kClock ? counters : nullptr};
 	WaveQueue q;
 	Tracer L;
 	L.c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
@@ -1290,26 +1301,30 @@ __device__ __forceinline__ void persistent_trace(
 	Counts k;
 	int next = kStepGo;
 	while (next == kStepGo) {
-		pipe_step<kAnyHit, kCount, kTreelet, false>(P, L, st, q, A, B, k);
-		next = pipe_step<kAnyHit, kCount, kTreelet, true>(P, L, st, q, B, A, k);
+		pipe_step<kAnyHit, kCount, kTreelet, false, kClock>(P, L, st, q, A, B, k);
+		next = pipe_step<kAnyHit, kCount, kTreelet, true, kClock>(P, L, st, q, B, A, k);
 	}
-	if (L.doneIdx != kInvalid) store_result<kAnyHit>(P, L.doneIdx, L.c);
+	if (L.doneIdx != kInvalid) {
+		store_result<kAnyHit>(P, L.doneIdx, L.c);
+		if (kClock) P.clock[L.doneIdx] = wall_clock64();
+	}
 	if (!kCount) {
 		// Solo: keep the first busy lane's ray, offer the others to the block, then trace offered
 		// rays until none is left (the loads still in flight land in A, which is not read again).
 		const uint64_t busy = next == kStepSolo ? __ballot(L.kind != kRecIdle) : 0ull;
 		handoff_offer(handoff, busy & (busy - 1u), L, st, 0u);
-		if (busy != 0) solo_run<kAnyHit>(S, solo_start_lane(L, st, uint32_t(__builtin_ctzll(busy))));
+		if (busy != 0)
+			solo_run<kAnyHit, kClock>(S, solo_start_lane(L, st, uint32_t(__builtin_ctzll(busy))));
 		for (uint32_t slot = handoff_claim(handoff); slot != kInvalid; slot = handoff_claim(handoff)) {
 			uint32_t batch;
-			solo_run<kAnyHit>(S, solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, depth,
+			solo_run<kAnyHit, kClock>(S, solo_start_slot(handoff, slot, ldsStacks, spill, st.stride, depth,
 			                                     batch));
 		}
 	}
 	if (kCount) add_counts(counters, k);
 	retire_wave(work);
 }
-template <bool kCount, bool kTreelet>
+template <bool kCount, bool kTreelet, bool kClock = false>
 __global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	const float4* __restrict__ nodes, const float4* __restrict__ tris,
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
@@ -1317,9 +1332,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_closest_persistent(
 	uint32_t finiteBounds, uint32_t topCount, uint32_t helpers, uint32_t depth, uint32_t window,
 	unsigned long long* __restrict__ counters)
 {
-	persistent_trace<false, kCount, kTreelet>(nodes, tris, rays, hits, nullptr, n, work, spill,
-	                                          refillMin, finiteBounds != 0, topCount, helpers,
-	                                          int(depth), window, counters);
+	persistent_trace<false, kCount, kTreelet, kClock>(nodes, tris, rays, hits, nullptr, n, work,
+	                                                  spill, refillMin, finiteBounds != 0,
+	                                                  topCount, helpers, int(depth), window,
+	                                                  counters);
 }
 
 template <bool kTreelet>
@@ -1844,6 +1860,23 @@ hipError_t launchClosest(const SceneView& s, const PhxRayIn* rays, uint32_t n, P
 	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, nodes, tris, r4, h4, n,
 	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, shape.helperWaves,
 	                   L.ldsDepth, L.window, counters);
+	return hipGetLastError();
+}
+
+hipError_t launchClosestClock(const SceneView& s, const PhxRayIn* rays, uint32_t n, PhxRayHit* hits,
+                              const TraceLaunch& L, unsigned long long* clock, hipStream_t stream)
+{
+	if (n == 0) return hipSuccess;
+	if (n > 0x7FFFFFFFu) return hipErrorInvalidValue;
+	uint32_t grid, block, top, lds;
+	persistentShape(L, s, n, grid, block, top, lds);
+	auto kernel = top ? k_closest_persistent<false, true, true> : k_closest_persistent<false, false, true>;
+	hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream,
+	                   reinterpret_cast<const float4*>(s.nodes),
+	                   reinterpret_cast<const float4*>(s.trisPre),
+	                   reinterpret_cast<const float4*>(rays), reinterpret_cast<uint4*>(hits), n,
+	                   L.work, L.spill, L.refillMin, s.finiteBounds, top, L.helperWaves, L.ldsDepth,
+	                   L.window, clock);
 	return hipGetLastError();
 }
 

@@ -177,6 +177,7 @@ _SIGNATURES = {
     "phx_interpret_hits": (C.c_int, [_P, _P, _P, _P, _U32, _P, _U32]),
     "phx_cast_closest": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
     "phx_cast_closest_noalloc": (C.c_int, [_P, _P, _P, _U32, _P, _U32]),
+    "phx_cast_closest_timeline": (C.c_int, [_P, _P, _P, _U32, _P, _P, _U32]),
     "phx_instances_build": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
     "phx_instances_release": (None, [_P]),
     "phx_cast_closest_instances": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P, _U32]),
@@ -744,6 +745,14 @@ class Context:
                             mode: int = TRAVERSAL_REFERENCE_ORDER, asynchronous: bool = False):
         flags = MEM_DEVICE | mode | (ASYNC if asynchronous else 0)
         check(lib().phx_cast_closest(self.handle, scene.handle, rays_ptr, n, hits_ptr, flags))
+
+    def cast_closest_timeline(self, scene: Scene, rays_ptr: int, n: int, hits_ptr: int) -> np.ndarray:
+        """phx_cast_closest_timeline: device rays and hits; returns, per ray, milliseconds from the
+        launch's first block start to the write of its result."""
+        ms = np.empty(n, np.float32)
+        check(lib().phx_cast_closest_timeline(self.handle, scene.handle, rays_ptr, n, hits_ptr,
+                                              _ptr(ms), MEM_DEVICE))
+        return ms
 
     def cast_closest_noalloc(self, scene: Scene, rays_ptr: int, n: int, hits_ptr: int,
                              asynchronous: bool = False) -> None:

@@ -464,6 +464,42 @@ def test_extreme_coordinates_match_oracle(ctx, far, kernel):
         s.release()
 
 
+@pytest.mark.parametrize("helpers", [0, 4])
+def test_completion_timeline(ctx, c2, helpers):
+    """phx_cast_closest_timeline: the same hits as phx_cast_closest, and a finishing time for
+    every ray (lane-mode and solo rays alike) within the launch; device buffers only."""
+    say(f"[gpu] completion timeline, {helpers} helper waves")
+    b, s = c2
+    cam = phx.camera_look_at(*C2_CAM, math.radians(60), 1920, 1080)
+    rays = ctx.gen_primary_rays(cam, 480, 270)
+    hits = ctx.cast_closest(s, rays)
+    bounce = ctx.gen_bounce_rays(s, rays, hits, seed=9, frame=2)
+    n = len(bounce)
+    before = ctx.launch_params()
+    ctx.set_launch_params(grid_closest=256, grid_anyhit=256, refill_min=8, block_threads=512,
+                          top_nodes=0, helper_waves=helpers, rays_per_fetch=128,
+                          stack_lds_entries=24)
+    rb, hb = ctx.alloc(n * 32).upload(bounce), ctx.alloc(n * 16)
+    try:
+        ms = ctx.cast_closest_timeline(s, rb.ptr, n, hb.ptr)
+        assert_same_hits(hb.download(phx.HIT, n), ctx.cast_closest(s, bounce), "timeline hits")
+        assert (ms >= 0).all() and ms.max() < 50.0
+        assert np.median(ms) < ms.max()
+        st = phx.lib().phx_cast_closest_timeline(ctx.handle, s.handle, rb.ptr, n, hb.ptr,
+                                                 phx._ptr(ms), 0)
+        assert st == phx.ERR_INVALID_ARGUMENT
+    finally:
+        rb.free()
+        hb.free()
+        ctx.set_launch_params(grid_closest=before["gridBlocksClosest"],
+                              grid_anyhit=before["gridBlocksAnyhit"],
+                              refill_min=before["refillMin"],
+                              block_threads=before["blockThreads"],
+                              top_nodes=before["topNodes"], helper_waves=before["helperWaves"],
+                              rays_per_fetch=before["raysPerFetch"],
+                              stack_lds_entries=before["stackLdsEntries"])
+
+
 def test_fast_reciprocal_matches_ieee_division_exhaustively(ctx):
     """The traversal's 1/a (v_rcp_f32 + one Newton step) equals IEEE division for every float
     with 2^-126 <= |a| < 2^126 on this device."""
