@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <unordered_set>
@@ -518,11 +519,12 @@ phx_status phx_synchronize(PhxContext* ctx)
 	return ok();
 }
 
-phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t numNodes,
-                            const PhxTriangleVertices* triangles, const PhxTriangleData* datas,
-                            uint32_t numTriangles, const PhxMaterial* materials,
-                            uint32_t numMaterials, const PhxSphereLight* lights,
-                            uint32_t numLights, PhxScene** outScene)
+static phx_status phx_scene_upload_impl(PhxContext* ctx, const PhxBVHNode* nodes,
+                                        uint32_t numNodes, const PhxTriangleVertices* triangles,
+                                        const PhxTriangleData* datas, uint32_t numTriangles,
+                                        const PhxMaterial* materials, uint32_t numMaterials,
+                                        const PhxSphereLight* lights, uint32_t numLights,
+                                        PhxScene** outScene)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -588,6 +590,18 @@ phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t n
 	registry().add(s);
 	*outScene = s;
 	return ok();
+}
+
+phx_status phx_scene_upload(PhxContext* ctx, const PhxBVHNode* nodes, uint32_t numNodes,
+                            const PhxTriangleVertices* triangles, const PhxTriangleData* datas,
+                            uint32_t numTriangles, const PhxMaterial* materials,
+                            uint32_t numMaterials, const PhxSphereLight* lights,
+                            uint32_t numLights, PhxScene** outScene)
+{
+	return guarded("phx_scene_upload", [&] {
+		return phx_scene_upload_impl(ctx, nodes, numNodes, triangles, datas, numTriangles,
+		                             materials, numMaterials, lights, numLights, outScene);
+	});
 }
 
 void phx_scene_release(PhxScene* scene)
@@ -744,15 +758,16 @@ phx_status phx_cast_closest_timeline(PhxContext* ctx, const PhxScene* scene, con
 	        "phx_cast_closest_timeline: clock rate");
 	if (khz <= 0) return fail(PHX_ERR_DEVICE, "phx_cast_closest_timeline: no device clock rate");
 	const size_t words = size_t(numRays) + 1u;
+	std::unique_ptr<unsigned long long[]> host(new (std::nothrow) unsigned long long[words]);
+	if (!host) return fail(PHX_ERR_OUT_OF_MEMORY, "phx_cast_closest_timeline: host memory");
 	unsigned long long* clock = nullptr;
 	if (hipMalloc(reinterpret_cast<void**>(&clock), words * 8u) != hipSuccess)
 		return fail(PHX_ERR_OUT_OF_MEMORY, "phx_cast_closest_timeline: %zu bytes", words * 8u);
-	std::vector<unsigned long long> host(words);
 	const TraceLaunch L = traceLaunch(ctx, 0, false);
 	hipError_t e = hipMemsetAsync(clock, 0xFF, words * 8u, ctx->stream);
 	if (e == hipSuccess) e = launchClosestClock(view(scene), rays, numRays, hits, L, clock, ctx->stream);
 	if (e == hipSuccess)
-		e = hipMemcpyAsync(host.data(), clock, words * 8u, hipMemcpyDeviceToHost, ctx->stream);
+		e = hipMemcpyAsync(host.get(), clock, words * 8u, hipMemcpyDeviceToHost, ctx->stream);
 	const hipError_t e2 = hipStreamSynchronize(ctx->stream);
 	hipFree(clock);
 	if (e != hipSuccess) return hipFail(e, "phx_cast_closest_timeline");
@@ -955,8 +970,8 @@ phx_status phx_interpret_hits(PhxContext* ctx, const PhxScene* scene, const PhxR
 	return ok();
 }
 
-phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t numBatches,
-                           uint32_t flags, PhxPlan** outPlan)
+static phx_status phx_plan_create_impl(PhxContext* ctx, const PhxBatch* batches,
+                                       uint32_t numBatches, uint32_t flags, PhxPlan** outPlan)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -1039,6 +1054,14 @@ phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t nu
 	registry().add(p);
 	*outPlan = p;
 	return ok();
+}
+
+phx_status phx_plan_create(PhxContext* ctx, const PhxBatch* batches, uint32_t numBatches,
+                           uint32_t flags, PhxPlan** outPlan)
+{
+	return guarded("phx_plan_create", [&] {
+		return phx_plan_create_impl(ctx, batches, numBatches, flags, outPlan);
+	});
 }
 
 phx_status phx_plan_launch(PhxContext* ctx, PhxPlan* plan, uint32_t flags)
@@ -1280,8 +1303,8 @@ phx_status refine(PhxContext* ctx, const PhxPlan* plan, int reps, Launch launch,
 
 extern "C" {
 
-phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
-                        uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen)
+static phx_status phx_autotune_impl(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                                    uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -1328,6 +1351,14 @@ phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* 
 	return ok();
 }
 
+phx_status phx_autotune(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,
+                        uint32_t numRays, uint32_t flags, PhxLaunchParams* chosen)
+{
+	return guarded("phx_autotune", [&] {
+		return phx_autotune_impl(ctx, scene, rays, numRays, flags, chosen);
+	});
+}
+
 phx_status phx_plan_set_launch_params(PhxContext* ctx, PhxPlan* plan, const PhxLaunchParams* p)
 {
 	phx_status st = checkCtx(ctx);
@@ -1348,7 +1379,7 @@ phx_status phx_plan_set_launch_params(PhxContext* ctx, PhxPlan* plan, const PhxL
 	return ok();
 }
 
-phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* chosen)
+static phx_status phx_plan_autotune_impl(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* chosen)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -1368,6 +1399,13 @@ phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* ch
 	if (st != PHX_OK) return st;
 	if (chosen) *chosen = best.params;
 	return ok();
+}
+
+phx_status phx_plan_autotune(PhxContext* ctx, PhxPlan* plan, PhxLaunchParams* chosen)
+{
+	return guarded("phx_plan_autotune", [&] {
+		return phx_plan_autotune_impl(ctx, plan, chosen);
+	});
 }
 
 void phx_plan_release(PhxPlan* plan)

@@ -204,8 +204,8 @@ bool instanceSetValid(PhxContext* ctx, const PhxInstanceSet* s) { return checkSe
 
 extern "C" {
 
-phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, uint32_t count,
-                               PhxInstanceSet** outSet)
+static phx_status phx_instances_build_impl(PhxContext* ctx, const PhxInstance* instances,
+                                           uint32_t count, PhxInstanceSet** outSet)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -285,6 +285,14 @@ phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, ui
 	registry().add(s);
 	*outSet = s;
 	return ok();
+}
+
+phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, uint32_t count,
+                               PhxInstanceSet** outSet)
+{
+	return guarded("phx_instances_build", [&] {
+		return phx_instances_build_impl(ctx, instances, count, outSet);
+	});
 }
 
 void phx_instances_release(PhxInstanceSet* set)

@@ -4,6 +4,8 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -25,6 +27,20 @@ phx_status fail(phx_status status, const char* fmt, ...);
 
 // Clears the calling thread's last error and returns PHX_OK.
 phx_status ok();
+
+// Runs f (which returns a phx_status) and turns a C++ exception -- a failed host allocation, or a
+// thread that could not be started -- into PHX_ERR_OUT_OF_MEMORY, so that none crosses the C ABI.
+template <class F>
+phx_status guarded(const char* what, F&& f)
+{
+	try {
+		return f();
+	} catch (const std::bad_alloc&) {
+		return fail(PHX_ERR_OUT_OF_MEMORY, "%s: out of host memory", what);
+	} catch (const std::exception& e) {
+		return fail(PHX_ERR_OUT_OF_MEMORY, "%s: %s", what, e.what());
+	}
+}
 
 // Host-side result of the SBVH build: the three arrays of the reference's BVH container
 // (ray_tracer_common/BVH.hpp:19-26) after sanitizeBVH.

@@ -346,8 +346,8 @@ void destroyServer(PhxServer* s)
 
 extern "C" {
 
-phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t numSlots,
-                             uint32_t flags, PhxServer** outServer)
+static phx_status phx_server_create_impl(PhxContext* ctx, uint64_t budgetBytes, uint32_t numSlots,
+                                         uint32_t flags, PhxServer** outServer)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -420,6 +420,14 @@ phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t num
 	return ok();
 }
 
+phx_status phx_server_create(PhxContext* ctx, uint64_t budgetBytes, uint32_t numSlots,
+                             uint32_t flags, PhxServer** outServer)
+{
+	return guarded("phx_server_create", [&] {
+		return phx_server_create_impl(ctx, budgetBytes, numSlots, flags, outServer);
+	});
+}
+
 void phx_server_destroy(PhxServer* s)
 {
 	if (!s || !registry().remove(s)) return;
@@ -466,7 +474,7 @@ phx_status phx_server_add_scene(PhxServer* s, const PhxScene* scene)
 	return ok();
 }
 
-phx_status phx_server_submit(PhxServer* s, const PhxRequest* req, uint64_t* ticket)
+static phx_status phx_server_submit_impl(PhxServer* s, const PhxRequest* req, uint64_t* ticket)
 {
 	if (!s || !registry().has(s)) return fail(PHX_ERR_INVALID_HANDLE, "invalid server handle");
 	if (!req || !ticket) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_server_submit: NULL argument");
@@ -509,6 +517,13 @@ phx_status phx_server_submit(PhxServer* s, const PhxRequest* req, uint64_t* tick
 	*ticket = r->ticket;
 	s->cvWork.notify_one();
 	return ok();
+}
+
+phx_status phx_server_submit(PhxServer* s, const PhxRequest* req, uint64_t* ticket)
+{
+	return guarded("phx_server_submit", [&] {
+		return phx_server_submit_impl(s, req, ticket);
+	});
 }
 
 phx_status phx_server_poll(PhxServer* s, uint64_t ticket, int* done, double* latencyMs)
