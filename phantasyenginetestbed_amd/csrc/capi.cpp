@@ -626,8 +626,8 @@ size_t phx_scene_device_bytes(const PhxScene* scene)
 	return scene->bytes;
 }
 
-phx_status phx_scene_set_textures(PhxContext* ctx, PhxScene* scene, const PhxTexture* textures,
-                                  uint32_t count)
+static phx_status phx_scene_set_textures_impl(PhxContext* ctx, PhxScene* scene,
+                                              const PhxTexture* textures, uint32_t count)
 {
 	phx_status st = checkCtx(ctx);
 	if (st != PHX_OK) return st;
@@ -711,6 +711,14 @@ phx_status phx_scene_set_textures(PhxContext* ctx, PhxScene* scene, const PhxTex
 	scene->tex.count = count;
 	scene->version = nextSceneStamp();
 	return ok();
+}
+
+phx_status phx_scene_set_textures(PhxContext* ctx, PhxScene* scene, const PhxTexture* textures,
+                                  uint32_t count)
+{
+	return guarded("phx_scene_set_textures", [&] {
+		return phx_scene_set_textures_impl(ctx, scene, textures, count);
+	});
 }
 
 phx_status phx_cast_closest(PhxContext* ctx, const PhxScene* scene, const PhxRayIn* rays,

@@ -439,14 +439,14 @@ void phx_server_destroy(PhxServer* s)
 	s->cvWork.notify_all();
 	if (s->dispatcher.joinable()) s->dispatcher.join();
 	hipSetDevice(s->ctx->device);
-	const std::vector<const PhxScene*> held(s->scenes.begin(), s->scenes.end());
+	const std::unordered_set<const PhxScene*> held = std::move(s->scenes);  // no allocation
 	destroyServer(s);
 	for (const PhxScene* sc : held) dropScene(sc);
 }
 
 // Makes `scene` resident for the server: its bytes count against the budget, and the request
 // arena is what remains. Scenes are added before requests are submitted.
-phx_status phx_server_add_scene(PhxServer* s, const PhxScene* scene)
+static phx_status phx_server_add_scene_impl(PhxServer* s, const PhxScene* scene)
 {
 	if (!s || !registry().has(s)) return fail(PHX_ERR_INVALID_HANDLE, "invalid server handle");
 	phx_status st = checkScene(s->ctx, scene);
@@ -472,6 +472,13 @@ phx_status phx_server_add_scene(PhxServer* s, const PhxScene* scene)
 	s->scenes.insert(scene);
 	holdScene(scene);
 	return ok();
+}
+
+phx_status phx_server_add_scene(PhxServer* s, const PhxScene* scene)
+{
+	return guarded("phx_server_add_scene", [&] {
+		return phx_server_add_scene_impl(s, scene);
+	});
 }
 
 static phx_status phx_server_submit_impl(PhxServer* s, const PhxRequest* req, uint64_t* ticket)
@@ -620,8 +627,9 @@ phx_status phx_server_stats(PhxServer* s, PhxServerStats* out)
 	return ok();
 }
 
-phx_status phx_server_replay(PhxServer* s, const PhxRequest* reqs, const double* arrivalMs,
-                             uint32_t n, double* latencyMs, double* makespanMs)
+static phx_status phx_server_replay_impl(PhxServer* s, const PhxRequest* reqs,
+                                         const double* arrivalMs, uint32_t n, double* latencyMs,
+                                         double* makespanMs)
 {
 	if (!s || !registry().has(s)) return fail(PHX_ERR_INVALID_HANDLE, "invalid server handle");
 	if (n && (!reqs || !arrivalMs)) return fail(PHX_ERR_INVALID_ARGUMENT, "phx_server_replay: NULL");
@@ -651,6 +659,14 @@ phx_status phx_server_replay(PhxServer* s, const PhxRequest* reqs, const double*
 	if (makespanMs) *makespanMs = msSince(t0);
 	if (worst != PHX_OK) return fail(worst, "%s", msg.c_str());
 	return ok();
+}
+
+phx_status phx_server_replay(PhxServer* s, const PhxRequest* reqs, const double* arrivalMs,
+                             uint32_t n, double* latencyMs, double* makespanMs)
+{
+	return guarded("phx_server_replay", [&] {
+		return phx_server_replay_impl(s, reqs, arrivalMs, n, latencyMs, makespanMs);
+	});
 }
 
 }  // extern "C"
