@@ -16,7 +16,7 @@ def test_shape_key_orders_the_launch_fields():
 
 
 def test_traffic_is_read_only_for_a_profiled_shape():
-    for lds in (16, 24, 32):
+    for lds in (8, 16, 24, 32):
         t = bench.profiled_traffic(f"256,512,0,8,4,128,{lds}")
         # HBM bytes per launch: well below the 1.75 GB of algorithmic bytes, above the 32 MB of rays
         assert isinstance(t, int) and 32_000_000 < t < 1_754_000_000
