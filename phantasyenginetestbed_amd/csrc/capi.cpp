@@ -12,31 +12,6 @@
 
 #include "runtime.hpp"
 
-namespace phx {
-
-namespace {
-thread_local std::string tLastError;
-}
-
-phx_status fail(phx_status status, const char* fmt, ...)
-{
-	char buf[1024];
-	va_list ap;
-	va_start(ap, fmt);
-	vsnprintf(buf, sizeof(buf), fmt, ap);
-	va_end(ap);
-	tLastError = buf;
-	return status;
-}
-
-phx_status ok()
-{
-	tLastError.clear();
-	return PHX_OK;
-}
-
-}  // namespace phx
-
 using namespace phx;
 
 namespace {
@@ -206,8 +181,6 @@ void dropScene(const PhxScene* cs)
 }  // namespace phx
 
 extern "C" {
-
-const char* phx_last_error(void) { return tLastError.c_str(); }
 
 const char* phx_version(void) { return "phx 0.1.0 gfx950"; }
 

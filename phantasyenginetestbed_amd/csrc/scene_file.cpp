@@ -77,6 +77,10 @@ phx_status readSection(FILE* f, std::vector<T>& out, uint32_t count, uint64_t& h
 	unsigned char pad[64];
 	const size_t p = pad64(bytes);
 	if (p && std::fread(pad, 1, p, f) != p) return fail(PHX_ERR_INVALID_SCENE, "%s: truncated file", path);
+	// Padding is written as zeros. Checking it keeps a count that is one record short, whose last
+	// record then reads as padding under the same checksum, from loading.
+	for (size_t i = 0; i < p; i++)
+		if (pad[i] != 0) return fail(PHX_ERR_INVALID_SCENE, "%s: nonzero section padding", path);
 	h = fnv1a(h, pad, p);
 	return PHX_OK;
 }
@@ -195,6 +199,24 @@ phx_status phx_scene_file_read(const char* path, PhxSceneFile** out)
 	    h.numLights > (1u << 24) || total > (uint64_t(1) << 40)) {
 		std::fclose(f);
 		return fail(PHX_ERR_INVALID_SCENE, "%s: implausible counts", path);
+	}
+	// The counts fix the file's size; checking it first keeps a damaged or truncated header from
+	// sizing the section buffers.
+	const uint64_t sectionBytes[5] = {
+		uint64_t(h.numNodes) * sizeof(PhxBVHNode), uint64_t(h.numTris) * sizeof(PhxTriangleVertices),
+		(h.flags & 1u) ? uint64_t(h.numTris) * sizeof(PhxTriangleData) : 0u,
+		uint64_t(h.numMaterials) * sizeof(PhxMaterial), uint64_t(h.numLights) * sizeof(PhxSphereLight)};
+	uint64_t expected = kHeaderBytes;
+	for (uint64_t b : sectionBytes) expected += b + pad64(size_t(b));
+	long fileBytes = -1;
+	if (std::fseek(f, 0, SEEK_END) == 0) fileBytes = std::ftell(f);
+	if (fileBytes < 0 || uint64_t(fileBytes) != expected ||
+	    std::fseek(f, long(kHeaderBytes), SEEK_SET) != 0) {
+		std::fclose(f);
+		return fail(PHX_ERR_INVALID_SCENE, "%s: %s (%ld bytes, the header's counts give %llu)", path,
+		            fileBytes >= 0 && uint64_t(fileBytes) > expected
+		                ? "trailing bytes after the last section" : "truncated file",
+		            fileBytes, (unsigned long long)expected);
 	}
 	PhxSceneFile* s = new (std::nothrow) PhxSceneFile();
 	if (!s) {

@@ -73,6 +73,11 @@ def test_damaged_files_are_refused(tmp_path, scene):
     assert "version" in refused(bad)
     assert "truncated" in refused(raw[:-100])
     assert "trailing" in refused(raw + b"\0" * 64)
+    # A header whose counts the file cannot hold is refused before any section is sized from
+    # it (here 2^27 triangles: 15 GB of triangle and data sections).
+    big = bytearray(raw)
+    big[20:24] = (1 << 27).to_bytes(4, "little")
+    assert "truncated" in refused(big)
     flipped = bytearray(raw)
     flipped[64 + 5] ^= 0x40  # a bit inside the first node
     assert "checksum" in refused(flipped)
