@@ -13,6 +13,12 @@
  *    otherwise they are host pointers and the call copies through a staging buffer.
  *  - A context owns one HIP stream. Calls on one context are issued in order on that stream.
  *    Calls without PHX_ASYNC return after the work has completed.
+ *  - Threads: different contexts may be used from different threads at once. Calls on one
+ *    context (and on the scenes, plans and renderers made from it) must not overlap in time: they
+ *    share the context's staging buffers, ray counters and launch shape. Serialise them, or give
+ *    each thread a context of its own. A server's dispatcher thread is internal to the server: it
+ *    only enqueues on the server's slot streams (slot 0 is the context's stream) with the server's
+ *    own counters, so calls on the context may run beside it.
  */
 #ifndef PHX_H
 #define PHX_H
