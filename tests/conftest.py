@@ -24,3 +24,23 @@ def golden():
 def load_fixture(name):
     import numpy as np
     return dict(np.load(GOLDEN / f"{name}.npz"))
+
+
+def tiny_scene(n, rng):
+    """n random triangles in [-1, 1]^3 and 4,000 rays, every other one aimed at a triangle's
+    centroid."""
+    import numpy as np
+    from phantasyenginetestbed_amd import phx
+    tris = np.zeros(n, phx.TRI)
+    tris["v"][:, :, :3] = rng.uniform(-1, 1, (n, 3, 3)).astype(np.float32)
+    rays = np.zeros(4000, phx.RAY)
+    rays["o"] = rng.uniform(-2, 2, (len(rays), 3)).astype(np.float32)
+    # every other ray aimed at the centroid of a random triangle, so many of them hit
+    target = tris["v"][rng.integers(0, n, len(rays)), :, :3].mean(axis=1)
+    d = rng.normal(size=(len(rays), 3))
+    d[::2] = target[::2] - rays["o"][::2]
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmin"] = np.float32(1e-4)
+    rays["tmax"] = np.float32(phx.FLT_MAX)
+    return tris, rays
+

@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import load_fixture
+from conftest import load_fixture, tiny_scene
 from oracle import oracle as O
 from phantasyenginetestbed_amd import phx
 
@@ -335,6 +335,63 @@ def test_fused_plan_with_tiny_batches(ctx, c1, rays_per_fetch):
         rb.free()
         ob.free()
 
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_tiny_scenes_match_oracle(ctx, n, kernel):
+    """Scenes of one to five triangles are a lone root with two leaf children (one leaf in both
+    slots where the SBVH's root is a leaf): closest hits, counters and any hits equal the oracle
+    in every launch model, and a fused plan over the four scenes equals the per-scene casts."""
+    say(f"[gpu] tiny scene of {n} triangles, kernel {kernel}")
+    tris, rays = tiny_scene(n, np.random.default_rng(100 + n))
+    b = phx.BVH(tris, None)
+    s = ctx.upload_scene(b.nodes, b.tris)
+    try:
+        assert_same_hits(ctx.cast_closest(s, rays, kernel), O.cast_closest(b.nodes, b.tris, rays),
+                         f"{n} triangles")
+        _, tot, _ = O.cast_debug(b.nodes, b.tris, rays)
+        assert ctx.traversal_stats(s, rays, kernel) == tot
+        for tmax in (np.float32(0.5), np.float32(phx.FLT_MAX)):
+            r = rays.copy()
+            r["tmax"] = tmax
+            assert (ctx.cast_anyhit(s, r, kernel) == O.cast_anyhit(b.nodes, b.tris, r)).all()
+    finally:
+        s.release()
+
+
+def test_fused_plan_over_tiny_scenes(ctx):
+    scenes, data, batches, bufs = [], [], [], []
+    try:
+        for n in (1, 2, 3, 5):
+            tris, rays = tiny_scene(n, np.random.default_rng(200 + n))
+            b = phx.BVH(tris, None)
+            s = ctx.upload_scene(b.nodes, b.tris)
+            scenes.append(s)
+            rb = ctx.alloc(len(rays) * 32).upload(rays)
+            ob = ctx.alloc(len(rays) * 16)
+            bufs += [rb, ob]
+            kind = phx.BATCH_ANYHIT if n == 3 else phx.BATCH_CLOSEST
+            batches.append((s, rb.ptr, ob.ptr, len(rays), kind))
+            data.append((b, rays, ob, kind))
+        plan = phx.Plan(ctx, batches)
+        try:
+            for graph in (False, True):
+                plan.launch(graph=graph)
+                for b, rays, ob, kind in data:
+                    if kind == phx.BATCH_ANYHIT:
+                        got = ob.download(np.uint8, len(rays))
+                        assert (got == O.cast_anyhit(b.nodes, b.tris, rays)).all()
+                    else:
+                        assert_same_hits(ob.download(phx.HIT, len(rays)),
+                                         O.cast_closest(b.nodes, b.tris, rays), "fused tiny")
+        finally:
+            plan.release()
+    finally:
+        for x in bufs:
+            x.free()
+        for s in scenes:
+            s.release()
 
 def test_scene_file_upload_matches_oracle(ctx, tmp_path):
     """A scene written to and read back from a scene file traces exactly like the built one."""
