@@ -44,3 +44,29 @@ def tiny_scene(n, rng):
     rays["tmax"] = np.float32(phx.FLT_MAX)
     return tris, rays
 
+
+def degenerate_rays(fx, rng):
+    """Rays the API accepts but no camera makes: zero, NaN and infinite directions or origins,
+    subnormal and negative-zero direction components, empty and inverted [tMin, tMax], tMax 0,
+    negative or NaN, NaN tMin. Each case is applied to 64 rays of fixture `fx` that hit as they
+    are; the result is shuffled so the cases mix within waves."""
+    import numpy as np
+    nan, inf = np.float32(np.nan), np.float32(np.inf)
+    cases = [  # (field, component or None for all, value)
+        [("d", None, 0.0)], [("d", 1, nan)], [("o", 0, nan)], [("d", 2, inf)], [("o", 1, -inf)],
+        [("d", 0, np.float32(1e-41))], [("d", 1, np.float32(-0.0))],
+        [("tmin", None, 2.0), ("tmax", None, 1.0)], [("tmax", None, 0.0)],
+        [("tmax", None, -1.0)], [("tmin", None, nan)], [("tmax", None, nan)],
+        [("tmax", None, inf)], [("o", None, np.float32(3e38))]]
+    base = np.ascontiguousarray(fx["rays"][fx["hits"]["tri"] != 0xFFFFFFFF][:64])
+    out = []
+    for case in cases:
+        r = base.copy()
+        for field, comp, value in case:
+            if comp is None:
+                r[field] = value
+            else:
+                r[field][:, comp] = value
+        out.append(r)
+    rays = np.concatenate(out)
+    return rays[rng.permutation(len(rays))]

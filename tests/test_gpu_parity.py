@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import load_fixture, tiny_scene
+from conftest import degenerate_rays, load_fixture, tiny_scene
 from oracle import oracle as O
 from phantasyenginetestbed_amd import phx
 
@@ -392,6 +392,44 @@ def test_fused_plan_over_tiny_scenes(ctx):
             x.free()
         for s in scenes:
             s.release()
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=KERNEL_IDS)
+@pytest.mark.parametrize("name", ["rand2k", "hf32", "edges"])
+def test_degenerate_rays_match_oracle(ctx, name, kernel):
+    """Zero, NaN and infinite ray components, subnormal directions and empty, inverted or NaN
+    intervals, mixed within waves: closest hits, counters and any hits equal the oracle (itself
+    equal to the reference's compiled castRay on these rays) in every launch model, and in a
+    fused plan."""
+    say(f"[gpu] degenerate rays on {name}, kernel {kernel}")
+    fx = load_fixture(name)
+    rays = degenerate_rays(fx, np.random.default_rng(7))
+    s = ctx.upload_scene(fx["nodes"], fx["tris"])
+    try:
+        with np.errstate(all="ignore"):
+            ref = O.cast_closest(fx["nodes"], fx["tris"], rays)
+            _, tot, _ = O.cast_debug(fx["nodes"], fx["tris"], rays)
+            lit_ref = O.cast_anyhit(fx["nodes"], fx["tris"], rays)
+        assert (ref["tri"] != phx.MISS).sum() > 50
+        assert_same_hits(ctx.cast_closest(s, rays, kernel), ref, name)
+        assert ctx.traversal_stats(s, rays, kernel) == tot
+        assert (ctx.cast_anyhit(s, rays, kernel) == lit_ref).all()
+        if kernel == 0:
+            rb = ctx.alloc(len(rays) * 32).upload(rays)
+            ob = ctx.alloc(len(rays) * 16)
+            lb = ctx.alloc(len(rays))
+            plan = phx.Plan(ctx, [(s, rb.ptr, ob.ptr, len(rays), phx.BATCH_CLOSEST),
+                                  (s, rb.ptr, lb.ptr, len(rays), phx.BATCH_ANYHIT)])
+            try:
+                plan.launch()
+                assert_same_hits(ob.download(phx.HIT, len(rays)), ref, f"{name} fused")
+                assert (lb.download(np.uint8, len(rays)) == lit_ref).all()
+            finally:
+                plan.release()
+                for x in (rb, ob, lb):
+                    x.free()
+    finally:
+        s.release()
 
 def test_scene_file_upload_matches_oracle(ctx, tmp_path):
     """A scene written to and read back from a scene file traces exactly like the built one."""

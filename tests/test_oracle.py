@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import load_fixture, tiny_scene
+from conftest import degenerate_rays, load_fixture, tiny_scene
 from oracle import oracle as O
 from phantasyenginetestbed_amd import phx
 
@@ -234,3 +234,17 @@ def test_tiny_scenes_agree_with_brute_force_and_reference(n):
     assert ((lit == 0) == hit).all()
     if O.have_ref():
         assert O.ref_cast(bvh.nodes, bvh.tris, rays, 1).tobytes() == hits.tobytes()
+
+
+@pytest.mark.parametrize("name", ["rand2k", "hf32", "edges"])
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (needs the reference tree)")
+def test_degenerate_rays_match_reference(name):
+    """NaN, infinite, zero and subnormal ray components and empty or inverted intervals: the
+    oracle's closest hits (and its counters' hits) equal the reference's compiled castRay."""
+    fx = load_fixture(name)
+    rays = degenerate_rays(fx, np.random.default_rng(7))
+    with np.errstate(all="ignore"):
+        ref = O.ref_cast(fx["nodes"], fx["tris"], rays, 1)
+        assert (ref["tri"] != phx.MISS).sum() > 50
+        assert O.cast_closest(fx["nodes"], fx["tris"], rays).tobytes() == ref.tobytes()
+        assert O.cast_debug(fx["nodes"], fx["tris"], rays)[0].tobytes() == ref.tobytes()
