@@ -337,8 +337,10 @@ typedef struct PhxSceneFile PhxSceneFile;
  * (magic "PHXSCENE", version, counts, flags, FNV-1a 64 checksum of the payload) followed by the
  * nodes, triangles, datas, materials and lights, each section 64-byte aligned. */
 phx_status phx_scene_file_write(const char* path, const PhxSceneArrays* arrays);
-/* Reads a scene file; rejects a wrong magic or version, truncation, trailing bytes, a checksum
- * mismatch and a structurally invalid BVH with PHX_ERR_INVALID_SCENE / PHX_ERR_MALFORMED_BVH. */
+/* Reads a scene file; rejects a wrong magic or version, a size that differs from what the
+ * header's counts give (truncation, trailing bytes; checked before any buffer is sized), nonzero
+ * section padding, a checksum mismatch and a structurally invalid BVH with PHX_ERR_INVALID_SCENE /
+ * PHX_ERR_MALFORMED_BVH. */
 phx_status phx_scene_file_read(const char* path, PhxSceneFile** outFile);
 /* Imports the reference's raw cache files (BVHNode, TriangleVertices and, if datasPath is not
  * NULL, TriangleData records) with the same validation (Main.cpp:120-134 checks sizes only). */
