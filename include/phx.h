@@ -201,6 +201,12 @@ typedef struct PhxScene PhxScene;
 /* Replaces the CudaTracerRenderer constructor's device setup (CudaTracerRenderer.cpp:123-167):
  * binds `device`, reads its properties, creates the context stream. No device reset on destroy. */
 phx_status phx_create(int device, PhxContext** outCtx);
+/* The context handle is invalid once this returns. Scenes, plans, renderers, instance sets and
+ * servers made from it stay valid until they are released: calls that take the context itself
+ * (casts, plan launches, copies) fail with PHX_ERR_INVALID_HANDLE, a server keeps serving. The
+ * context's stream and buffers are freed with the last of them, so any release order is safe.
+ * Device buffers from phx_device_alloc are plain hipMalloc allocations: free them first, or with
+ * hipFree afterwards. */
 void phx_destroy(PhxContext* ctx);
 
 /* The context's hipStream_t, as an opaque pointer (for events / interop). */

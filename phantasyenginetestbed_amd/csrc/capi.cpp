@@ -65,6 +65,18 @@ void freeScene(PhxScene* s)
 	delete s;
 }
 
+void freeContext(PhxContext* ctx)
+{
+	hipSetDevice(ctx->device);
+	hipStreamSynchronize(ctx->stream);
+	hipFree(ctx->stageIn);
+	hipFree(ctx->stageOut);
+	hipFree(ctx->work);
+	hipFree(ctx->spill);
+	hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
 std::mutex& sceneHoldMutex()
 {
 	static std::mutex mu;
@@ -173,9 +185,26 @@ void dropScene(const PhxScene* cs)
 		std::lock_guard<std::mutex> g(sceneHoldMutex());
 		if (--s->holders != 0 || !s->released) return;
 	}
-	hipSetDevice(s->ctx->device);
-	hipStreamSynchronize(s->ctx->stream);
+	PhxContext* ctx = s->ctx;
+	hipSetDevice(ctx->device);
+	hipStreamSynchronize(ctx->stream);
 	freeScene(s);
+	dropContext(ctx);
+}
+
+void holdContext(PhxContext* ctx)
+{
+	std::lock_guard<std::mutex> g(sceneHoldMutex());
+	ctx->holders++;
+}
+
+void dropContext(PhxContext* ctx)
+{
+	{
+		std::lock_guard<std::mutex> g(sceneHoldMutex());
+		if (--ctx->holders != 0 || !ctx->released) return;
+	}
+	freeContext(ctx);
 }
 
 }  // namespace phx
@@ -329,14 +358,14 @@ phx_status phx_check_reciprocal(PhxContext* ctx, uint32_t flags, uint64_t counts
 void phx_destroy(PhxContext* ctx)
 {
 	if (!ctx || !registry().remove(ctx)) return;
-	hipSetDevice(ctx->device);
-	hipStreamSynchronize(ctx->stream);
-	hipFree(ctx->stageIn);
-	hipFree(ctx->stageOut);
-	hipFree(ctx->work);
-	hipFree(ctx->spill);
-	hipStreamDestroy(ctx->stream);
-	delete ctx;
+	{
+		std::lock_guard<std::mutex> g(sceneHoldMutex());
+		if (ctx->holders) {  // freed when the last object made from it is released
+			ctx->released = true;
+			return;
+		}
+	}
+	freeContext(ctx);
 }
 
 void* phx_context_stream(PhxContext* ctx)
@@ -560,6 +589,7 @@ static phx_status phx_scene_upload_impl(PhxContext* ctx, const PhxBVHNode* nodes
 		return st;
 	}
 	s->version = nextSceneStamp();
+	holdContext(ctx);
 	registry().add(s);
 	*outScene = s;
 	return ok();
@@ -587,9 +617,11 @@ void phx_scene_release(PhxScene* scene)
 			return;
 		}
 	}
-	hipSetDevice(scene->ctx->device);
-	hipStreamSynchronize(scene->ctx->stream);
+	PhxContext* ctx = scene->ctx;
+	hipSetDevice(ctx->device);
+	hipStreamSynchronize(ctx->stream);
 	freeScene(scene);
+	dropContext(ctx);
 }
 
 
@@ -1032,6 +1064,7 @@ static phx_status phx_plan_create_impl(PhxContext* ctx, const PhxBatch* batches,
 			p->scenes.push_back(batches[b].scene);
 			holdScene(batches[b].scene);
 		}
+	holdContext(ctx);
 	registry().add(p);
 	*outPlan = p;
 	return ok();
@@ -1400,7 +1433,9 @@ void phx_plan_release(PhxPlan* plan)
 	hipFree(plan->offsets);
 	hipFree(plan->clock);
 	for (const PhxScene* sc : plan->scenes) dropScene(sc);
+	PhxContext* ctx = plan->ctx;
 	delete plan;
+	dropContext(ctx);
 }
 
 phx_status phx_gen_primary_rays(PhxContext* ctx, const PhxCameraDef* cam, uint32_t width,

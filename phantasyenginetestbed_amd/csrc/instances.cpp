@@ -282,6 +282,7 @@ static phx_status phx_instances_build_impl(PhxContext* ctx, const PhxInstance* i
 			s->meshes.push_back(instances[i].mesh);
 			holdScene(instances[i].mesh);
 		}
+	holdContext(ctx);
 	registry().add(s);
 	*outSet = s;
 	return ok();
@@ -298,9 +299,11 @@ phx_status phx_instances_build(PhxContext* ctx, const PhxInstance* instances, ui
 void phx_instances_release(PhxInstanceSet* set)
 {
 	if (!set || !registry().remove(set)) return;
-	hipSetDevice(set->ctx->device);
-	hipStreamSynchronize(set->ctx->stream);
+	PhxContext* ctx = set->ctx;
+	hipSetDevice(ctx->device);
+	hipStreamSynchronize(ctx->stream);
 	freeSet(set);
+	dropContext(ctx);
 }
 
 namespace {

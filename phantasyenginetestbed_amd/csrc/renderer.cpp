@@ -276,6 +276,7 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 		return st;
 	}
 	r->lastOut = r->color;
+	holdContext(ctx);
 	registry().add(r);
 	*outRenderer = r;
 	return ok();
@@ -284,9 +285,11 @@ phx_status phx_renderer_create(PhxContext* ctx, uint32_t width, uint32_t height,
 void phx_renderer_release(PhxRenderer* r)
 {
 	if (!r || !registry().remove(r)) return;
-	hipSetDevice(r->ctx->device);
-	hipStreamSynchronize(r->ctx->stream);
+	PhxContext* ctx = r->ctx;
+	hipSetDevice(ctx->device);
+	hipStreamSynchronize(ctx->stream);
 	freeRenderer(r);
+	dropContext(ctx);
 }
 
 phx_status phx_renderer_buffer(PhxRenderer* r, uint32_t which, const void** devPtr, size_t* bytes)

@@ -79,6 +79,11 @@ struct PhxContext {
 	size_t stageInBytes = 0;
 	void* stageOut = nullptr;
 	size_t stageOutBytes = 0;
+	// Scenes, plans, renderers, instance sets and servers made from this context (guarded by
+	// sceneHoldMutex()). A context destroyed while any of them exists leaves the registry at once
+	// but keeps its stream and buffers until the last of them is released.
+	uint32_t holders = 0;
+	bool released = false;
 };
 
 struct PhxPlan {
@@ -155,6 +160,9 @@ uint64_t nextSceneStamp();
 // frees it.
 void holdScene(const PhxScene* s);
 void dropScene(const PhxScene* s);
+// A context's holders: every object made from it holds it from creation to release.
+void holdContext(PhxContext* ctx);
+void dropContext(PhxContext* ctx);
 
 inline SceneView view(const PhxScene* s)
 {

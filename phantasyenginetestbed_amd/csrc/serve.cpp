@@ -17,6 +17,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <unordered_set>
@@ -414,8 +415,16 @@ static phx_status phx_server_create_impl(PhxContext* ctx, uint64_t budgetBytes, 
 		return hipFail(e, "phx_server_create");
 	}
 	s->arena.reset(0);
+	holdContext(ctx);
 	registry().add(s);
-	s->dispatcher = std::thread(dispatchLoop, s);
+	try {
+		s->dispatcher = std::thread(dispatchLoop, s);
+	} catch (const std::system_error& err) {
+		registry().remove(s);
+		destroyServer(s);
+		dropContext(ctx);
+		return fail(PHX_ERR_OUT_OF_MEMORY, "phx_server_create: dispatcher thread: %s", err.what());
+	}
 	*outServer = s;
 	return ok();
 }
@@ -438,10 +447,12 @@ void phx_server_destroy(PhxServer* s)
 	}
 	s->cvWork.notify_all();
 	if (s->dispatcher.joinable()) s->dispatcher.join();
-	hipSetDevice(s->ctx->device);
+	PhxContext* ctx = s->ctx;
+	hipSetDevice(ctx->device);
 	const std::unordered_set<const PhxScene*> held = std::move(s->scenes);  // no allocation
 	destroyServer(s);
 	for (const PhxScene* sc : held) dropScene(sc);
+	dropContext(ctx);
 }
 
 // Makes `scene` resident for the server: its bytes count against the budget, and the request

@@ -152,6 +152,50 @@ def test_released_scene_stays_valid_for_its_plan_and_server(ctx):
         hbuf.free()
 
 
+
+def test_context_destroyed_before_its_objects(ctx):
+    """phx_destroy while a scene, plan, renderer, instance set and server made from the context
+    still exist: the handle is refused at once (so is a plan launch, which takes it), the server
+    still serves with the right hits, and releasing the objects afterwards, scene first, frees
+    the rest."""
+    say("[gpu-life] destroy a context before its objects")
+    c = phx.Context(0)
+    bvh, _, _, s = heightfield(c, 40, 20)
+    rays = c.gen_primary_rays(W.camera(96, 64), 96, 64)
+    want = O.cast_closest(bvh.nodes, bvh.tris, rays)
+    # buffers of the long-lived context, readable after `c` is gone
+    rbuf = ctx.alloc(len(rays) * 32).upload(rays)
+    hbuf = ctx.alloc(len(rays) * 16)
+    sbuf = ctx.alloc(len(rays) * 16)
+    plan = phx.Plan(c, [(s, rbuf.ptr, hbuf.ptr, len(rays), phx.BATCH_CLOSEST)])
+    plan.launch()
+    assert hbuf.download(phx.HIT, len(rays)).tobytes() == want.tobytes()
+    r = c.renderer(64, 48)
+    iset = phx.InstanceSet(c, [(s, np.eye(3, 4, dtype=np.float32))])
+    srv = phx.Server(c, 1 << 28, slots=2)
+    srv.add_scene(s)
+    handle = c.handle
+    assert phx.lib().phx_context_num_cus(handle) > 0
+    c.destroy()
+    try:
+        assert phx.lib().phx_context_num_cus(handle) == 0  # the handle is gone at once
+        with pytest.raises(phx.PhxError) as ei:
+            plan.launch(graph=True)
+        assert ei.value.status == phx.ERR_INVALID_HANDLE
+        t = srv.submit(phx.Request(s.handle, rbuf.ptr, sbuf.ptr, len(rays), phx.BATCH_CLOSEST,
+                                   phx.MEM_DEVICE, 0))
+        srv.wait(t)
+        assert sbuf.download(phx.HIT, len(rays)).tobytes() == want.tobytes()
+    finally:
+        # any order: the scene first (still held by the plan, the set and the server)
+        s.release()
+        srv.destroy()
+        r.release()
+        plan.release()
+        iset.release()  # the last holder: the context's stream and buffers go here
+        for b in (rbuf, hbuf, sbuf):
+            b.free()
+
 def test_frame_graph_recaptures_for_a_new_scene(ctx):
     """Release a scene, upload another (often at the same host address): a renderer replaying
     captured frames must capture again and render the new scene exactly as a direct frame."""
