@@ -646,7 +646,10 @@ def main() -> int:
                      "algorithmic_bytes_per_ray": round(bytes_per_ray, 1)},
     }
     if not args.no_overlap and kernel == "persistent":
-        result["overlapped"] = overlapped_form(ctx, scene, bvh, rays, n, gpu_hits)
+        try:  # a side measurement: its failure must not cost the headline line
+            result["overlapped"] = overlapped_form(ctx, scene, bvh, rays, n, gpu_hits)
+        except phx.PhxError as e:
+            result["overlapped"] = {"error": str(e)}
         log(f"overlapped: {result['overlapped']}")
     if not args.no_batched:
         plan_launch = tuple(int(x) for x in args.plan_launch.split(",")) if args.plan_launch \
