@@ -412,6 +412,44 @@ def overlapped_form(ctx, scene, bvh, rays, n: int, want: np.ndarray, launches: i
                     "so one launch's completion tail runs beside the next batch's rays"}
 
 
+def instances_form(ctx, reps: int = 5) -> dict:
+    """Dynamic meshes: 1M rays against 300 instances of three meshes under a top tree
+    (phx_cast_closest_instances / phx_cast_anyhit_instances, device buffers), median of `reps`
+    casts each after a warm-up."""
+    from phantasyenginetestbed_amd import phx, workloads as W
+    wl = W.instance_workload(ctx)
+    n = len(wl.rays)
+    iset = phx.InstanceSet(ctx, wl.instances)
+    rb = ctx.alloc(n * 32).upload(wl.rays)
+    hb, ib, lb = ctx.alloc(n * 16), ctx.alloc(n * 4), ctx.alloc(n)
+    try:
+        def closest():
+            iset.cast_closest_device(rb.ptr, n, hb.ptr, ib.ptr)
+
+        def anyhit():
+            phx.check(phx.lib().phx_cast_anyhit_instances(ctx.handle, None, iset.handle, rb.ptr,
+                                                          n, lb.ptr, phx.MEM_DEVICE))
+        out = {"instances": len(wl.instances), "mesh_tris": [300, 4000, 20000], "rays": n}
+        for name, fn in (("closest", closest), ("anyhit", anyhit)):
+            fn()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+            ms = float(np.median(ts)) * 1e3
+            out[name] = {"ms": round(ms, 3), "mrays": round(n / ms / 1e3, 1)}
+        hits = hb.download(phx.HIT, n)
+        out["closest"]["hit_fraction"] = round(float((hits["tri"] != phx.MISS).mean()), 4)
+        out["anyhit"]["occluded_fraction"] = round(float((lb.download(np.uint8, n) == 0).mean()), 4)
+    finally:
+        iset.release()
+        for b in (rb, hb, ib, lb):
+            b.free()
+        for m in wl.meshes:
+            m.release()
+    return out
+
 def served_form(ctx, n_requests: int = 1500, budget_gb: float = 64.0, slots: int = 3) -> dict:
     """Poisson-arriving requests against 4 resident scenes through the request server, at
     saturation (all requests at once; after a warm-up, the median of 5 runs by makespan) and at
@@ -506,6 +544,7 @@ def main() -> int:
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-overlap", action="store_true")
+    ap.add_argument("--no-instances", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default=None,
                     metavar="GRID,BLOCK,TOP,REFILL[,HELPERS[,RAYS_PER_FETCH[,LDS_STACK]]]",
@@ -651,6 +690,9 @@ def main() -> int:
         except phx.PhxError as e:
             result["overlapped"] = {"error": str(e)}
         log(f"overlapped: {result['overlapped']}")
+    if not args.no_instances:
+        result["instances"] = instances_form(ctx)
+        log(f"instances: {result['instances']}")
     if not args.no_batched:
         plan_launch = tuple(int(x) for x in args.plan_launch.split(",")) if args.plan_launch \
             else None

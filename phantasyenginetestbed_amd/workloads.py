@@ -286,3 +286,48 @@ def request_output(rs: RequestSet, i: int) -> np.ndarray:
 def poisson_arrivals(n: int, rate_per_ms: float, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return np.cumsum(rng.exponential(1.0 / rate_per_ms, n))
+
+
+INSTANCE_SEED = 0xC0FFEE05
+
+
+@dataclass
+class InstanceWorkload:
+    meshes: list       # phx.Scene per mesh
+    instances: list    # (phx.Scene, 3x4 object -> world matrix)
+    rays: np.ndarray
+
+
+def instance_workload(ctx: phx.Context, count: int = 300, num_rays: int = 1 << 20,
+                      seed: int = INSTANCE_SEED) -> InstanceWorkload:
+    """Dynamic-mesh workload: three random-triangle meshes (300 / 4,000 / 20,000 triangles, edges
+    0.01-0.1) placed as `count` instances (a random rotation from a unit quaternion, scale 0.5-2
+    per axis, every 7th instance mirrored, positions uniform in [-6, 6]^3), and `num_rays` rays
+    from origins uniform in [-9, 9]^3 towards points uniform in [-6, 6]^3 (tMin 1e-4)."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for k, n in enumerate((300, 4000, 20000)):
+        t, d = phx.random_triangles(n, seed + k, 0.01, 0.1)
+        b = phx.BVH(t, d)
+        meshes.append(ctx.upload_scene(b.nodes, b.tris, b.datas))
+    instances = []
+    for k in range(count):
+        q = rng.normal(size=4)
+        w, x, y, z = q / np.linalg.norm(q)
+        rot = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        scale = rng.uniform(0.5, 2.0, 3)
+        if k % 7 == 3:
+            scale[0] = -scale[0]
+        m = np.zeros((3, 4))
+        m[:, :3] = rot @ np.diag(scale)
+        m[:, 3] = rng.uniform(-6, 6, 3)
+        instances.append((meshes[k % 3], m.astype(np.float32)))
+    rays = np.zeros(num_rays, phx.RAY)
+    rays["o"] = rng.uniform(-9, 9, (num_rays, 3))
+    d = rng.uniform(-6, 6, (num_rays, 3)) - rays["o"]
+    rays["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmin"] = np.float32(1e-4)
+    rays["tmax"] = np.float32(phx.FLT_MAX)
+    return InstanceWorkload(meshes, instances, rays)
