@@ -691,7 +691,10 @@ def main() -> int:
             result["overlapped"] = {"error": str(e)}
         log(f"overlapped: {result['overlapped']}")
     if not args.no_instances:
-        result["instances"] = instances_form(ctx)
+        try:  # a side measurement: its failure must not cost the headline line
+            result["instances"] = instances_form(ctx)
+        except phx.PhxError as e:
+            result["instances"] = {"error": str(e)}
         log(f"instances: {result['instances']}")
     if not args.no_batched:
         plan_launch = tuple(int(x) for x in args.plan_launch.split(",")) if args.plan_launch \
