@@ -121,17 +121,43 @@ constexpr int kSimpleStack = 72;
 // castRay<144> of one ray (BVHTraversal.hpp:37-128) into `c` (which holds the miss result on
 // entry): pop, decide both children, left first; a visited leaf is tested at once, a visited
 // inner node pushed.
-template <bool kCount>
+// castRay's pending stack for one lane: its first kLds entries in the lane's LDS column (entry e
+// at col[e * kBlock], one 4-B word per lane per row, so a wave's accesses are conflict-free), the
+// rest in private memory. A pop is on the path to the next node's load, and an LDS pop returns
+// sooner than a private (scratch) one.
+template <int kLds>
+struct ColumnStack {
+	uint32_t* col;
+	uint32_t deep[kSimpleStack - kLds];
+	__device__ __forceinline__ void put(int e, uint32_t v)
+	{
+		if (e < kLds)
+			col[e * kBlock] = v;
+		else
+			deep[e - kLds] = v;
+	}
+	__device__ __forceinline__ uint32_t get(int e) const
+	{
+		return e < kLds ? col[e * kBlock] : deep[e - kLds];
+	}
+};
+
+// LDS stack entries per lane: 24 for one-ray-per-lane casts (24 KB per 256-thread block), 16 for
+// the castRay of an entered instance (16 KB). Larger splits lower the blocks a CU holds and
+// measured slower.
+constexpr int kSimpleLds = 24;
+constexpr int kInstanceLds = 16;
+
+template <bool kCount, class Stack>
 __device__ __forceinline__ void closest_one(const float4* __restrict__ nodes,
                                             const float4* __restrict__ tris, const dev::Ray& r,
-                                            Closest& c, Counts& k)
+                                            Closest& c, Counts& k, Stack& stack)
 {
-	uint32_t stack[kSimpleStack];
-	stack[0] = 0u;
+	stack.put(0, 0u);
 	int sp = 1;
 	while (sp > 0) {
 		sp--;
-		const NodeWords nd = load_node(nodes, stack[sp]);
+		const NodeWords nd = load_node(nodes, stack.get(sp));
 		if (kCount) k.nodes++;
 		float inL, outL, inR, outR;
 		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
@@ -142,7 +168,7 @@ __device__ __forceinline__ void closest_one(const float4* __restrict__ nodes,
 		if (visitL) {
 			if (kCount) k.boxHits++;
 			if (nd.w3.z == 0) {
-				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.x);
+				if (sp < kSimpleStack) stack.put(sp++, child_index(nd.w3.x));
 			} else {
 				closest_leaf<kCount>(r, tris, child_index(nd.w3.x), nd.w3.z, c, k);
 			}
@@ -150,7 +176,7 @@ __device__ __forceinline__ void closest_one(const float4* __restrict__ nodes,
 		if (visitR) {
 			if (kCount) k.boxHits++;
 			if (nd.w3.w == 0) {
-				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.y);
+				if (sp < kSimpleStack) stack.put(sp++, child_index(nd.w3.y));
 			} else {
 				closest_leaf<kCount>(r, tris, child_index(nd.w3.y), nd.w3.w, c, k);
 			}
@@ -177,18 +203,17 @@ __device__ __forceinline__ bool anyhit_leaf_counted(const dev::Ray& r,
 
 // Any hit of one ray: castRay's traversal with tCurrMax = tMax, stopping at the first triangle
 // with tMin <= t < tMax. kCount adds castDebugRay-style counters of that traversal to k.
-template <bool kCount>
+template <bool kCount, class Stack>
 __device__ __forceinline__ bool anyhit_one(const float4* __restrict__ nodes,
                                            const float4* __restrict__ tris, const dev::Ray& r,
-                                           Counts& k)
+                                           Counts& k, Stack& stack)
 {
-	uint32_t stack[kSimpleStack];
-	stack[0] = 0u;
+	stack.put(0, 0u);
 	int sp = 1;
 	bool occluded = false;
 	while (sp > 0 && !occluded) {
 		sp--;
-		const NodeWords nd = load_node(nodes, stack[sp]);
+		const NodeWords nd = load_node(nodes, stack.get(sp));
 		if (kCount) k.nodes++;
 		float inL, outL, inR, outR;
 		dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
@@ -198,7 +223,7 @@ __device__ __forceinline__ bool anyhit_one(const float4* __restrict__ nodes,
 		if (visitL) {
 			if (kCount) k.boxHits++;
 			if (nd.w3.z == 0) {
-				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.x);
+				if (sp < kSimpleStack) stack.put(sp++, child_index(nd.w3.x));
 			} else {
 				occluded = kCount ? anyhit_leaf_counted(r, tris, child_index(nd.w3.x), nd.w3.z, k)
 				                  : anyhit_leaf(r, tris, child_index(nd.w3.x), nd.w3.z);
@@ -207,7 +232,7 @@ __device__ __forceinline__ bool anyhit_one(const float4* __restrict__ nodes,
 		if (visitR && !occluded) {
 			if (kCount) k.boxHits++;
 			if (nd.w3.w == 0) {
-				if (sp < kSimpleStack) stack[sp++] = child_index(nd.w3.y);
+				if (sp < kSimpleStack) stack.put(sp++, child_index(nd.w3.y));
 			} else {
 				occluded = kCount ? anyhit_leaf_counted(r, tris, child_index(nd.w3.y), nd.w3.w, k)
 				                  : anyhit_leaf(r, tris, child_index(nd.w3.y), nd.w3.w);
@@ -223,12 +248,15 @@ __global__ __launch_bounds__(kBlock) void k_closest_simple(
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t n,
 	unsigned long long* __restrict__ counters)
 {
+	__shared__ uint32_t stackLds[kSimpleLds * kBlock];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
 	Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
 	Counts k;
-	closest_one<kCount>(nodes, tris, r, c, k);
+	ColumnStack<kSimpleLds> st;
+	st.col = stackLds + threadIdx.x;
+	closest_one<kCount>(nodes, tris, r, c, k, st);
 	hits[i] = make_uint4(c.tri, __float_as_uint(c.t), __float_as_uint(c.u), __float_as_uint(c.v));
 	if (kCount) add_counts(counters, k);
 }
@@ -240,11 +268,14 @@ __global__ __launch_bounds__(kBlock) void k_anyhit_simple(const float4* __restri
                                                           uint8_t* __restrict__ inLight, uint32_t n,
                                                           unsigned long long* __restrict__ counters)
 {
+	__shared__ uint32_t stackLds[kSimpleLds * kBlock];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
 	Counts k;
-	inLight[i] = anyhit_one<kCount>(nodes, tris, r, k) ? 0 : 1;
+	ColumnStack<kSimpleLds> st;
+	st.col = stackLds + threadIdx.x;
+	inLight[i] = anyhit_one<kCount>(nodes, tris, r, k, st) ? 0 : 1;
 	if (kCount) add_counts(counters, k);
 }
 
@@ -279,11 +310,14 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 	const float4* __restrict__ rays, uint4* __restrict__ hits, uint32_t* __restrict__ instOut,
 	uint8_t* __restrict__ inLight, uint32_t n, uint32_t seeded)
 {
+	__shared__ uint32_t innerLds[kInstanceLds * kBlock];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const dev::Ray r = dev::make_ray(rays[2 * size_t(i)], rays[2 * size_t(i) + 1]);
 	Closest best = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
 	uint32_t bestInst = kInvalid;
+	ColumnStack<kInstanceLds> inner;  // the entered instance's castRay stack
+	inner.col = innerLds + threadIdx.x;
 	bool seededHit = false;
 	if (!kAnyHit && seeded) {
 		const uint4 h = hits[i];
@@ -318,11 +352,11 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 				const dev::Ray o = object_ray(in, r);
 				if (kAnyHit) {
 					Counts none;
-					occluded = anyhit_one<false>(in.nodes, in.tris, o, none);
+					occluded = anyhit_one<false>(in.nodes, in.tris, o, none, inner);
 				} else {
 					Closest c = {kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
 					Counts none;
-					closest_one<false>(in.nodes, in.tris, o, c, none);
+					closest_one<false>(in.nodes, in.tris, o, c, none, inner);
 					const bool tieWins = c.t == best.t && !seededHit &&
 					                     (bestInst == kInvalid || id < bestInst);
 					if (c.tri != kInvalid && (c.t < best.t || tieWins)) {
