@@ -142,9 +142,9 @@ struct ColumnStack {
 	}
 };
 
-// LDS stack entries per lane: 24 for one-ray-per-lane casts (24 KB per 256-thread block), 16 for
-// the castRay of an entered instance (16 KB). Larger splits lower the blocks a CU holds and
-// measured slower.
+// LDS stack entries per lane: 24 for one-ray-per-lane casts (24 KB per 256-thread block); 16 each
+// for an instance cast's top-tree stack and its entered instance's castRay stack (32 KB, four
+// blocks per CU). Larger splits lower the blocks a CU holds and measured slower.
 constexpr int kSimpleLds = 24;
 constexpr int kInstanceLds = 16;
 
@@ -320,12 +320,14 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 	uint8_t* __restrict__ inLight, uint32_t n, uint32_t seeded, uint32_t raysPerWave)
 {
 	__shared__ uint32_t innerLds[kInstanceLds * kBlock];
+	__shared__ uint32_t topLds[kInstanceLds * kBlock];
 	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	uint32_t next = wave * raysPerWave;  // wave-uniform
 	const uint32_t end = next < n ? (n - next < raysPerWave ? n : next + raysPerWave) : next;
 	ColumnStack<kInstanceLds> inner;
 	inner.col = innerLds + threadIdx.x;
-	uint32_t topStk[kSimpleStack];
+	ColumnStack<kInstanceLds> topStk;
+	topStk.col = topLds + threadIdx.x;
 	uint32_t ray = kInvalid;
 	dev::Ray r = dev::make_ray(make_float4(0.f, 0.f, 0.f, 0.f), make_float4(1.f, 1.f, 1.f, 0.f));
 	dev::Ray o = r;
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 			}
 			if (kAnyHit && seeded) occluded = inLight[ray] == 0;
 			tsp = numInst ? 1 : 0;
-			topStk[0] = 0u;
+			topStk.put(0, 0u);
 			pend0 = pend0End = pend1 = pend1End = 0u;
 			inInner = false;
 		}
@@ -432,15 +434,15 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 		} else if (tsp > 0) {
 			// One top-tree node: decide both children with the best t so far.
 			tsp--;
-			const NodeWords nd = load_node(top, topStk[tsp]);
+			const NodeWords nd = load_node(top, topStk.get(tsp));
 			float inL, outL, inR, outR;
 			dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
 			dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
 			const float lim = kAnyHit ? r.tMax : dev::min_first(r.tMax, best.t);
 			const bool visitL = (outL >= inL) && outL > r.tMin && (kAnyHit ? inL < lim : inL <= lim);
 			const bool visitR = (outR >= inR) && outR > r.tMin && (kAnyHit ? inR < lim : inR <= lim);
-			if (visitL && nd.w3.z == 0 && tsp < kSimpleStack) topStk[tsp++] = child_index(nd.w3.x);
-			if (visitR && nd.w3.w == 0 && tsp < kSimpleStack) topStk[tsp++] = child_index(nd.w3.y);
+			if (visitL && nd.w3.z == 0 && tsp < kSimpleStack) topStk.put(tsp++, child_index(nd.w3.x));
+			if (visitR && nd.w3.w == 0 && tsp < kSimpleStack) topStk.put(tsp++, child_index(nd.w3.y));
 			if (visitL && nd.w3.z != 0) {
 				pend0 = child_index(nd.w3.x);
 				pend0End = pend0 + uint32_t(nd.w3.z);
