@@ -372,7 +372,43 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 			pend0 = pend0End = pend1 = pend1End = 0u;
 			inInner = false;
 		}
+		// A lane may take a top-tree node, enter the instance it leaves pending and take that
+		// instance's first node in one iteration; the order of each ray's work is unchanged.
 		bool done = false;
+		if (!inInner && !occluded && pend0 >= pend0End && pend1 >= pend1End && tsp > 0) {
+			// One top-tree node: decide both children with the best t so far.
+			tsp--;
+			const NodeWords nd = load_node(top, topStk.get(tsp));
+			float inL, outL, inR, outR;
+			dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
+			dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
+			const float lim = kAnyHit ? r.tMax : dev::min_first(r.tMax, best.t);
+			const bool visitL = (outL >= inL) && outL > r.tMin && (kAnyHit ? inL < lim : inL <= lim);
+			const bool visitR = (outR >= inR) && outR > r.tMin && (kAnyHit ? inR < lim : inR <= lim);
+			if (visitL && nd.w3.z == 0 && tsp < kSimpleStack) topStk.put(tsp++, child_index(nd.w3.x));
+			if (visitR && nd.w3.w == 0 && tsp < kSimpleStack) topStk.put(tsp++, child_index(nd.w3.y));
+			if (visitL && nd.w3.z != 0) {
+				pend0 = child_index(nd.w3.x);
+				pend0End = pend0 + uint32_t(nd.w3.z);
+			}
+			if (visitR && nd.w3.w != 0) {
+				pend1 = child_index(nd.w3.y);
+				pend1End = pend1 + uint32_t(nd.w3.w);
+			}
+		}
+		if (!inInner && !occluded && (pend0 < pend0End || pend1 < pend1End)) {
+			// Enter the next instance of the visited leaves (left leaf first).
+			const uint32_t j = pend0 < pend0End ? pend0++ : pend1++;
+			const DevInstance& in = inst[j];
+			curId = in.index;
+			iNodes = in.nodes;
+			iTris = in.tris;
+			o = object_ray(in, r);
+			c = Closest{kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
+			inner.put(0, 0u);
+			isp = 1;
+			inInner = true;
+		}
 		if (inInner) {
 			// One node of the entered instance's castRay (closest_one / anyhit_one's loop body).
 			isp--;
@@ -417,43 +453,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_instances(
 					}
 				}
 			}
-		} else if (occluded) {
-			done = true;
-		} else if (pend0 < pend0End || pend1 < pend1End) {
-			// Enter the next instance of the visited leaves (left leaf first).
-			const uint32_t j = pend0 < pend0End ? pend0++ : pend1++;
-			const DevInstance& in = inst[j];
-			curId = in.index;
-			iNodes = in.nodes;
-			iTris = in.tris;
-			o = object_ray(in, r);
-			c = Closest{kInvalid, FLT_MAX, FLT_MAX, FLT_MAX};
-			inner.put(0, 0u);
-			isp = 1;
-			inInner = true;
-		} else if (tsp > 0) {
-			// One top-tree node: decide both children with the best t so far.
-			tsp--;
-			const NodeWords nd = load_node(top, topStk.get(tsp));
-			float inL, outL, inR, outR;
-			dev::slab(r, nd.w0.x, nd.w0.y, nd.w0.z, nd.w0.w, nd.w1.x, nd.w1.y, inL, outL);
-			dev::slab(r, nd.w1.z, nd.w1.w, nd.w2.x, nd.w2.y, nd.w2.z, nd.w2.w, inR, outR);
-			const float lim = kAnyHit ? r.tMax : dev::min_first(r.tMax, best.t);
-			const bool visitL = (outL >= inL) && outL > r.tMin && (kAnyHit ? inL < lim : inL <= lim);
-			const bool visitR = (outR >= inR) && outR > r.tMin && (kAnyHit ? inR < lim : inR <= lim);
-			if (visitL && nd.w3.z == 0 && tsp < kSimpleStack) topStk.put(tsp++, child_index(nd.w3.x));
-			if (visitR && nd.w3.w == 0 && tsp < kSimpleStack) topStk.put(tsp++, child_index(nd.w3.y));
-			if (visitL && nd.w3.z != 0) {
-				pend0 = child_index(nd.w3.x);
-				pend0End = pend0 + uint32_t(nd.w3.z);
-			}
-			if (visitR && nd.w3.w != 0) {
-				pend1 = child_index(nd.w3.y);
-				pend1End = pend1 + uint32_t(nd.w3.w);
-			}
-		} else {
-			done = true;
 		}
+		if (!done && !inInner)
+			done = occluded || (pend0 >= pend0End && pend1 >= pend1End && tsp == 0);
 		if (done) {
 			if (kAnyHit) {
 				inLight[ray] = occluded ? 0 : 1;
