@@ -142,11 +142,12 @@ struct ColumnStack {
 	}
 };
 
-// LDS stack entries per lane: 24 for one-ray-per-lane casts (24 KB per 256-thread block); 16 each
-// for an instance cast's top-tree stack and its entered instance's castRay stack (32 KB, four
-// blocks per CU). Larger splits lower the blocks a CU holds and measured slower.
+// LDS stack entries per lane: 24 for one-ray-per-lane casts (24 KB per 256-thread block); 12 each
+// for an instance cast's top-tree stack and its entered instance's castRay stack (24 KB, so the
+// five blocks per CU of its launch fit). Larger splits lower the blocks a CU holds and measured
+// slower.
 constexpr int kSimpleLds = 24;
-constexpr int kInstanceLds = 16;
+constexpr int kInstanceLds = 12;
 
 template <bool kCount, class Stack>
 __device__ __forceinline__ void closest_one(const float4* __restrict__ nodes,
@@ -2024,8 +2025,9 @@ hipError_t launchInstances(const PhxBVHNode* top, const DevInstance* inst, uint3
 	if (n == 0) return hipSuccess;
 	const float4* t4 = reinterpret_cast<const float4*>(top);
 	const float4* r4 = reinterpret_cast<const float4*>(rays);
-	// Rays per wave: enough waves to fill the device (about 4,096), at least one ray per lane.
-	const uint32_t perWave = std::max<uint32_t>(64u, std::min<uint32_t>(1024u, (n + 4095u) / 4096u));
+	// Rays per wave: enough waves to fill the device (about 5,120: five per SIMD, the most the
+	// kernel's registers allow), at least one ray per lane.
+	const uint32_t perWave = std::max<uint32_t>(64u, std::min<uint32_t>(1024u, (n + 5119u) / 5120u));
 	const uint32_t waves = (n + perWave - 1u) / perWave;
 	const dim3 grid((waves + kBlock / 64u - 1u) / (kBlock / 64u));
 	if (inLight)
